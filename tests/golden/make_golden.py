@@ -1,0 +1,181 @@
+"""Write golden vectors from the compiled reference CFDwavemaker.
+
+Run in the development container only (it needs the reference tree to build
+oracle/_ref/):   make -C oracle ref && python tests/golden/make_golden.py
+
+Outputs (small npz fixtures, committed):
+  points.npz  every Irregular evaluation (eta1 ... uvw2) at seeded points for
+              four sea states that exercise mtheta/tofmax/x0/swl, short-crested
+              duplicates (k_i == k_m), bandwidth/cutoff filters, individual
+              ramps, shallow and deep water;
+  pairs.npz   the reference pair cache (Irregular::rebuild_pair_cache) of those seas;
+  grid.npz    lsGridSpline fields after initialize_kinematics and three
+              wave_force_update steps (plain and ignore_at_init), plus spline
+              queries at each step;
+  abi_*.npz   a waveinput.dat and a program of C-ABI calls (ref_abi.py) with
+              the reference's answers: direct mode with ramps, s-grid mode
+              with updates, and the reference's t005 focused-wave input.
+"""
+import os
+import subprocess
+import sys
+import tempfile
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, ROOT)
+
+from oracle import KINDS, GRID_FIELDS, RefHarness, REF_DIR  # noqa: E402
+from cfdwavemaker_amd.seastate import make_sea, write_waveinput  # noqa: E402
+from tests.golden.cases import sea_cases, points_for, GRID_CASE  # noqa: E402
+
+REF_SRC = "/root/reference"
+
+
+def make_points(R):
+    out = {}
+    for name, sea in sea_cases().items():
+        R.setup(sea)
+        t, x, y, z = points_for(sea, seed=ord(name))
+        for key in ("omega", "A", "k", "phase", "theta", "ramptime", "starttime", "duration"):
+            if sea.get(key) is not None:
+                out[f"{name}__{key}"] = np.asarray(sea[key], float)
+        for key in ("depth", "g", "rho", "swl", "mtheta", "tofmax", "x0", "y0", "bandwidth", "cutoff", "order"):
+            out[f"{name}__{key}"] = np.float64(sea[key])
+        out[f"{name}__pts"] = np.stack([t, x, y, z])
+        for kind in KINDS:
+            out[f"{name}__{kind}"] = R.eval(kind, t, x, y, z)
+    np.savez_compressed(os.path.join(HERE, "points.npz"), **out)
+
+
+def make_pairs(R):
+    out = {}
+    for name, sea in sea_cases().items():
+        R.setup(sea)
+        im, fl, v = R.pairs()
+        out[f"{name}__im"], out[f"{name}__flags"], out[f"{name}__vals"] = im, fl, v
+        out[f"{name}__kmax"] = np.float64(R.kmax())
+    np.savez_compressed(os.path.join(HERE, "pairs.npz"), **out)
+
+
+def make_grid(R):
+    sea = sea_cases()["B"]
+    R.setup(sea)
+    out = {}
+    g = GRID_CASE
+    for ign in (0, 1):
+        R.grid_init(g["domain"], g["nx"], g["ny"], g["nl"], g["t0"], g["dt"], g["tan_a"],
+                    g["tan_b"], g["depth"], 0.0, (30.0, 70.0, -25.0, 25.0), 1, ign)
+        for step in range(4):
+            if step:
+                R.grid_force_update(g["t0"] + step * g["dt"] + 0.05)
+            st = R.grid_state()
+            for f in GRID_FIELDS:
+                out[f"ign{ign}__s{step}__{f}"] = R.grid_export(f)
+            rng = np.random.default_rng(100 + 10 * ign + step)
+            n = 64
+            q = np.stack([st["t0"] + rng.uniform(0, g["dt"], n), rng.uniform(-5, 125, n),
+                          rng.uniform(-25, 25, n), rng.uniform(-g["depth"] - 2, 3.0, n)])
+            out[f"ign{ign}__s{step}__q"] = q
+            out[f"ign{ign}__s{step}__qout"] = R.grid_query(*q)
+            out[f"ign{ign}__s{step}__state"] = np.array([st["tstep"], st["t0"], st["dx"], st["dy"], st["ds"]])
+    np.savez_compressed(os.path.join(HERE, "grid.npz"), **out)
+
+
+def run_abi(lib, waveinput_text, prog, extra_files=()):
+    with tempfile.TemporaryDirectory() as wd:
+        with open(os.path.join(wd, "waveinput.dat"), "w") as f:
+            f.write(waveinput_text)
+        pp = os.path.join(wd, "prog.npz")
+        op = os.path.join(wd, "out.npz")
+        np.savez(pp, **prog)
+        subprocess.run([sys.executable, os.path.join(HERE, "ref_abi.py"), lib, wd, pp, op],
+                       check=True, stdout=subprocess.DEVNULL)
+        r = np.load(op)
+        return int(r["rc"]), r["out"]
+
+
+def prog_direct(seed, n=60, depth=30.0):
+    rng = np.random.default_rng(seed)
+    ops, args = [], []
+    for q in range(n):
+        x, y = rng.uniform(-50, 300), rng.uniform(-30, 30)
+        z, t = rng.uniform(-depth - 1, 2.0), rng.uniform(-1.0, 25.0)
+        for o in (0, 1, 2, 3, 4):
+            ops.append(o)
+            args.append((x, y, z, t))
+    for o, opt in ((7, 1), (7, 2), (8, 1), (8, 2), (9, 1), (9, 2), (10, 0)):
+        ops.append(o)
+        args.append((opt, 0, 0, 0))
+    return dict(op=np.array(ops, np.int32), args=np.array(args, float))
+
+
+def prog_grid(seed, lsg, depth, nsteps=3, nq=40):
+    rng = np.random.default_rng(seed)
+    ops, args = [], []
+    b = lsg["bounds"]
+    for step in range(nsteps + 1):
+        t0 = lsg["t0"] + step * lsg["dt"]
+        if step:
+            ops.append(6)
+            args.append((0, 0, 0, t0 + 0.01))
+        for q in range(nq):
+            x, y = rng.uniform(b[0] - 5, b[1] + 5), rng.uniform(b[2] - 5, b[3] + 5)
+            z, t = rng.uniform(-depth - 1, 1.0), t0 + rng.uniform(0.0, lsg["dt"])
+            for o in (0, 1, 2, 3, 5):
+                ops.append(o)
+                args.append((x, y, z, t))
+    return dict(op=np.array(ops, np.int32), args=np.array(args, float))
+
+
+def make_abi():
+    lib = os.path.join(REF_DIR, "libCFDwavemaker_ref.so")
+    # 1. direct mode (wavetype 1) with ramps, filters, swl, reference point
+    sea = make_sea(24, 3.0, 10.0, 30.0, seed=21)
+    sea.swl, sea.mtheta, sea.tofmax, sea.x0, sea.y0 = 0.25, 15.0, 5.0, 10.0, 2.0
+    with tempfile.NamedTemporaryFile("r", suffix=".dat") as tf:
+        write_waveinput(tf.name, sea, ramps={"time_rampup": (0.0, 4.0), "x_rampdown": (250.0, 280.0)},
+                        bandwidth="0.6", cutoff=2.2, amplify=1.3)
+        text = open(tf.name).read()
+    prog = prog_direct(21)
+    rc, out = run_abi(lib, text, prog)
+    np.savez_compressed(os.path.join(HERE, "abi_direct.npz"), waveinput=text, rc=rc, out=out, **prog)
+    # 2. s-grid mode (wavetype 5): short-crested, three updates
+    sea = make_sea(8, 4.0, 11.0, 40.0, seed=22, ndir=4)
+    lsg = dict(bounds=(0.0, 150.0, -30.0, 30.0), nx=9, ny=5, nl=6, t0=0.0, dt=0.5, tan_a=0.7, tan_b=1.5)
+    with tempfile.NamedTemporaryFile("r", suffix=".dat") as tf:
+        write_waveinput(tf.name, sea, lsgrid=lsg, extrap=2)
+        text = open(tf.name).read()
+    prog = prog_grid(22, lsg, sea.depth)
+    rc, out = run_abi(lib, text, prog)
+    np.savez_compressed(os.path.join(HERE, "abi_grid.npz"), waveinput=text, rc=rc, out=out, **prog)
+    # 3. the reference's own t005 focused-wave input (data file of its test
+    #    suite), sampled where its targetwave.dat was written (x=10, y=0)
+    t005 = os.path.join(REF_SRC, "tests", "t005_focused_wave")
+    text = open(os.path.join(t005, "waveinput.dat")).read()
+    tw = np.loadtxt(os.path.join(t005, "targetwave.dat"))
+    sel = np.arange(0, tw.shape[0], 16)
+    ops = np.zeros(sel.size, np.int32)
+    args = np.zeros((sel.size, 4))
+    args[:, 0] = 10.0
+    args[:, 3] = tw[sel, 0]
+    prog = dict(op=ops, args=args)
+    rc, out = run_abi(lib, text, prog)
+    np.savez_compressed(os.path.join(HERE, "abi_t005.npz"), waveinput=text, rc=rc, out=out,
+                        targetwave=tw[sel], **prog)
+
+
+def main():
+    subprocess.run(["make", "-s", "-C", os.path.join(ROOT, "oracle"), "ref"], check=True)
+    R = RefHarness()
+    make_points(R)
+    make_pairs(R)
+    make_grid(R)
+    make_abi()
+    print("golden fixtures written to", HERE)
+
+
+if __name__ == "__main__":
+    main()
