@@ -1,0 +1,211 @@
+"""Benchmark: second-order kinematics points/s at N=1024 components on one MI355X.
+
+Workload (the N=1024 short-crested configuration): 64 frequencies x 16
+directions (cos^2s, s=4) = 1024 components, Hs 8 m, Tp 14 s, depth 100 m;
+s-grid 256 x 256 columns x 16 sigma levels x 4 time levels over 1 km x 1 km,
+then 1e7 cubic-spline queries at uniform random points in the 1 km x 1 km x
+(-100 m, eta) box at t = 0.25 s.
+
+One step = one full initialize_kinematics of the s-grid (column pass: eta1 +
+eta2 + Taylor gradients; level pass: first + second order + Taylor at every
+level; FD gradients) plus the 1e7 queries, all on the GPU with inputs resident
+in HBM.  value = grid points (4 x 256 x 256 x 16) per second of step time.
+
+usage: python bench.py [--gpus 1] [--steps 3] [--warmup 1] [--precision fp32|fp64]
+"""
+import argparse
+import json
+import math
+import os
+import subprocess
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+PEAK = {"fp32": 157.3e12, "fp64": 78.6e12}   # MI355X vector peaks (FLOP/s)
+NX = NY = 256
+NL = 16
+DOMAIN = (0.0, 1000.0, 0.0, 1000.0)
+T0, DT, TAN_A, TAN_B, DEPTH, HS = 0.0, 0.5, 0.7, 1.5, 100.0, 8.0
+
+
+def cpu_worker(ncols: int, out_path: str) -> None:
+    """Times the reference's OpenMP s-grid fill (initialize_kinematics) on an
+    ncols x ncols subset of the same grid (same spacing), or the oracle's
+    column body when the compiled reference is not present."""
+    from cfdwavemaker_amd.seastate import config_sea
+    s = config_sea(3).as_dict()
+    dx = (DOMAIN[1] - DOMAIN[0]) / (NX - 1)
+    dom = (0.0, dx * (ncols - 1), 0.0, dx * (ncols - 1))
+    res = {}
+    try:
+        from oracle import RefHarness
+        R = RefHarness()
+        R.setup(s)
+        sec = R.grid_init(dom, ncols, ncols, NL, T0, DT, TAN_A, TAN_B, DEPTH)
+        res = dict(kind="reference", seconds=sec, points=4 * ncols * ncols * NL)
+    except FileNotFoundError:
+        from oracle import Oracle
+        O = Oracle().sea(s)
+        xs, ys = np.meshgrid(np.arange(ncols) * dx, np.arange(ncols) * dx, indexing="ij")
+        tot = 0.0
+        for t in (T0 - DT, T0, T0 + DT, T0 + 2 * DT):
+            sec, *_ = O.column_fill(xs.ravel(), ys.ravel(), t, NL, TAN_A, TAN_B, DEPTH)
+            tot += sec
+        res = dict(kind="port", seconds=tot, points=4 * ncols * ncols * NL)
+    with open(out_path, "w") as f:
+        json.dump(res, f)
+
+
+def cpu_baseline(ncols: int, threads: int):
+    import tempfile
+    env = dict(os.environ, OMP_NUM_THREADS=str(threads), OMP_PROC_BIND="close", OMP_PLACES="cores")
+    with tempfile.TemporaryDirectory() as td:
+        out = os.path.join(td, "cpu.json")
+        r = subprocess.run([sys.executable, os.path.abspath(__file__), "--cpu-worker", str(ncols), out],
+                           cwd=td, env=env, stdout=subprocess.DEVNULL, stderr=subprocess.PIPE, text=True,
+                           timeout=900)
+        if r.returncode != 0:
+            raise RuntimeError(r.stderr[-2000:])
+        with open(out) as f:
+            return json.load(f)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--precision", default="fp32", choices=["fp32", "fp64"])
+    ap.add_argument("--queries", type=int, default=10_000_000)
+    ap.add_argument("--cpu-cols", type=int, default=4)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-worker", nargs=2, default=None)
+    a = ap.parse_args()
+    if a.cpu_worker:
+        cpu_worker(int(a.cpu_worker[0]), a.cpu_worker[1])
+        return
+    if a.gpus != 1:
+        print(f"bench.py runs on one GPU: the s-grid fill shards by columns across GPUs "
+              f"(see DESIGN.md), but only N=1 is measured here; got --gpus {a.gpus}", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    from cfdwavemaker_amd import engine
+    from cfdwavemaker_amd.seastate import config_sea
+
+    prec = engine.FP32 if a.precision == "fp32" else engine.FP64
+    s = config_sea(3).as_dict()
+    print(f"[bench] building N={len(s['omega'])} sea ({a.precision})", file=sys.stderr, flush=True)
+    sea = engine.Sea(s, precision=prec, with_direct=False)
+    pair_terms, diag_terms, _ = sea.terms()
+    T = pair_terms + diag_terms
+    t_build = time.perf_counter()
+    grid = sea.grid(DOMAIN, NX, NY, NL, T0, DT, TAN_A, TAN_B, DEPTH)
+    print(f"[bench] first fill {time.perf_counter() - t_build:.2f} s", file=sys.stderr, flush=True)
+
+    # query points: uniform in the box, z uniform between the seabed and the local surface
+    dev = torch.device("cuda:0")
+    g = torch.Generator(device="cpu").manual_seed(3)
+    nq = a.queries
+    qx = (torch.rand(nq, generator=g, dtype=torch.float32) * 1000.0).to(dev)
+    qy = (torch.rand(nq, generator=g, dtype=torch.float32) * 1000.0).to(dev)
+    qz = torch.zeros(nq, dtype=torch.float32, device=dev)
+    out = [torch.empty(nq, dtype=torch.float32, device=dev) for _ in range(4)]
+    torch.cuda.synchronize()
+    tq = 0.25
+
+    def query():
+        grid.query_device_f32(nq, tq, qx.data_ptr(), qy.data_ptr(), qz.data_ptr(),
+                              *[o.data_ptr() for o in out])
+
+    query()
+    engine.sync()
+    u = torch.rand(nq, generator=g, dtype=torch.float32).to(dev)
+    qz.copy_(-DEPTH + u * (out[0] + DEPTH))
+    torch.cuda.synchronize()
+
+    def step():
+        grid.refill()
+        query()
+        engine.sync()
+
+    for _ in range(a.warmup):
+        step()
+    engine.prof_enable(True)
+    engine.prof_read(reset=True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for k in range(a.steps):
+        step()
+        print(f"[bench] step {k + 1}/{a.steps} done", file=sys.stderr, flush=True)
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t0
+    prof = engine.prof_read(reset=True)
+    engine.prof_enable(False)
+
+    points = 4 * NX * NY * NL
+    value = points * a.steps / el
+    lev_ms, lev_n = prof.get("k_levels_grid", (float("nan"), 1))
+    lev_avg = lev_ms / max(lev_n, 1) * 1e-3
+    flop_per_launch = 16.0 * T * points
+    achieved = flop_per_launch / lev_avg
+    q_ms, q_n = prof.get("k_query", (float("nan"), 1))
+
+    # accuracy gate: FP32 (or FP64) grid vs an FP64 grid on an 8 x 8 block of the same columns
+    dx = (DOMAIN[1] - DOMAIN[0]) / (NX - 1)
+    sub = (0.0, 7 * dx, 0.0, 7 * dx)
+    g64 = engine.Sea(s, precision=engine.FP64, with_direct=False).grid(sub, 8, 8, NL, T0, DT, TAN_A, TAN_B, DEPTH)
+    gxx = sea.grid(sub, 8, 8, NL, T0, DT, TAN_A, TAN_B, DEPTH)
+    eta_err = float(np.max(np.abs(gxx.export("ETA") - g64.export("ETA"))))
+    uref = g64.export("UX")
+    u_err = float(np.max(np.abs(gxx.export("UX") - uref)) / np.max(np.abs(uref)))
+
+    cpu = None
+    if not a.no_cpu:
+        threads = min(16, os.cpu_count() or 1)
+        print(f"[bench] cpu baseline ({a.cpu_cols}x{a.cpu_cols} columns, {threads} threads)", file=sys.stderr, flush=True)
+        try:
+            c = cpu_baseline(a.cpu_cols, threads)
+            cpu = {"value": c["points"] / c["seconds"], "unit": "points/s", "cores": threads, "kind": c["kind"],
+                   "sample": f"s-grid fill of {a.cpu_cols}x{a.cpu_cols} columns x {NL} levels x 4 time levels "
+                             f"({c['points']} points, same N=1024 sea and spacing), {c['seconds']:.2f} s"}
+        except Exception as e:  # reported, not fatal
+            cpu = {"value": None, "error": str(e)[-300:]}
+
+    line = {
+        "metric": "2nd-order kinematics points/s (N=1024), exact GPU",
+        "value": value,
+        "unit": "points/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": el / a.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32" if a.precision == "fp32" else "f64",
+        "data": "synthetic (JONSWAP + cos^2s spreading, seeded)",
+        "config": {"workload": "N=1024 short-crested sea, s-grid 256x256 columns x 16 levels x 4 time "
+                               "levels + 1e7 spline queries", "components": len(s["omega"]),
+                   "pair_terms": pair_terms, "diag_terms": diag_terms,
+                   "points_per_step": points, "queries_per_step": nq, "precision": a.precision},
+        "roofline": {"bound": "valu", "kernel": "k_levels_grid", "achieved": achieved / 1e12,
+                     "peak": PEAK[a.precision] / 1e12, "unit": "TFLOP/s",
+                     "frac": achieved / PEAK[a.precision], "traffic": None,
+                     "avg_launch_ms": lev_avg * 1e3, "flop_per_launch": flop_per_launch},
+        "kernels_ms": {k: v[0] / max(v[1], 1) for k, v in prof.items()},
+        "query_rate": nq / (q_ms / max(q_n, 1) * 1e-3) if q_ms == q_ms else None,
+        "accuracy": {"eta_max_err_m": eta_err, "eta_err_over_hs": eta_err / HS, "ux_scaled_err": u_err,
+                     "gate": "eta error <= 1e-3 Hs vs FP64", "pass": eta_err <= 1e-3 * HS},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

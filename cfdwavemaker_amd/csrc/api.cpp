@@ -1,0 +1,798 @@
+// C ABI of libcfdwm: the reference's wave_* entry points (process-global wave
+// field configured from waveinput.dat) and the batched / object entries.
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <fstream>
+#include <iostream>
+#include <memory>
+#include <numeric>
+#include <sstream>
+#include <string>
+#include <unistd.h>
+#include <vector>
+
+#include "../../include/cfdwm.h"
+#include "engine.hpp"
+#include "sea_tables.hpp"
+
+struct cfdwm_sea {
+    std::unique_ptr<cfdwm::Sea> s;
+    bool with_direct = false;
+};
+struct cfdwm_grid {
+    std::unique_ptr<cfdwm::Grid> g;
+};
+
+namespace cfdwm {
+namespace {
+
+// ------------------------------------------------------------------ Ramp
+// Ramp::ramp (src/Utils.cpp:26-68): smallest of the active linear ramps.
+struct RampCfg {
+    bool init = false;
+    bool on[6] = {false, false, false, false, false, false};
+    double se[12] = {0};
+    static double r1d(double x, double xs, double xe, bool inv) {
+        const double v = inv ? 1. - ((x - xs) / (xe - xs)) : ((x - xs) / (xe - xs));
+        return std::max(std::min(v, 1.), 0.);
+    }
+    double operator()(double t, double x, double y) const {
+        if (!init) return 1.0;
+        double r[6] = {1.0, 1.0, 1.0, 1.0, 1.0, 1.0};
+        const double a[6] = {t, t, x, x, y, y};
+        for (int q = 0; q < 6; ++q)
+            if (on[q]) r[q] = r1d(a[q], se[2 * q], se[2 * q + 1], q % 2 == 1);
+        return *std::min_element(r, r + 6);
+    }
+};
+
+// ---------------------------------------------------- process-global state
+// Mirrors the singletons of src/CFDwavemaker.cpp:69-163.
+struct Global {
+    bool initialized = false;
+    int wavetype = 0;                 // 1 direct irregular, 5 s-grid irregular, others unsupported
+    double depth = 0, x_pos = 0, y_pos = 0, tofmax = 0, mtheta = 0, swl = 0;
+    double rho = 1., gravity = 9.81, ampl = 1.;
+    bool bw_auto = false, property_read = false;
+    // irregular
+    SeaInput sea;
+    int nfreq = 0, ndir = 1, extrapolation = 0, order = 2;
+    bool normalize = false, individual_ramp = false;
+    int vertical_theory = 0;
+    // lsgrid
+    bool has_grid = false;
+    double lsg_domain[4] = {0, 0, 0, 0};
+    int lsg_nx = 5, lsg_ny = 0, lsg_nl = 4;
+    double lsg_t0 = 0., lsg_dt = 0.1, lsg_tan_a = 0., lsg_tan_b = 0.;
+    double lsg_ignore[4] = {1.E12, -1.E12, 1.E12, -1.E12};
+    bool lsg_ignore_domain = false, lsg_ignore_at_init = false, lsg_init_only = false;
+    RampCfg ramp;
+    int precision = kFP64;
+    cfdwm_sea* csea = nullptr;
+    cfdwm_grid* cgrid = nullptr;
+};
+Global G;
+
+void trim(std::string& s) {
+    auto ns = [](int ch) { return !std::isspace(ch); };
+    s.erase(s.begin(), std::find_if(s.begin(), s.end(), ns));
+    s.erase(std::find_if(s.rbegin(), s.rend(), ns).base(), s.end());
+}
+bool starts(const std::string& s, const char* key) { return s.compare(0, std::strlen(key), key) == 0; }
+
+[[noreturn]] void input_error(const std::string& msg) {
+    std::cerr << "INPUTFILE ERROR: " << msg << std::endl;
+    std::exit(-1);
+}
+
+template <typename T>
+void read_after_key(const std::string& line, T& v) {
+    std::istringstream b(line);
+    std::string key;
+    b >> key >> v;
+}
+
+// waveinput.dat sections for irregular waves and the s-grid
+// (src/CFDwavemaker.cpp:232-1203).  Line handling follows the reference:
+// a section loop reads trimmed lines until a line starting with '[', which is
+// then examined by the following section tests without reading a new line.
+void parse(const std::string& res) {
+    std::istringstream f(res);
+    std::string lineA;
+    bool skip = false;
+    while (!f.eof()) {
+        if (skip) skip = false;
+        else { std::getline(f, lineA); trim(lineA); }
+        if (lineA == "[wave type]") {
+            std::getline(f, lineA);
+            trim(lineA);
+            if (lineA == "irregular" || lineA == "1") G.wavetype = 1;
+            else if (lineA == "piston") G.wavetype = 11;
+            else if (lineA == "regular") G.wavetype = 21;
+            else if (lineA == "swd") G.wavetype = 31;
+            else if (lineA == "vtk") G.wavetype = 41;
+            else if (lineA == "slope") G.wavetype = 51;
+            else input_error("Unknown wave type specified. Valid alternatives are: irregular regular piston swd vtk slope");
+            if (G.wavetype != 1)
+                std::cerr << "cfdwm: wave type '" << lineA << "' is not supported in this build; "
+                          << "kinematics calls return 0.0" << std::endl;
+        }
+        if (lineA == "[general input data]") {
+            while (!f.eof()) {
+                std::getline(f, lineA);
+                trim(lineA);
+                if (starts(lineA, "depth")) read_after_key(lineA, G.depth);
+                if (starts(lineA, "mtheta")) read_after_key(lineA, G.mtheta);
+                if (starts(lineA, "normalize")) read_after_key(lineA, G.normalize);
+                if (starts(lineA, "amplify")) read_after_key(lineA, G.ampl);
+                if (starts(lineA, "swl")) read_after_key(lineA, G.swl);
+                if (starts(lineA, "gravity")) read_after_key(lineA, G.gravity);
+                if (starts(lineA, "rho")) read_after_key(lineA, G.rho);
+                if (starts(lineA, "[")) { skip = true; break; }
+            }
+        }
+        if (lineA == "[wave reference point]") {
+            while (!f.eof()) {
+                std::getline(f, lineA);
+                trim(lineA);
+                if (starts(lineA, "time")) read_after_key(lineA, G.tofmax);
+                if (starts(lineA, "x")) read_after_key(lineA, G.x_pos);
+                if (starts(lineA, "y")) read_after_key(lineA, G.y_pos);
+                if (starts(lineA, "[")) { skip = true; break; }
+            }
+        }
+        if (lineA == "[ramps]") {
+            G.ramp.init = true;
+            static const char* keys[6] = {"time_rampup", "time_rampdown", "x_rampup", "x_rampdown", "y_rampup", "y_rampdown"};
+            while (!f.eof()) {
+                std::getline(f, lineA);
+                trim(lineA);
+                for (int q = 0; q < 6; ++q) {
+                    if (starts(lineA, keys[q])) {
+                        std::istringstream b(lineA);
+                        std::string key;
+                        b >> key >> G.ramp.se[2 * q] >> G.ramp.se[2 * q + 1];
+                        G.ramp.on[q] = true;
+                    }
+                }
+                if (starts(lineA, "[")) { skip = true; break; }
+            }
+        }
+        if (lineA == "[wave input irregular]") {
+            if (G.wavetype != 1) input_error("irregular wave components does not match the specified wave type. Check inputfile");
+            while (!f.eof()) {
+                std::getline(f, lineA);
+                trim(lineA);
+                if (starts(lineA, "kinematics_surf_extrap")) {
+                    read_after_key(lineA, G.extrapolation);
+                    if (G.extrapolation == 2) G.order = 2;
+                    else if (G.extrapolation == 1 || G.extrapolation == 0) G.order = 0;
+                    else { std::cout << "Unknown extrapolation method. try a value between 0 and 2." << std::endl; std::exit(-1); }
+                }
+                if (starts(lineA, "cutoff")) read_after_key(lineA, G.sea.cutoff);
+                if (starts(lineA, "bandwidth")) {
+                    std::string v;
+                    read_after_key(lineA, v);
+                    if (!v.compare(0, 3, "off")) {}
+                    else if (!v.compare(0, 4, "auto")) G.bw_auto = true;
+                    else G.sea.bandwidth = std::atof(v.c_str());
+                }
+                if (starts(lineA, "individual_ramp")) {
+                    std::string v;
+                    read_after_key(lineA, v);
+                    if (!v.compare(0, 4, "true")) G.individual_ramp = true;
+                }
+                if (starts(lineA, "vertical_theory")) {
+                    std::string v;
+                    read_after_key(lineA, v);
+                    G.vertical_theory = !v.compare(0, 4, "smit") ? 1 : 0;
+                    if (G.vertical_theory == 1)
+                        std::cerr << "cfdwm: vertical_theory 'smit' (experimental in the reference) is not "
+                                     "supported; Sharma & Dean kinematics are used" << std::endl;
+                }
+                if (starts(lineA, "nfreq")) break;
+            }
+            if (starts(lineA, "nfreq")) read_after_key(lineA, G.nfreq);
+            else input_error("parameter nfreq missing or not specified correctly");
+            std::getline(f, lineA);
+            trim(lineA);
+            if (starts(lineA, "ndir")) read_after_key(lineA, G.ndir);
+            else input_error("parameter ndir missing or not specified correctly");
+            std::vector<double> om, A, k, ph, th, rt, ts, du;
+            if (G.ndir == 0) {
+                // one direction per frequency row (:596-705), optional ramp triplet
+                G.ndir = 1;
+                for (int i = 0; i < G.nfreq; ++i) {
+                    std::getline(f, lineA);
+                    if (starts(lineA, "["))
+                        input_error("Not enough frequency components specified. please check to make sure that the number of lines matches the specified number of frequency components");
+                    std::istringstream b(lineA);
+                    double v[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+                    const int nv = G.individual_ramp ? 8 : 5;
+                    for (int q = 0; q < nv; ++q) b >> v[q];
+                    om.push_back(v[0]); A.push_back(v[1]); k.push_back(v[2]); ph.push_back(v[3]); th.push_back(v[4]);
+                    if (G.individual_ramp) { rt.push_back(v[5]); ts.push_back(v[6]); du.push_back(v[7]); }
+                    else { rt.push_back(0.); du.push_back(kHuge); ts.push_back(-kSmall); }
+                }
+            } else {
+                // frequency rows then direction rows, expanded frequency-major (:706-797)
+                std::vector<double> fo(G.nfreq), fa(G.nfreq), fk(G.nfreq), fp(G.nfreq);
+                for (int i = 0; i < G.nfreq; ++i) {
+                    std::getline(f, lineA);
+                    std::istringstream b(lineA);
+                    b >> fo[i] >> fa[i] >> fk[i] >> fp[i];
+                }
+                std::vector<double> dt_(G.ndir), dd(G.ndir);
+                for (int j = 0; j < G.ndir; ++j) {
+                    std::getline(f, lineA);
+                    if (starts(lineA, "[")) input_error("too few directional components..");
+                    std::istringstream b(lineA);
+                    b >> dt_[j] >> dd[j];
+                }
+                for (int i = 0; i < G.nfreq; ++i)
+                    for (int j = 0; j < G.ndir; ++j) {
+                        om.push_back(fo[i]); k.push_back(fk[i]); A.push_back(fa[i] * dd[j]);
+                        ph.push_back(fp[i]); th.push_back(dt_[j]);
+                        rt.push_back(0.); du.push_back(kHuge); ts.push_back(-kSmall);
+                    }
+            }
+            // stable sort by omega (:663-673)
+            std::vector<size_t> idx(om.size());
+            std::iota(idx.begin(), idx.end(), 0);
+            std::stable_sort(idx.begin(), idx.end(), [&](size_t a, size_t b) { return om[a] < om[b]; });
+            SeaInput& S = G.sea;
+            for (size_t q : idx) {
+                S.omega.push_back(om[q]); S.A.push_back(A[q]); S.k.push_back(k[q]);
+                S.phase.push_back(ph[q]); S.theta.push_back(th[q]);
+                S.ramptime.push_back(rt[q]); S.starttime.push_back(ts[q]); S.duration.push_back(du[q]);
+            }
+            G.property_read = true;
+        }
+        if (lineA == "[lsgrid]") {
+            while (!f.eof()) {
+                std::getline(f, lineA);
+                trim(lineA);
+                if (starts(lineA, "bounds")) {
+                    std::istringstream b(lineA);
+                    std::string key;
+                    b >> key >> G.lsg_domain[0] >> G.lsg_domain[1] >> G.lsg_domain[2] >> G.lsg_domain[3];
+                }
+                if (starts(lineA, "nx")) read_after_key(lineA, G.lsg_nx);
+                if (starts(lineA, "ny")) read_after_key(lineA, G.lsg_ny);
+                if (starts(lineA, "nl")) read_after_key(lineA, G.lsg_nl);
+                if (starts(lineA, "t0")) read_after_key(lineA, G.lsg_t0);
+                if (starts(lineA, "dt")) read_after_key(lineA, G.lsg_dt);
+                if (starts(lineA, "stretch_params")) {
+                    std::istringstream b(lineA);
+                    std::string key;
+                    b >> key >> G.lsg_tan_a >> G.lsg_tan_b;
+                }
+                if (starts(lineA, "ignore_subdomain")) {
+                    std::istringstream b(lineA);
+                    std::string key;
+                    b >> key >> G.lsg_ignore[0] >> G.lsg_ignore[1] >> G.lsg_ignore[2] >> G.lsg_ignore[3];
+                    G.lsg_ignore_domain = true;
+                }
+                if (starts(lineA, "ignore_at_init")) read_after_key(lineA, G.lsg_ignore_at_init);
+                if (starts(lineA, "init_only")) read_after_key(lineA, G.lsg_init_only);
+                if (starts(lineA, "[")) { skip = true; break; }
+            }
+            if (G.wavetype == 1) G.wavetype = 5;
+            else if (G.wavetype == 31) G.wavetype = 34;
+            else input_error("LS grid may currently only be used with irregular second order wave theory or swd");
+            G.has_grid = true;
+        }
+    }
+}
+
+// Irregular::dumpSpectralComponents (src/Irregular.cpp:1456-1468)
+void dump_spectral_components() {
+    FILE* fp = std::fopen("./spectral_components.dat", "w");
+    if (!fp) return;
+    std::fprintf(fp, "# spectral wave data output\n# [irregular wave components]\n# nfreq %d\n# ndir 0\n", G.nfreq);
+    std::fprintf(fp, "# omega [rad/s]    a[m]       k[rad/m]        phi[rad]    theta[rad]\n");
+    const SeaInput& S = G.sea;
+    for (int i = 0; i < G.nfreq && i < (int)S.omega.size(); ++i)
+        std::fprintf(fp, "%12.5E  %12.5E  %12.5E  %12.5E  %12.5E\n", S.omega[i], S.A[i], S.k[i], S.phase[i], S.theta[i]);
+    std::fclose(fp);
+}
+
+int env_precision() {
+    const char* e = std::getenv("CFDWM_PRECISION");
+    return (e && (!std::strcmp(e, "fp32") || !std::strcmp(e, "FP32"))) ? kFP32 : kFP64;
+}
+
+[[noreturn]] void gpu_fatal(const char* where) {
+    std::cerr << "cfdwm: " << where << " failed: " << last_error() << std::endl;
+    std::abort();
+}
+
+// post-processing of process_inputdata (src/CFDwavemaker.cpp:1334-1437)
+int setup_after_parse() {
+    if (G.wavetype == 5 && G.order != 2)
+        input_error("LSgrid interpolation uses strictly second order theory. Set kinematics_surf_extrap to 2 under [wave input irregular], or remove [lsgrid].");
+    if (G.wavetype != 1 && G.wavetype != 5) return 0;
+    if (!G.property_read) input_error("Irregular wave selected, but no wave components/wave properties found in input file.");
+    SeaInput& S = G.sea;
+    S.depth = G.depth; S.mtheta = G.mtheta; S.tofmax = G.tofmax; S.x0 = G.x_pos; S.y0 = G.y_pos;
+    S.swl = G.swl; S.g = G.gravity; S.rho = G.rho; S.order = G.order;
+    if (G.bw_auto) {                                                    // bandwidth_estimator
+        if (G.ndir > 1) input_error("Automatic calculation of bandwidth parameter only supported for 2D wave spectra");
+        double m[3];
+        spectral_moments(S, m);
+        S.bandwidth = 0.7 * (m[1] / m[0]);
+    }
+    const double sumA = std::accumulate(S.A.begin(), S.A.end(), 0.);   // normalize_data
+    for (size_t i = 0; i < S.A.size(); ++i) {
+        if (G.normalize) { S.A[i] = G.ampl * S.A[i] / sumA; S.phase[i] = 0.; }
+        else S.A[i] = G.ampl * S.A[i];
+    }
+    dump_spectral_components();
+    {
+        double m[3];
+        spectral_moments(S, m);
+        std::cout << "cfdwm: " << S.omega.size() << " components, Hs " << 4 * std::sqrt(m[0])
+                  << " m, Tm " << mean_period(S, 1) << " s, Tz " << mean_period(S, 2) << " s" << std::endl;
+    }
+    G.precision = env_precision();
+    cfdwm_sea_params p{};
+    p.depth = S.depth; p.g = S.g; p.rho = S.rho; p.swl = S.swl; p.mtheta = S.mtheta;
+    p.tofmax = S.tofmax; p.x0 = S.x0; p.y0 = S.y0; p.bandwidth = S.bandwidth; p.cutoff = S.cutoff;
+    p.ampl = 1.0; p.order = S.order; p.normalize = 0; p.bw_auto = 0; p.precision = G.precision;
+    p.with_direct = G.wavetype == 1 ? 1 : 0;
+    const int n = (int)S.omega.size();
+    int rc = cfdwm_sea_create(&G.csea, n, S.omega.data(), S.A.data(), S.k.data(), S.phase.data(), S.theta.data(),
+                              S.ramptime.data(), S.starttime.data(), S.duration.data(), &p);
+    if (rc) gpu_fatal("sea set-up");
+    if (G.wavetype == 5) {
+        if (G.lsg_ny < 1 || G.lsg_nx < 1 || G.lsg_nl < 1)
+            input_error("lsgrid needs nx, ny and nl of at least 1");
+        cfdwm_grid_desc d{};
+        std::memcpy(d.domain, G.lsg_domain, sizeof(d.domain));
+        d.nx = G.lsg_nx; d.ny = G.lsg_ny; d.nl = G.lsg_nl; d.t0 = G.lsg_t0; d.dt = G.lsg_dt;
+        d.tan_a = G.lsg_tan_a; d.tan_b = G.lsg_tan_b; d.water_depth = G.depth; d.swl = G.swl;
+        std::memcpy(d.domain_ignore, G.lsg_ignore, sizeof(d.domain_ignore));
+        d.ignore_at_init = G.lsg_ignore_at_init ? 1 : 0;
+        rc = cfdwm_grid_create(&G.cgrid, G.csea, &d);
+        if (rc) gpu_fatal("s-grid initialisation");
+    }
+    return 0;
+}
+
+// ---------------------------------------------------- direct point helpers
+uint32_t order_flags(int order) {
+    if (order == 2) return EV_LIN | EV_SEC | EV_DIRECT | EV_TAYLOR | EV_ZCLAMP;
+    if (order == 1) return EV_LIN | EV_SEC | EV_DIRECT | EV_ZCLAMP;
+    return EV_LIN | EV_ZSHIFT;
+}
+
+// One direct-mode point: eta, u, v, w (and p) of Irregular::eta/u/v/w/dp.
+void direct_point(double x, double y, double z, double t, double* eta, double* u, double* v, double* w, double* p) {
+    Sea& s = *G.csea->s;
+    double e1 = 0, e2 = 0;
+    const bool we = eta != nullptr;
+    uint32_t fl = order_flags(G.order) | (p ? EV_P : 0u);
+    int rc = s.eval_points(1, &x, &y, &z, &t, fl, G.order == 2, we ? &e1 : nullptr, we && G.order == 2 ? &e2 : nullptr,
+                           nullptr, u, v, w, p, false);
+    if (rc) gpu_fatal("point evaluation");
+    if (eta) *eta = G.swl + e1 + (G.order == 2 ? e2 : 0.0);
+}
+
+double grid_point(double x, double y, double z, double t, int comp) {
+    double o4[4];
+    if (G.cgrid->g->query(1, &t, &x, &y, &z, o4, nullptr, nullptr)) gpu_fatal("s-grid query");
+    return o4[comp];
+}
+
+double velo(double x, double y, double z, double t, int comp) {
+    if (!G.initialized) return 0.0;
+    z = std::max(-G.depth, z);                       // CFDwavemaker.cpp:1547
+    if (G.wavetype == 1) {
+        double r = 0;
+        direct_point(x, y, z, t, nullptr, comp == 1 ? &r : nullptr, comp == 2 ? &r : nullptr, comp == 3 ? &r : nullptr, nullptr);
+        return G.ramp(t, x, y) * r;
+    }
+    if (G.wavetype == 5) return grid_point(x, y, z, t, comp) * G.ramp(t, x, y);
+    return 0.0;
+}
+
+}  // namespace
+}  // namespace cfdwm
+
+using namespace cfdwm;
+
+// =================================================================== Part 1
+extern "C" {
+
+int CFDwavemaker_is_initialized(void) { return G.initialized ? 1 : 0; }
+
+int wave_Initialize(void) {
+    std::cout << "CFDWAVEMAKER (cfdwm, MI355X) " << std::endl;
+    const char* paths[4] = {"./waveinput.dat", "./constant/waveinput.dat", "../input_files/waveinput.dat", "../waveinput.dat"};
+    std::ifstream fid;
+    for (const char* p : paths) {
+        fid.open(p);
+        if (!fid.fail()) break;
+        fid.clear();
+    }
+    if (!fid.is_open() || fid.fail()) {
+        std::cerr << "Could not open file (is it really there?) " << std::endl;
+        return -1;
+    }
+    std::string res, line;
+    while (fid.good()) {                              // strip '#' comments, drop empty lines
+        std::getline(fid, line);
+        line.erase(std::find(line.begin(), line.end(), '#'), line.end());
+        if (line.length() > 0) res += line + "\n";
+    }
+    fid.close();
+    std::istringstream f(res);
+    std::getline(f, line);
+    trim(line);
+    if (line.substr(0, 2) != "@v") {
+        std::cout << "Please specify the input file version on the first line of the input file (hint: @v310)" << std::endl;
+        std::exit(0);
+    }
+    if (std::stoi(line.substr(2, 3)) < 300) {
+        std::cout << "your waveinput.dat file is outdated and does not match the current version of CFDwavemaker." << std::endl;
+        std::exit(0);
+    }
+    if (G.csea || G.cgrid) wave_Cleanup();
+    G = Global();
+    parse(res);
+    setup_after_parse();
+    G.initialized = true;
+    return 0;
+}
+
+int wave_Cleanup(void) {
+    if (G.cgrid) { cfdwm_grid_destroy(G.cgrid); G.cgrid = nullptr; }
+    if (G.csea) { cfdwm_sea_destroy(G.csea); G.csea = nullptr; }
+    return 0;
+}
+
+double wave_VeloX(double x, double y, double z, double t) { return velo(x, y, z, t, 1); }
+double wave_VeloY(double x, double y, double z, double t) { return velo(x, y, z, t, 2); }
+double wave_VeloZ(double x, double y, double z, double t) { return velo(x, y, z, t, 3); }
+
+double wave_DynPres(double x, double y, double z, double t) {
+    if (!G.initialized || G.wavetype != 1) return 0.0;
+    z = std::max(-G.depth, z);
+    double p = 0;
+    direct_point(x, y, z, t, nullptr, nullptr, nullptr, nullptr, &p);
+    return G.ramp(t, x, y) * p;
+}
+
+double wave_nhPres(double x, double y, double z, double t) {
+    if (!G.initialized || G.wavetype != 1) return 0.0;
+    z = std::max(-G.depth, z);
+    double p = 0, eta = 0;
+    direct_point(x, y, z, t, &eta, nullptr, nullptr, nullptr, &p);
+    return G.ramp(t, x, y) * (p - G.rho * G.gravity * eta);
+}
+
+double wave_SurfElev(double x, double y, double t) {
+    if (!G.initialized) return 0.0;
+    if (G.wavetype == 1) {
+        double eta = 0;
+        direct_point(x, y, 0.0, t, &eta, nullptr, nullptr, nullptr, nullptr);
+        return G.ramp(t, x, y) * eta;
+    }
+    if (G.wavetype == 5) return grid_point(x, y, 0.0, t, 0) * G.ramp(t, x, y);
+    return 0.0;
+}
+
+double wave_VFrac(double x, double y, double z, double t, double dc) {
+    const double e = wave_SurfElev(x, y, t);
+    if (e < z - (dc / 2.)) return 0.0;
+    if (e > z + (dc / 2.)) return 1.0;
+    return (e - (z - (dc / 2.))) / dc;
+}
+
+double wave_Seabed(double, double) { return -G.depth; }
+
+double* wave_Kinematics(double x, double y, double z, double t) {
+    thread_local double res[4];
+    res[0] = res[1] = res[2] = res[3] = 0.0;
+    if (!G.initialized) return res;
+    if (G.wavetype == 5) {
+        double o4[4];
+        if (G.cgrid->g->query(1, &t, &x, &y, &z, o4, nullptr, nullptr)) gpu_fatal("s-grid query");
+        const double r = G.ramp(t, x, y);
+        for (int q = 0; q < 4; ++q) res[q] = o4[q] * r;
+    } else if (G.wavetype == 1) {
+        const double zc = std::max(-G.depth, z);
+        direct_point(x, y, zc, t, &res[0], &res[1], &res[2], &res[3], nullptr);
+        const double r = G.ramp(t, x, y);
+        for (int q = 0; q < 4; ++q) res[q] *= r;
+    }
+    return res;
+}
+
+double wave_piston_position(double) { return 0.0; }
+
+void wave_force_update(double t) {
+    if (!G.initialized || G.wavetype != 5) return;
+    int n = 0;
+    if (cfdwm_grid_force_update(G.cgrid, t, &n)) gpu_fatal("s-grid update");
+}
+
+void wave_set_mpi_comm(int) {}
+void update_probes(double) {}
+
+double wave_phase_speed(int opt) { return G.wavetype == 1 ? phase_speed(G.sea, opt) : 0.; }
+double wave_mean_period(int opt) { return G.wavetype == 1 ? mean_period(G.sea, opt) : 0.; }
+double wave_mean_length(int opt) { return G.wavetype == 1 ? mean_length(G.sea, opt) : 0.; }
+double wave_water_depth(void) { return G.depth; }
+
+double VelocityX(int, int, int, double x, double y, double z, double t) { return wave_VeloX(x, y, z, t); }
+double VelocityY(int, int, int, double x, double y, double z, double t) { return wave_VeloY(x, y, z, t); }
+double VelocityZ(int, int, int, double x, double y, double z, double t) { return wave_VeloZ(x, y, z, t); }
+double DynamicPressure(int, int, int, double x, double y, double z, double t) { return wave_DynPres(x, y, z, t); }
+double SurfaceElevation(int, int, double x, double y, double t) { return wave_SurfElev(x, y, t); }
+double VolumeFraction(double x, double y, double z, double t, double d) { return wave_VFrac(x, y, z, t, d); }
+int Init(void*) { return wave_Initialize(); }
+int Prepare(double, int) { return 0; }
+int Cleanup(void) { return wave_Cleanup(); }
+
+// =================================================================== Part 2
+static int eval_points_impl(int64_t n, const double* x, const double* y, const double* z, const double* t,
+                            double* eta, double* u, double* v, double* w, double* p, uint32_t flags, bool dev) {
+    if (!G.initialized) { set_error("wave_Initialize has not been called"); return -1; }
+    if (n < 0 || !x || !y || !t || ((u || v || w || p) && !z)) { set_error("bad arguments"); return -1; }
+    if (n == 0) return 0;
+    if (G.wavetype == 1) {
+        Sea& s = *G.csea->s;
+        std::vector<double> zc, e1, e2;
+        const double* zz = z;
+        // z >= -depth (CFDwavemaker.cpp:1547) and the ramps are applied on the host
+        std::vector<double> hx, hy, ht, hz;
+        if (dev) {
+            hx.resize(n); hy.resize(n); ht.resize(n);
+            CFDWM_HIP(hipMemcpy(hx.data(), x, n * 8, hipMemcpyDeviceToHost));
+            CFDWM_HIP(hipMemcpy(hy.data(), y, n * 8, hipMemcpyDeviceToHost));
+            CFDWM_HIP(hipMemcpy(ht.data(), t, n * 8, hipMemcpyDeviceToHost));
+            if (z) { hz.resize(n); CFDWM_HIP(hipMemcpy(hz.data(), z, n * 8, hipMemcpyDeviceToHost)); }
+            x = hx.data(); y = hy.data(); t = ht.data(); z = z ? hz.data() : nullptr;
+        }
+        if (z) {
+            zc.resize(n);
+            for (int64_t q = 0; q < n; ++q) zc[q] = std::max(-G.depth, z[q]);
+            zz = zc.data();
+        }
+        std::vector<double> hu, hv, hw, hp, he;
+        auto want = [&](double* o, std::vector<double>& b) -> double* { if (!o) return nullptr; b.resize(n); return b.data(); };
+        double *ou = want(u, hu), *ov = want(v, hv), *ow = want(w, hw), *op = want(p, hp);
+        if (eta) { e1.resize(n); e2.resize(n); }
+        const uint32_t fl = order_flags(G.order) | (p ? EV_P : 0u);
+        int rc = s.eval_points(n, x, y, zz, t, fl, G.order == 2, eta ? e1.data() : nullptr,
+                               eta && G.order == 2 ? e2.data() : nullptr, nullptr, ou, ov, ow, op, false);
+        if (rc) return rc;
+        if (eta) { he.resize(n); for (int64_t q = 0; q < n; ++q) he[q] = G.swl + e1[q] + (G.order == 2 ? e2[q] : 0.0); }
+        auto fin = [&](double* dst, std::vector<double>& src) -> int {
+            if (!dst) return 0;
+            for (int64_t q = 0; q < n; ++q) src[q] *= G.ramp(t[q], x[q], y[q]);
+            if (dev) CFDWM_HIP(hipMemcpy(dst, src.data(), n * 8, hipMemcpyHostToDevice));
+            else std::memcpy(dst, src.data(), n * 8);
+            return 0;
+        };
+        if (fin(eta, he) || fin(u, hu) || fin(v, hv) || fin(w, hw) || fin(p, hp)) return -3;
+        return 0;
+    }
+    if (G.wavetype == 5) {
+        // s-grid: eta at z = 0 (wave_SurfElev) and velocities at z (wave_Velo*)
+        std::vector<double> hx(x, x + (dev ? 0 : n)), hy, ht, hz;
+        if (dev) {
+            hx.resize(n); hy.resize(n); ht.resize(n);
+            CFDWM_HIP(hipMemcpy(hx.data(), x, n * 8, hipMemcpyDeviceToHost));
+            CFDWM_HIP(hipMemcpy(hy.data(), y, n * 8, hipMemcpyDeviceToHost));
+            CFDWM_HIP(hipMemcpy(ht.data(), t, n * 8, hipMemcpyDeviceToHost));
+            if (z) { hz.resize(n); CFDWM_HIP(hipMemcpy(hz.data(), z, n * 8, hipMemcpyDeviceToHost)); }
+        } else {
+            hy.assign(y, y + n); ht.assign(t, t + n);
+            if (z) hz.assign(z, z + n);
+        }
+        std::vector<double> o4(4 * n), o0;
+        Grid& g = *G.cgrid->g;
+        if (u || v || w) {
+            std::vector<double> zc(n);
+            for (int64_t q = 0; q < n; ++q) zc[q] = std::max(-G.depth, hz[q]);
+            if (int rc = g.query(n, ht.data(), hx.data(), hy.data(), zc.data(), o4.data(), nullptr, nullptr)) return rc;
+        }
+        if (eta) {
+            std::vector<double> z0(n, 0.0);
+            o0.resize(4 * n);
+            if (int rc = g.query(n, ht.data(), hx.data(), hy.data(), z0.data(), o0.data(), nullptr, nullptr)) return rc;
+        }
+        if (p) { set_error("dynamic pressure is defined for wave type 1 only (CFDwavemaker.cpp:1800-1813)"); }
+        std::vector<double> col(n);
+        auto fin = [&](double* dst, const std::vector<double>& src, int c) -> int {
+            if (!dst) return 0;
+            for (int64_t q = 0; q < n; ++q) col[q] = src[4 * q + c] * G.ramp(ht[q], hx[q], hy[q]);
+            if (dev) CFDWM_HIP(hipMemcpy(dst, col.data(), n * 8, hipMemcpyHostToDevice));
+            else std::memcpy(dst, col.data(), n * 8);
+            return 0;
+        };
+        if (fin(eta, o0, 0) || fin(u, o4, 1) || fin(v, o4, 2) || fin(w, o4, 3)) return -3;
+        if (p) {
+            std::vector<double> zero(n, 0.0);
+            if (dev) CFDWM_HIP(hipMemcpy(p, zero.data(), n * 8, hipMemcpyHostToDevice));
+            else std::memcpy(p, zero.data(), n * 8);
+        }
+        return 0;
+    }
+    set_error("wave type not supported by this build");
+    return -1;
+}
+
+int wave_EvalPoints(int64_t n, const double* x, const double* y, const double* z, const double* t,
+                    double* eta, double* u, double* v, double* w, double* p, uint32_t flags) {
+    return eval_points_impl(n, x, y, z, t, eta, u, v, w, p, flags, false);
+}
+int wave_EvalPointsDevice(int64_t n, const double* x, const double* y, const double* z, const double* t,
+                          double* eta, double* u, double* v, double* w, double* p, uint32_t flags) {
+    return eval_points_impl(n, x, y, z, t, eta, u, v, w, p, flags, true);
+}
+
+int cfdwm_sea_create(cfdwm_sea** out, int n, const double* omega, const double* A, const double* k,
+                     const double* phase, const double* theta, const double* ramptime,
+                     const double* starttime, const double* duration, const cfdwm_sea_params* P) {
+    if (!out || n <= 0 || !omega || !A || !k || !phase || !theta || !P) { set_error("bad arguments"); return -1; }
+    SeaInput in;
+    in.omega.assign(omega, omega + n); in.A.assign(A, A + n); in.k.assign(k, k + n);
+    in.phase.assign(phase, phase + n); in.theta.assign(theta, theta + n);
+    if (ramptime) in.ramptime.assign(ramptime, ramptime + n);
+    if (starttime) in.starttime.assign(starttime, starttime + n);
+    if (duration) in.duration.assign(duration, duration + n);
+    in.depth = P->depth; in.g = P->g; in.rho = P->rho; in.swl = P->swl; in.mtheta = P->mtheta;
+    in.tofmax = P->tofmax; in.x0 = P->x0; in.y0 = P->y0; in.bandwidth = P->bandwidth; in.cutoff = P->cutoff;
+    in.order = P->order;
+    if (P->bw_auto) {
+        double m[3];
+        spectral_moments(in, m);
+        in.bandwidth = 0.7 * (m[1] / m[0]);
+    }
+    const double ampl = P->ampl == 0.0 ? 1.0 : P->ampl;
+    const double sumA = std::accumulate(in.A.begin(), in.A.end(), 0.);
+    for (int i = 0; i < n; ++i) {
+        if (P->normalize) { in.A[i] = ampl * in.A[i] / sumA; in.phase[i] = 0.; }
+        else in.A[i] = ampl * in.A[i];
+    }
+    auto cs = std::make_unique<cfdwm_sea>();
+    cs->s = std::make_unique<Sea>();
+    cs->s->prec = P->precision == 1 ? kFP32 : kFP64;
+    cs->with_direct = P->with_direct != 0;
+    std::string err = build_tables(in, cs->s->tab, cs->with_direct);
+    if (!err.empty()) { set_error(err); return -1; }
+    if (int rc = cs->s->upload(cs->with_direct)) return rc;
+    *out = cs.release();
+    return 0;
+}
+
+int cfdwm_sea_destroy(cfdwm_sea* s) { delete s; return 0; }
+
+int cfdwm_sea_terms(cfdwm_sea* s, int64_t* pair_terms, int64_t* diag_terms, int64_t* direct_terms) {
+    if (!s) return -1;
+    if (pair_terms) *pair_terms = s->s->tab.filt.nterms;
+    if (diag_terms) *diag_terms = s->s->tab.diag_filtered;
+    if (direct_terms) *direct_terms = s->s->tab.direct.nterms;
+    return 0;
+}
+
+int cfdwm_eval(cfdwm_sea* cs, int kind, int64_t n, const double* t, const double* x, const double* y,
+               const double* z, double* out) {
+    if (!cs || n < 0 || !t || !x || !y || !out) { set_error("bad arguments"); return -1; }
+    Sea& s = *cs->s;
+    const int order = s.tab.in.order;
+    std::vector<double> a(n), b(n), gr(4 * n);
+    auto needz = [&]() { return z != nullptr; };
+    switch (kind) {
+    case 0: case 1: case 2: {
+        const bool e2 = kind != 0 && (kind == 1 || order == 2);
+        if (int rc = s.eval_points(n, x, y, z, t, 0, e2, a.data(), e2 ? b.data() : nullptr, nullptr,
+                                   nullptr, nullptr, nullptr, nullptr, false)) return rc;
+        for (int64_t q = 0; q < n; ++q)
+            out[q] = kind == 0 ? a[q] : (kind == 1 ? b[q] : s.tab.in.swl + a[q] + (e2 ? b[q] : 0.0));
+        return 0;
+    }
+    case 7: case 8: case 9: case 10: {
+        if (int rc = s.eval_points(n, x, y, z, t, 0, false, nullptr, nullptr, gr.data(), nullptr, nullptr,
+                                   nullptr, nullptr, false)) return rc;
+        std::memcpy(out, gr.data() + (kind - 7) * n, n * 8);
+        return 0;
+    }
+    default: break;
+    }
+    if (!needz()) { set_error("z required"); return -1; }
+    uint32_t fl = 0;
+    int comp = 0;   // 0 u 1 v 2 w 3 p
+    switch (kind) {
+    case 3: case 4: case 5: fl = EV_LIN; comp = kind - 3; break;
+    case 6: fl = EV_LIN | EV_P; comp = 3; break;
+    case 11: case 12: case 13: fl = EV_SEC | EV_DIRECT; comp = kind - 11; break;
+    case 14: fl = EV_SEC | EV_DIRECT | EV_P; comp = 3; break;
+    case 15: case 16: case 17: fl = order_flags(order); comp = kind - 15; break;
+    case 18: fl = order_flags(order) | EV_P; comp = 3; break;
+    case 19: case 20: case 21: fl = EV_SEC; comp = kind - 19; break;
+    default:
+        set_error("evaluation kind not supported (uvw2.p is read by no API path of the reference)");
+        return -1;
+    }
+    if ((fl & EV_DIRECT) && !cs->with_direct) { set_error("sea created without the unfiltered table"); return -1; }
+    double* o[4] = {nullptr, nullptr, nullptr, nullptr};
+    o[comp] = out;
+    return s.eval_points(n, x, y, z, t, fl, false, nullptr, nullptr, nullptr, o[0], o[1], o[2], o[3], false);
+}
+
+int cfdwm_grid_create(cfdwm_grid** out, cfdwm_sea* sea, const cfdwm_grid_desc* d) {
+    if (!out || !sea || !d) { set_error("bad arguments"); return -1; }
+    GridDesc gd{};
+    std::memcpy(gd.domain, d->domain, sizeof(gd.domain));
+    gd.nx = d->nx; gd.ny = d->ny; gd.nl = d->nl; gd.t0 = d->t0; gd.dt = d->dt;
+    gd.tan_a = d->tan_a; gd.tan_b = d->tan_b; gd.water_depth = d->water_depth; gd.swl = d->swl;
+    std::memcpy(gd.domain_ignore, d->domain_ignore, sizeof(gd.domain_ignore));
+    gd.ignore_at_init = d->ignore_at_init;
+    auto cg = std::make_unique<cfdwm_grid>();
+    cg->g = std::make_unique<Grid>(sea->s.get(), gd);
+    if (int rc = cg->g->init()) return rc;
+    *out = cg.release();
+    return 0;
+}
+int cfdwm_grid_destroy(cfdwm_grid* g) { delete g; return 0; }
+int cfdwm_grid_refill(cfdwm_grid* g) { return g ? g->g->refill() : -1; }
+int cfdwm_grid_force_update(cfdwm_grid* g, double t, int* n) { return g ? g->g->force_update(t, n) : -1; }
+int cfdwm_grid_state(cfdwm_grid* g, int* tstep, double* t0, double* dx, double* dy, double* ds) {
+    if (!g) return -1;
+    if (tstep) *tstep = g->g->tstep;
+    if (t0) *t0 = g->g->t0;
+    if (dx) *dx = g->g->g.dx;
+    if (dy) *dy = g->g->g.dy;
+    if (ds) *ds = g->g->g.ds;
+    return 0;
+}
+int64_t cfdwm_grid_export(cfdwm_grid* g, int which, double* out) { return g ? g->g->export_field(which, out) : -1; }
+int cfdwm_grid_query(cfdwm_grid* g, int64_t n, const double* t, const double* x, const double* y,
+                     const double* z, double* out4) {
+    if (!g || n < 0 || !t || !x || !y || !z || !out4) { set_error("bad arguments"); return -1; }
+    return g->g->query(n, t, x, y, z, out4, nullptr, nullptr);
+}
+int cfdwm_grid_query_device_f32(cfdwm_grid* g, int64_t n, double t, const float* x, const float* y,
+                                const float* z, float* eta, float* u, float* v, float* w) {
+    if (!g || n < 0 || !x || !y || !z || !eta || !u || !v || !w) { set_error("bad arguments"); return -1; }
+    return g->g->query_device_f32(n, t, x, y, z, eta, u, v, w);
+}
+int cfdwm_sync(void) {
+    if (Engine::get().init()) return -2;
+    CFDWM_HIP(hipStreamSynchronize(Engine::get().stream()));
+    return 0;
+}
+int cfdwm_last_error(char* buf, int len) {
+    const std::string e = last_error();
+    if (buf && len > 0) {
+        std::strncpy(buf, e.c_str(), (size_t)len - 1);
+        buf[len - 1] = 0;
+    }
+    return (int)e.size();
+}
+int cfdwm_prof_enable(int on) {
+    if (int e = Engine::get().init()) return e;
+    Engine::get().prof_enable(on != 0);
+    return 0;
+}
+int cfdwm_prof_read(char* names, double* total_ms, int64_t* count, int cap, int reset) {
+    auto m = Engine::get().prof_read(reset != 0);
+    int q = 0;
+    for (auto& kv : m) {
+        if (q >= cap) break;
+        if (names) { std::strncpy(names + 64 * q, kv.first.c_str(), 63); names[64 * q + 63] = 0; }
+        if (total_ms) total_ms[q] = kv.second.total_ms;
+        if (count) count[q] = kv.second.count;
+        ++q;
+    }
+    return q;
+}
+
+}  // extern "C"

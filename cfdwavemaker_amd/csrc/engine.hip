@@ -1,0 +1,489 @@
+// Device engine: HIP stream, device-resident sea tables and s-grids,
+// launch orchestration for the column / level / gradient / query kernels.
+#include "engine.hpp"
+
+#include <cmath>
+#include <cstring>
+
+namespace cfdwm {
+
+// ------------------------------------------------------------------ errors
+namespace {
+std::mutex g_err_mu;
+std::string g_err;
+}  // namespace
+void set_error(const std::string& msg) {
+    std::lock_guard<std::mutex> lk(g_err_mu);
+    g_err = msg;
+}
+std::string last_error() {
+    std::lock_guard<std::mutex> lk(g_err_mu);
+    return g_err;
+}
+
+// ------------------------------------------------------------------ DevBuf
+DevBuf::~DevBuf() { release(); }
+void DevBuf::release() {
+    if (p_) (void)hipFree(p_);
+    p_ = nullptr;
+    n_ = 0;
+}
+int DevBuf::alloc(size_t bytes) {
+    if (bytes <= n_ && p_) return 0;
+    release();
+    if (bytes == 0) return 0;
+    CFDWM_HIP(hipMalloc(&p_, bytes));
+    n_ = bytes;
+    return 0;
+}
+int DevBuf::upload(const void* src, size_t bytes, hipStream_t st) {
+    if (alloc(bytes)) return -3;
+    if (bytes) CFDWM_HIP(hipMemcpyAsync(p_, src, bytes, hipMemcpyHostToDevice, st));
+    return 0;
+}
+
+// ------------------------------------------------------------------ Engine
+Engine& Engine::get() {
+    static Engine e;
+    return e;
+}
+int Engine::init() {
+    if (ready_) return 0;
+    int n = 0;
+    hipError_t e = hipGetDeviceCount(&n);
+    if (e != hipSuccess || n <= 0) {
+        set_error("no HIP device available (hipGetDeviceCount: " + std::string(hipGetErrorString(e)) + ")");
+        return -2;
+    }
+    CFDWM_HIP(hipSetDevice(0));
+    CFDWM_HIP(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
+    ready_ = true;
+    return 0;
+}
+hipEvent_t Engine::take_event() {
+    if (!pool_.empty()) {
+        hipEvent_t e = pool_.back();
+        pool_.pop_back();
+        return e;
+    }
+    hipEvent_t e = nullptr;
+    if (hipEventCreate(&e) != hipSuccess) return nullptr;
+    return e;
+}
+void Engine::prof_begin() {
+    if (!prof_) return;
+    cur_ = take_event();
+    if (cur_) (void)hipEventRecord(cur_, stream_);
+}
+void Engine::prof_end(const char* name) {
+    if (!prof_ || !cur_) return;
+    hipEvent_t e = take_event();
+    if (!e) return;
+    (void)hipEventRecord(e, stream_);
+    pending_.push_back({name, {cur_, e}});
+    cur_ = nullptr;
+}
+int Engine::prof_flush() {
+    for (auto& p : pending_) {
+        float ms = 0;
+        if (hipEventSynchronize(p.second.second) == hipSuccess &&
+            hipEventElapsedTime(&ms, p.second.first, p.second.second) == hipSuccess) {
+            KernelTimes& k = times_[p.first];
+            k.total_ms += ms;
+            k.count += 1;
+        }
+        pool_.push_back(p.second.first);
+        pool_.push_back(p.second.second);
+    }
+    pending_.clear();
+    return 0;
+}
+std::map<std::string, KernelTimes> Engine::prof_read(bool reset) {
+    prof_flush();
+    auto r = times_;
+    if (reset) times_.clear();
+    return r;
+}
+
+#define PROF(name, call)                    \
+    do {                                    \
+        Engine::get().prof_begin();         \
+        hipError_t pe_ = (call);            \
+        Engine::get().prof_end(name);       \
+        CFDWM_HIP(pe_);                     \
+    } while (0)
+
+// ------------------------------------------------------------------ Sea
+namespace {
+constexpr double kLog2e = 1.4426950408889634074;
+
+template <typename R>
+CompRec<R> comp_rec(const CompTable& c, int i, double h) {
+    CompRec<R> r{};
+    r.lu = (R)c.lu[i]; r.lv = (R)c.lv[i]; r.lw = (R)c.lw[i]; r.lp = (R)c.lp[i];
+    r.gxz = (R)c.gxz[i]; r.gyz = (R)c.gyz[i]; r.gzz = (R)c.gzz[i]; r.gpz = (R)c.gpz[i];
+    r.A = (R)c.A[i]; r.e2d = (R)c.e2d[i];
+    r.dfu = (R)c.dfu[i]; r.dfv = (R)c.dfv[i]; r.dfw = (R)c.dfw[i];
+    r.ddu = (R)c.ddu[i]; r.ddv = (R)c.ddv[i]; r.ddw = (R)c.ddw[i]; r.ddp = (R)c.ddp[i];
+    r.Xx = (R)c.Xx[i]; r.Xy = (R)c.Xy[i]; r.Zk = (R)c.Zk[i]; r.kd1 = (R)c.kd1[i]; r.kd2 = (R)c.kd2[i];
+    const double s = sizeof(R) == 4 ? kLog2e : 1.0;
+    r.pk = (R)(c.k[i] * s);
+    r.pg = (R)(-2.0 * h * c.k[i] * s);
+    r.p2k = (R)(2.0 * c.k[i] * s);
+    r.p2g = (R)(-4.0 * h * c.k[i] * s);
+    return r;
+}
+
+template <typename R>
+UvwRec<R> uvw_rec(const PairRec64& p, double h) {
+    const double s = sizeof(R) == 4 ? kLog2e : 1.0;
+    UvwRec<R> r{};
+    r.cuP = (R)p.cuP; r.cvP = (R)p.cvP; r.cwP = (R)p.cwP;
+    r.pkP = (R)(p.kP * s); r.pgP = (R)(-2.0 * h * p.kP * s);
+    r.cuM = (R)p.cuM; r.cvM = (R)p.cvM; r.cwM = (R)p.cwM;
+    r.pkM = (R)(p.kM * s); r.pgM = (R)(-2.0 * h * p.kM * s);
+    return r;
+}
+}  // namespace
+
+template <typename R>
+int Sea::upload_t(bool with_direct) {
+    Engine& E = Engine::get();
+    hipStream_t st = E.stream();
+    const CompTable& c = tab.comp;
+    const int N = c.n;
+    const double h = tab.in.depth;
+    std::vector<PhaseRec> ph(N);
+    std::vector<CompRec<R>> cr(N);
+    std::vector<uint8_t> act(N);
+    for (int i = 0; i < N; ++i) {
+        ph[i] = PhaseRec{c.k[i], c.cth[i], c.sth[i], c.omega[i], c.ph0[i], c.rt[i], c.ts[i], c.du[i]};
+        cr[i] = comp_rec<R>(c, i, h);
+        act[i] = c.rowActive[i];
+    }
+    if (ph_.upload(ph.data(), ph.size() * sizeof(PhaseRec), st)) return -3;
+    if (cr_.upload(cr.data(), cr.size() * sizeof(CompRec<R>), st)) return -3;
+    if (act_.upload(act.data(), act.size(), st)) return -3;
+    auto put_table = [&](const PairTable& P, bool eta, DevBuf& row, DevBuf* mhi, DevBuf& uvw, DevBuf& xr) -> int {
+        const int64_t ne = (int64_t)P.rec.size();
+        std::vector<UvwRec<R>> u(ne > 0 ? ne : 1);
+        std::vector<XRec<R>> x(ne > 0 ? ne : 1);
+        for (int64_t e = 0; e < ne; ++e) {
+            u[e] = uvw_rec<R>(P.rec[e], h);
+            x[e] = XRec<R>{(R)P.rec[e].xP, (R)P.rec[e].xM};
+        }
+        (void)eta;
+        if (row.upload(P.rowoff.data(), P.rowoff.size() * sizeof(int64_t), st)) return -3;
+        if (mhi && mhi->upload(P.mhi.data(), P.mhi.size() * sizeof(int32_t), st)) return -3;
+        if (uvw.upload(u.data(), u.size() * sizeof(UvwRec<R>), st)) return -3;
+        if (xr.upload(x.data(), x.size() * sizeof(XRec<R>), st)) return -3;
+        CFDWM_HIP(hipStreamSynchronize(st));   // host staging vectors go out of scope
+        return 0;
+    };
+    if (put_table(tab.filt, true, frow_, &fmhi_, fuvw_, fx_)) return -3;
+    if (with_direct && put_table(tab.direct, false, drow_, nullptr, duvw_, dx_)) return -3;
+    SeaDev<R> d{};
+    d.n = N;
+    d.h = h; d.swl = tab.in.swl; d.x0 = tab.in.x0; d.y0 = tab.in.y0; d.rho = tab.in.rho;
+    d.ph = (const PhaseRec*)ph_.get();
+    d.cr = (const CompRec<R>*)cr_.get();
+    d.rowActive = (const uint8_t*)act_.get();
+    d.frow = (const int64_t*)frow_.get();
+    d.fmhi = (const int32_t*)fmhi_.get();
+    d.fuvw = (const UvwRec<R>*)fuvw_.get();
+    d.fx = (const XRec<R>*)fx_.get();
+    d.drow = (const int64_t*)drow_.get();
+    d.duvw = (const UvwRec<R>*)duvw_.get();
+    d.dx = (const XRec<R>*)dx_.get();
+    if constexpr (sizeof(R) == 8) d64_ = d; else d32_ = d;
+    CFDWM_HIP(hipStreamSynchronize(st));
+    return 0;
+}
+
+int Sea::upload(bool with_direct) {
+    if (int e = Engine::get().init()) return e;
+    return prec == kFP32 ? upload_t<float>(with_direct) : upload_t<double>(with_direct);
+}
+
+template <>
+const SeaDev<double>& Sea::dev<double>() const { return d64_; }
+template <>
+const SeaDev<float>& Sea::dev<float>() const { return d32_; }
+
+namespace {
+template <typename R>
+int run_points(Sea& s, int64_t n, const double* dx, const double* dy, const double* dz, const double* dt,
+               uint32_t flags, bool want_cols, bool want_eta2, bool want_lev, double* o) {
+    hipStream_t st = Engine::get().stream();
+    const SeaDev<R>& S = s.dev<R>();
+    if (want_cols) {
+        ColumnJob<R> J;
+        J.nslot = 1; J.ncol = n;
+        J.x = dx; J.y = dy; J.t = dt;
+        J.want_eta2 = want_eta2 ? 1 : 0;
+        J.o_eta1 = o; J.o_eta2 = o + n; J.o_gx = o + 2 * n; J.o_gy = o + 3 * n; J.o_gz = o + 4 * n; J.o_gp = o + 5 * n;
+        PROF("k_columns_points", launch_columns<R>(S, J, false, st));
+    }
+    if (want_lev) {
+        LevelJob<R> J;
+        J.nslot = 1; J.ncol = n;
+        J.x = dx; J.y = dy; J.z = dz; J.t = dt;
+        J.flags = flags;
+        J.p_u = o + 6 * n; J.p_v = o + 7 * n; J.p_w = o + 8 * n; J.p_p = o + 9 * n;
+        PROF("k_levels_points", launch_levels<R>(S, J, false, 1, st));
+    }
+    return 0;
+}
+}  // namespace
+
+int Sea::eval_points(int64_t n, const double* x, const double* y, const double* z, const double* t,
+                     uint32_t flags, bool want_eta2, double* eta1, double* eta2, double* grads4,
+                     double* u, double* v, double* w, double* p, bool device_ptrs) {
+    if (n <= 0) return 0;
+    std::lock_guard<std::mutex> lk(mu);
+    hipStream_t st = Engine::get().stream();
+    const bool want_cols = eta1 || eta2 || grads4;
+    const bool want_lev = u || v || w || p;
+    const double *dx = x, *dy = y, *dz = z, *dt = t;
+    if (!device_ptrs) {
+        if (pts_in.alloc(4 * n * sizeof(double))) return -3;
+        double* b = (double*)pts_in.get();
+        CFDWM_HIP(hipMemcpyAsync(b, x, n * sizeof(double), hipMemcpyHostToDevice, st));
+        CFDWM_HIP(hipMemcpyAsync(b + n, y, n * sizeof(double), hipMemcpyHostToDevice, st));
+        CFDWM_HIP(hipMemcpyAsync(b + 3 * n, t, n * sizeof(double), hipMemcpyHostToDevice, st));
+        if (z) CFDWM_HIP(hipMemcpyAsync(b + 2 * n, z, n * sizeof(double), hipMemcpyHostToDevice, st));
+        else CFDWM_HIP(hipMemsetAsync(b + 2 * n, 0, n * sizeof(double), st));
+        dx = b; dy = b + n; dz = b + 2 * n; dt = b + 3 * n;
+    }
+    if (pts_out.alloc(10 * n * sizeof(double))) return -3;
+    double* o = (double*)pts_out.get();
+    int rc = prec == kFP32 ? run_points<float>(*this, n, dx, dy, dz, dt, flags, want_cols, want_eta2, want_lev, o)
+                           : run_points<double>(*this, n, dx, dy, dz, dt, flags, want_cols, want_eta2, want_lev, o);
+    if (rc) return rc;
+    const hipMemcpyKind kind = device_ptrs ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost;
+    auto get = [&](double* dst, int slot) -> int {
+        if (dst) CFDWM_HIP(hipMemcpyAsync(dst, o + slot * n, n * sizeof(double), kind, st));
+        return 0;
+    };
+    if (get(eta1, 0) || get(eta2, 1) || get(u, 6) || get(v, 7) || get(w, 8) || get(p, 9)) return -3;
+    if (grads4)
+        for (int q = 0; q < 4; ++q)
+            if (get(grads4 + q * n, 2 + q)) return -3;
+    CFDWM_HIP(hipStreamSynchronize(st));
+    return 0;
+}
+
+// ------------------------------------------------------------------ Grid
+Grid::Grid(Sea* s, const GridDesc& desc) : sea(s), d(desc), prec(s->prec) {}
+Grid::~Grid() = default;
+
+int64_t Grid::field_count(int which) const {
+    const int64_t c = (int64_t)g.nx * g.ny, v = c * g.nl;
+    switch (which) {
+    case 0: case 3: return 4 * c;
+    case 1: case 2: return 2 * c;
+    case 4: case 5: case 6: case 7: case 8: case 9: return 4 * v;
+    default: return which >= 10 && which <= 18 ? 2 * v : 0;
+    }
+}
+
+int Grid::init() {
+    if (d.nx < 1 || d.ny < 1 || d.nl < 1) {
+        set_error("s-grid needs nx, ny, nl >= 1");
+        return -1;
+    }
+    // set_ignore / initialize_kinematics geometry (lsgrid_spline.cpp:1406-1425, 672-675)
+    g.nx = d.nx; g.ny = d.ny; g.nl = d.nl;
+    g.x0 = d.domain[0]; g.y0 = d.domain[2];
+    g.dx = (d.domain[1] - d.domain[0]) / std::max(1., double(d.nx - 1));
+    g.dy = (d.domain[3] - d.domain[2]) / std::max(1., double(d.ny - 1));
+    g.ds = 1. / std::max(1., double(d.nl - 1));
+    g.h = d.water_depth;
+    g.swl = d.swl;
+    ignore_.assign((size_t)d.nx * d.ny, 0);
+    std::vector<int32_t> act;
+    for (int i = 0; i < d.nx; ++i) {
+        const double xp = d.domain[0] + g.dx * i;
+        for (int j = 0; j < d.ny; ++j) {
+            const double yp = d.domain[2] + g.dy * j;
+            const bool ig = xp >= d.domain_ignore[0] && xp <= d.domain_ignore[1] &&
+                            yp >= d.domain_ignore[2] && yp <= d.domain_ignore[3];
+            ignore_[(size_t)i * d.ny + j] = ig ? 1 : 0;
+            if (!ig) act.push_back(i * d.ny + j);
+        }
+    }
+    nact_ = (int64_t)act.size();
+    // sigma of each level: sigmaS2cart (lsgrid_spline.cpp:114-119), host libm
+    sigma_.assign(d.nl, 0.0);
+    for (int m = 0; m < d.nl; ++m) {
+        const double s = g.ds * m;
+        sigma_[m] = -std::pow(std::tan(-(s - 1) * d.tan_a), d.tan_b) / std::pow(std::tan(d.tan_a), d.tan_b);
+    }
+    hipStream_t st = Engine::get().stream();
+    if (sigma_d_.upload(sigma_.data(), sigma_.size() * sizeof(double), st)) return -3;
+    if (cols_act_d_.upload(act.data(), std::max<size_t>(1, act.size()) * sizeof(int32_t), st)) return -3;
+    const size_t esz = prec == kFP32 ? 4 : 8;
+    for (int f = 0; f < 19; ++f) {
+        const size_t b = (size_t)field_count(f) * esz;
+        if (f_[f].alloc(b)) return -3;
+        CFDWM_HIP(hipMemsetAsync(f_[f].get(), 0, b, st));
+    }
+    level_chunk = d.nl >= 16 ? 16 : (d.nl > 8 ? 16 : (d.nl > 4 ? 8 : (d.nl > 2 ? 4 : (d.nl > 1 ? 2 : 1))));
+    if (const char* e = std::getenv("CFDWM_LEVEL_CHUNK")) level_chunk = std::atoi(e);
+    return refill();
+}
+
+template <typename R>
+int Grid::fill_slots(int nslot, const double* slot_t, const int* slot_dst, bool use_ignore) {
+    const SeaDev<R>& S = sea->dev<R>();
+    hipStream_t st = Engine::get().stream();
+    const int64_t ncell = (int64_t)g.nx * g.ny;
+    ColumnJob<R> C;
+    C.nslot = nslot;
+    C.ncol = use_ignore ? nact_ : ncell;
+    C.ncell = ncell;
+    C.cols = use_ignore ? (const int32_t*)cols_act_d_.get() : nullptr;
+    C.g = g;
+    for (int s = 0; s < nslot; ++s) { C.slot_t[s] = slot_t[s]; C.slot_dst[s] = slot_dst[s]; }
+    C.eta_grid = (R*)f_[0].get();
+    C.want_eta2 = 1;
+    // initialize_kinematics: ETA = eta1 + eta2 + swl (lsgrid_spline.cpp:700)
+    PROF("k_columns_grid", launch_columns<R>(S, C, true, st));
+    LevelJob<R> L;
+    L.nslot = nslot; L.ncol = C.ncol; L.ncell = ncell; L.cols = C.cols; L.g = g;
+    for (int s = 0; s < nslot; ++s) { L.slot_t[s] = slot_t[s]; L.slot_dst[s] = slot_dst[s]; }
+    L.eta_grid = (const R*)f_[0].get();
+    L.sigma = (const double*)sigma_d_.get();
+    L.o_u = (R*)f_[4].get(); L.o_v = (R*)f_[5].get(); L.o_w = (R*)f_[6].get();
+    L.flags = EV_LIN | EV_SEC | EV_TAYLOR;
+    PROF("k_levels_grid", launch_levels<R>(S, L, true, level_chunk, st));
+    return 0;
+}
+
+template <typename R>
+int Grid::gradients() {
+    hipStream_t st = Engine::get().stream();
+    const int tid0 = (tstep + 1) % 4, tid1 = (tstep + 2) % 4;
+    const int64_t c = (int64_t)g.nx * g.ny, v = c * g.nl;
+    auto P = [&](int f) { return (R*)f_[f].get(); };
+    PROF("k_grad_xyz", launch_grad_xyz<R>(P(0), P(1), P(2), (R*)nullptr, g.nx, g.ny, 1, tid0, tid1, g.dx, g.dy, g.ds, st));
+    PROF("k_grad_dt", launch_grad_dt<R>(P(0), P(3), c, tstep, d.dt, st));
+    for (int f = 0; f < 3; ++f) {
+        PROF("k_grad_xyz", launch_grad_xyz<R>(P(4 + f), P(10 + 3 * f), P(11 + 3 * f), P(12 + 3 * f), g.nx, g.ny, g.nl,
+                                              tid0, tid1, g.dx, g.dy, g.ds, st));
+        PROF("k_grad_dt", launch_grad_dt<R>(P(4 + f), P(7 + f), v, tstep, d.dt, st));
+    }
+    return 0;
+}
+
+int Grid::refill() {
+    tstep = 0;
+    t0 = d.t0;
+    // initialize_kinematics (:678) / initialize_kinematics_with_ignore (:767)
+    const double ta[4] = {t0 - d.dt, t0, t0 + d.dt, t0 + 2 * d.dt};
+    const double tb[4] = {t0, t0, t0 + d.dt, t0 + 2 * d.dt};
+    const int dst[4] = {0, 1, 2, 3};
+    const bool ign = d.ignore_at_init != 0;
+    int rc = prec == kFP32 ? fill_slots<float>(4, ign ? tb : ta, dst, ign)
+                           : fill_slots<double>(4, ign ? tb : ta, dst, ign);
+    if (rc) return rc;
+    rc = prec == kFP32 ? gradients<float>() : gradients<double>();
+    if (rc) return rc;
+    CFDWM_HIP(hipStreamSynchronize(Engine::get().stream()));
+    return 0;
+}
+
+int Grid::force_update(double t, int* n) {
+    int k = 0;
+    // CheckTime / update (lsgrid_spline.cpp:610-617, 847-931)
+    while (t > t0 + d.dt) {
+        tstep++;
+        t0 += d.dt;
+        const double t2 = t0 + 2 * d.dt;
+        const int slot = (tstep + 3) % 4;
+        int rc = prec == kFP32 ? fill_slots<float>(1, &t2, &slot, true) : fill_slots<double>(1, &t2, &slot, true);
+        if (rc) return rc;
+        rc = prec == kFP32 ? gradients<float>() : gradients<double>();
+        if (rc) return rc;
+        ++k;
+    }
+    if (k) CFDWM_HIP(hipStreamSynchronize(Engine::get().stream()));
+    if (n) *n = k;
+    return 0;
+}
+
+int64_t Grid::export_field(int which, double* out) {
+    if (which == 19) {
+        if (out) for (size_t q = 0; q < ignore_.size(); ++q) out[q] = ignore_[q];
+        return (int64_t)ignore_.size();
+    }
+    const int64_t n = field_count(which);
+    if (n == 0 || !out) return n;
+    hipStream_t st = Engine::get().stream();
+    if (prec == kFP32) {
+        std::vector<float> tmp(n);
+        if (hipMemcpyAsync(tmp.data(), f_[which].get(), n * 4, hipMemcpyDeviceToHost, st) != hipSuccess ||
+            hipStreamSynchronize(st) != hipSuccess) return -3;
+        for (int64_t q = 0; q < n; ++q) out[q] = tmp[q];
+    } else {
+        if (hipMemcpyAsync(out, f_[which].get(), n * 8, hipMemcpyDeviceToHost, st) != hipSuccess ||
+            hipStreamSynchronize(st) != hipSuccess) return -3;
+    }
+    return n;
+}
+
+template <typename R, typename TQ>
+int Grid::run_query(int64_t n, const TQ* t, double tconst, const TQ* x, const TQ* y, const TQ* z,
+                    TQ* eta, TQ* u, TQ* v, TQ* w, const double* ramp) {
+    QueryJob<R, TQ> J;
+    J.n = n; J.g = g; J.tstep = tstep; J.t0 = t0; J.dt = d.dt;
+    J.tan_a = d.tan_a; J.tan_b = d.tan_b; J.tanab = std::pow(std::tan(d.tan_a), d.tan_b);
+    auto P = [&](int f) { return (const R*)f_[f].get(); };
+    J.eta = SplineField<R>{P(0), P(3), P(1), P(2), nullptr};
+    J.u = SplineField<R>{P(4), P(7), P(10), P(11), P(12)};
+    J.v = SplineField<R>{P(5), P(8), P(13), P(14), P(15)};
+    J.w = SplineField<R>{P(6), P(9), P(16), P(17), P(18)};
+    J.t = t; J.tconst = tconst; J.x = x; J.y = y; J.z = z;
+    J.ramp_scale = ramp;
+    J.o_eta = eta; J.o_u = u; J.o_v = v; J.o_w = w;
+    PROF("k_query", (launch_query<R, TQ>(J, Engine::get().stream())));
+    return 0;
+}
+
+int Grid::query(int64_t n, const double* t, const double* x, const double* y, const double* z,
+                double* out4, const int* ramp_flags, const double* ramp_se) {
+    (void)ramp_flags; (void)ramp_se;
+    if (n <= 0) return 0;
+    hipStream_t st = Engine::get().stream();
+    std::lock_guard<std::mutex> lk(sea->mu);
+    if (q_in_.alloc(4 * n * sizeof(double)) || q_out_.alloc(4 * n * sizeof(double))) return -3;
+    double* b = (double*)q_in_.get();
+    double* o = (double*)q_out_.get();
+    CFDWM_HIP(hipMemcpyAsync(b, t, n * 8, hipMemcpyHostToDevice, st));
+    CFDWM_HIP(hipMemcpyAsync(b + n, x, n * 8, hipMemcpyHostToDevice, st));
+    CFDWM_HIP(hipMemcpyAsync(b + 2 * n, y, n * 8, hipMemcpyHostToDevice, st));
+    CFDWM_HIP(hipMemcpyAsync(b + 3 * n, z, n * 8, hipMemcpyHostToDevice, st));
+    int rc = prec == kFP32 ? run_query<float, double>(n, b, 0.0, b + n, b + 2 * n, b + 3 * n, o, o + n, o + 2 * n, o + 3 * n, nullptr)
+                           : run_query<double, double>(n, b, 0.0, b + n, b + 2 * n, b + 3 * n, o, o + n, o + 2 * n, o + 3 * n, nullptr);
+    if (rc) return rc;
+    std::vector<double> tmp(4 * n);
+    CFDWM_HIP(hipMemcpyAsync(tmp.data(), o, 4 * n * 8, hipMemcpyDeviceToHost, st));
+    CFDWM_HIP(hipStreamSynchronize(st));
+    for (int64_t p = 0; p < n; ++p)
+        for (int c = 0; c < 4; ++c) out4[4 * p + c] = tmp[c * n + p];
+    return 0;
+}
+
+int Grid::query_device_f32(int64_t n, double t, const float* x, const float* y, const float* z,
+                           float* eta, float* u, float* v, float* w) {
+    if (prec == kFP32) return run_query<float, float>(n, nullptr, t, x, y, z, eta, u, v, w, nullptr);
+    return run_query<double, float>(n, nullptr, t, x, y, z, eta, u, v, w, nullptr);
+}
+
+int Grid::query_device_f64(int64_t n, const double* t, const double* x, const double* y, const double* z,
+                           double* eta, double* u, double* v, double* w) {
+    if (prec == kFP32) return run_query<float, double>(n, t, 0.0, x, y, z, eta, u, v, w, nullptr);
+    return run_query<double, double>(n, t, 0.0, x, y, z, eta, u, v, w, nullptr);
+}
+
+}  // namespace cfdwm

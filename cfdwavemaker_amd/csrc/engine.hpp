@@ -1,0 +1,212 @@
+// Device engine: sea states and s-grids resident in HBM, kernel launch jobs.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "kernels.hpp"
+#include "sea_tables.hpp"
+
+namespace cfdwm {
+
+// ---------------------------------------------------------------- jobs
+template <typename R>
+struct ColumnJob {
+    int64_t nslot = 1, ncol = 0, ncell = 0;
+    const int32_t* cols = nullptr;        // GRID: active columns (null: all)
+    GridGeom g{};
+    double slot_t[4] = {0, 0, 0, 0};
+    int slot_dst[4] = {0, 0, 0, 0};
+    R* eta_grid = nullptr;
+    const double *x = nullptr, *y = nullptr, *t = nullptr;
+    int want_eta2 = 1;
+    double *o_eta1 = nullptr, *o_eta2 = nullptr, *o_gx = nullptr, *o_gy = nullptr, *o_gz = nullptr, *o_gp = nullptr;
+};
+
+template <typename R>
+struct LevelJob {
+    int64_t nslot = 1, ncol = 0, ncell = 0;
+    const int32_t* cols = nullptr;
+    GridGeom g{};
+    double slot_t[4] = {0, 0, 0, 0};
+    int slot_dst[4] = {0, 0, 0, 0};
+    const R* eta_grid = nullptr;
+    const double* sigma = nullptr;
+    R *o_u = nullptr, *o_v = nullptr, *o_w = nullptr;
+    const double *x = nullptr, *y = nullptr, *t = nullptr, *z = nullptr;
+    double *p_u = nullptr, *p_v = nullptr, *p_w = nullptr, *p_p = nullptr;
+    uint32_t flags = 0;
+};
+
+template <typename R>
+struct SplineField {
+    const R *f, *fdt, *fdx, *fdy, *fds;
+};
+
+template <typename R, typename TQ>
+struct QueryJob {
+    int64_t n = 0;
+    GridGeom g{};
+    int tstep = 0;
+    double t0 = 0, dt = 1, tan_a = 0, tan_b = 1, tanab = 1;
+    SplineField<R> eta{}, u{}, v{}, w{};
+    const TQ *t = nullptr, *x = nullptr, *y = nullptr, *z = nullptr;
+    double tconst = 0;
+    const double* ramp_scale = nullptr;
+    TQ *o_eta = nullptr, *o_u = nullptr, *o_v = nullptr, *o_w = nullptr;
+};
+
+template <typename R>
+hipError_t launch_columns(const SeaDev<R>& S, const ColumnJob<R>& J, bool grid, hipStream_t st);
+template <typename R>
+hipError_t launch_levels(const SeaDev<R>& S, const LevelJob<R>& J, bool grid, int lc, hipStream_t st);
+template <typename R>
+hipError_t launch_grad_xyz(const R* D, R* GX, R* GY, R* GS, int nx, int ny, int L, int tid0, int tid1,
+                           double dx, double dy, double ds, hipStream_t st);
+template <typename R>
+hipError_t launch_grad_dt(const R* D, R* GD, int64_t per, int tstep, double dt, hipStream_t st);
+template <typename R, typename TQ>
+hipError_t launch_query(const QueryJob<R, TQ>& J, hipStream_t st);
+
+// ---------------------------------------------------------------- errors
+struct Error {
+    int code;
+    std::string msg;
+};
+void set_error(const std::string& msg);
+std::string last_error();
+
+#define CFDWM_HIP(expr)                                                                  \
+    do {                                                                                  \
+        hipError_t e_ = (expr);                                                           \
+        if (e_ != hipSuccess) {                                                           \
+            ::cfdwm::set_error(std::string(#expr) + ": " + hipGetErrorString(e_));        \
+            return -3;                                                                    \
+        }                                                                                 \
+    } while (0)
+
+// --------------------------------------------------------- device buffers
+class DevBuf {
+public:
+    DevBuf() = default;
+    ~DevBuf();
+    DevBuf(const DevBuf&) = delete;
+    DevBuf& operator=(const DevBuf&) = delete;
+    int alloc(size_t bytes);                 // keeps contents when large enough
+    int upload(const void* src, size_t bytes, hipStream_t st);
+    void* get() const { return p_; }
+    size_t size() const { return n_; }
+    void release();
+private:
+    void* p_ = nullptr;
+    size_t n_ = 0;
+};
+
+// Kernel timing with HIP events on the engine stream (enabled on request).
+struct KernelTimes {
+    double total_ms = 0;
+    int64_t count = 0;
+};
+
+class Engine {
+public:
+    static Engine& get();
+    int init();                              // device + stream, idempotent
+    hipStream_t stream() const { return stream_; }
+    // timing: record an event pair around a launch when enabled
+    void prof_enable(bool on) { prof_ = on; }
+    bool prof_on() const { return prof_; }
+    void prof_begin();
+    void prof_end(const char* name);
+    int prof_flush();                        // resolves pending events
+    std::map<std::string, KernelTimes> prof_read(bool reset);
+private:
+    Engine() = default;
+    hipStream_t stream_ = nullptr;
+    bool ready_ = false, prof_ = false;
+    std::vector<std::pair<std::string, std::pair<hipEvent_t, hipEvent_t>>> pending_;
+    hipEvent_t cur_ = nullptr;
+    std::map<std::string, KernelTimes> times_;
+    std::vector<hipEvent_t> pool_;
+    hipEvent_t take_event();
+};
+
+enum Precision { kFP64 = 0, kFP32 = 1 };
+
+// A sea state resident on the device.
+class Sea {
+public:
+    SeaTables tab;
+    int prec = kFP64;
+    int upload(bool with_direct);
+    template <typename R>
+    const SeaDev<R>& dev() const;
+    std::mutex mu;                           // serialises the per-point scratch use
+    // scratch for point batches (inputs 4n doubles, outputs 6n doubles)
+    DevBuf pts_in, pts_out;
+    int eval_points(int64_t n, const double* x, const double* y, const double* z, const double* t,
+                    uint32_t flags, bool want_eta2, double* eta1, double* eta2, double* grads4,
+                    double* u, double* v, double* w, double* p, bool device_ptrs);
+private:
+    DevBuf ph_, cr_, act_, frow_, fmhi_, fuvw_, fx_, drow_, duvw_, dx_;
+    SeaDev<double> d64_{};
+    SeaDev<float> d32_{};
+    template <typename R>
+    int upload_t(bool with_direct);
+};
+
+struct GridDesc {
+    double domain[4];
+    int nx, ny, nl;
+    double t0, dt, tan_a, tan_b, water_depth, swl;
+    double domain_ignore[4];
+    int ignore_at_init;
+};
+
+// lsGridSpline resident on the device (4-slot time ring + derivatives).
+class Grid {
+public:
+    Grid(Sea* sea, const GridDesc& d);
+    ~Grid();
+    int init();                              // allocate + initialize_kinematics
+    int refill();                            // initialize_kinematics again (tstep back to 0)
+    int force_update(double t, int* n);      // wave_force_update
+    int64_t export_field(int which, double* out);
+    int query(int64_t n, const double* t, const double* x, const double* y, const double* z,
+              double* out4, const int* ramp_flags, const double* ramp_se);
+    int query_device_f32(int64_t n, double t, const float* x, const float* y, const float* z,
+                         float* eta, float* u, float* v, float* w);
+    int query_device_f64(int64_t n, const double* t, const double* x, const double* y, const double* z,
+                         double* eta, double* u, double* v, double* w);
+    Sea* sea;
+    GridDesc d;
+    GridGeom g{};
+    int tstep = 0;
+    double t0 = 0;
+    int level_chunk = 16;
+    int64_t points_per_fill() const { return 4LL * g.nx * g.ny * g.nl; }
+private:
+    int prec;
+    std::vector<int32_t> ignore_;
+    std::vector<double> sigma_;
+    DevBuf sigma_d_, cols_all_d_, cols_act_d_;
+    int64_t nact_ = 0;
+    // fields: 0 ETA(4) 1 ETAdx(2) 2 ETAdy(2) 3 ETAdt(4) 4..6 U/V/W(4) 7..9 dt(4) 10..18 dx/dy/ds(2)
+    DevBuf f_[19];
+    DevBuf q_in_, q_out_;
+    template <typename R>
+    int fill_slots(int nslot, const double* slot_t, const int* slot_dst, bool use_ignore);
+    template <typename R>
+    int gradients();
+    template <typename R, typename TQ>
+    int run_query(int64_t n, const TQ* t, double tconst, const TQ* x, const TQ* y, const TQ* z,
+                  TQ* eta, TQ* u, TQ* v, TQ* w, const double* ramp);
+    int64_t field_count(int which) const;
+};
+
+}  // namespace cfdwm

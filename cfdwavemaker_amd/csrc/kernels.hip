@@ -1,0 +1,585 @@
+// HIP kernels for CFDwavemaker's second-order irregular-wave kinematics on
+// gfx950 (MI355X).  Work decomposition:
+//   * one thread owns one column (x, y, t) and LC vertical levels of it;
+//   * every lane of a wavefront walks the same component i and pair (i, m)
+//     at the same time, so component/pair coefficients are wave-uniform and
+//     arrive through scalar loads (SGPRs) -- no LDS traffic, no divergence;
+//   * the column's own phases cos/sin(phi_m) are recomputed on the fly
+//     (FP64 phase, range reduction, then FP32 or FP64 sincos), amortised over
+//     the LC levels x 2 interaction branches that share them.
+// Reference semantics: src/Irregular.cpp (eta1/eta2/u1..w1/uvw2/u2..dp2,
+// phi1_d*dz), src/lsgrid_spline.cpp (column fill, FD gradients, query).
+#include <hip/hip_runtime.h>
+
+#include "engine.hpp"
+#include "kernels.hpp"
+
+namespace cfdwm {
+
+namespace {
+
+constexpr double kInv2Pi = 0.15915494309189533577;
+
+// ---------------------------------------------------------------- helpers
+// Irregular::freq_ramp (Irregular.cpp:200-204)
+__device__ __forceinline__ double freq_ramp(double t, double rt, double ts, double du) {
+    const double a = 1. - ((t - ts - du + rt) / (rt + 1e-12));
+    const double b = (t - ts) / (rt + 1e-12);
+    const double down = a < 1. ? (a < 0. ? 0. : a) : 1.;
+    const double up = b < 1. ? (b < 0. ? 0. : b) : 1.;
+    return up * down;
+}
+
+__device__ __forceinline__ double phase_of(const PhaseRec& p, double dx, double dy, double t) {
+    return p.k * (p.cth * dx + p.sth * dy) - p.omega * t + p.ph0;
+}
+
+__device__ __forceinline__ void cs_of(double ph, double& c, double& s) { sincos(ph, &s, &c); }
+__device__ __forceinline__ void cs_of(double ph, float& c, float& s) {
+    // FP64 range reduction to [-1/2, 1/2] revolutions, then the hardware
+    // sin/cos (argument in revolutions).
+    double r = ph * kInv2Pi;
+    r -= rint(r);
+    const float f = (float)r;
+    s = __builtin_amdgcn_sinf(f);
+    c = __builtin_amdgcn_cosf(f);
+}
+
+// depth profile pieces: E = e^{kz}, G = e^{-k(z+2h)}; cosh ratio = C(E+G), sinh ratio = C(E-G)
+__device__ __forceinline__ void prof(double pk, double pg, double z, double& E, double& G) {
+    E = exp(pk * z);
+    G = exp(fma(-pk, z, pg));
+}
+__device__ __forceinline__ void prof(float pk, float pg, float z, float& E, float& G) {
+    E = __builtin_amdgcn_exp2f(pk * z);
+    G = __builtin_amdgcn_exp2f(fmaf(-pk, z, pg));
+}
+
+template <typename R>
+__device__ __forceinline__ R fmaR(R a, R b, R c) { return a * b + c; }
+
+}  // namespace
+
+// ============================================================ column kernel
+// Per column: eta1, filtered eta2, Taylor gradients (Irregular.cpp:178-293, 1125-1192).
+// GRID: columns are (slot, active column) of the s-grid; writes ETA of the slot.
+// POINTS: columns are points; writes eta1, eta2, gxz, gyz, gzz, gpz (FP64).
+template <typename R, bool GRID>
+__global__ __launch_bounds__(256) void k_columns(SeaDev<R> S, ColumnJob<R> J) {
+    const int64_t tid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t total = J.nslot * J.ncol;
+    const bool live = tid < total;
+    const int64_t q = live ? tid : total - 1;
+    double x, y, t;
+    int64_t col = 0;
+    int slot = 0;
+    if (GRID) {
+        slot = (int)(q / J.ncol);
+        col = J.cols ? J.cols[q % J.ncol] : q % J.ncol;
+        const int i = (int)(col / J.g.ny), j = (int)(col % J.g.ny);
+        x = J.g.x0 + J.g.dx * i;
+        y = J.g.y0 + J.g.dy * j;
+        t = J.slot_t[slot];
+    } else {
+        x = J.x[q]; y = J.y[q]; t = J.t[q];
+    }
+    const double dxp = x - S.x0, dyp = y - S.y0;
+    R e1 = 0, e2 = 0, gx = 0, gy = 0, gz = 0, gp = 0;
+    for (int i = 0; i < S.n; ++i) {
+        const PhaseRec pi = S.ph[i];
+        const CompRec<R>& ci_ = S.cr[i];
+        R ci, si;
+        cs_of(phase_of(pi, dxp, dyp, t), ci, si);
+        const R ri = (R)freq_ramp(t, pi.rt, pi.ts, pi.du);
+        const R cr = ci * ri, sr = si * ri;
+        e1 = fmaR(ci_.A, cr, e1);
+        gx = fmaR(ci_.gxz, cr, gx);
+        gy = fmaR(ci_.gyz, cr, gy);
+        gz = fmaR(ci_.gzz, sr, gz);
+        gp = fmaR(ci_.gpz, cr, gp);
+        if (!J.want_eta2) continue;
+        e2 = fmaR(ci_.e2d, (ci * ci - si * si) * ri, e2);
+        const int mhi = S.fmhi[i];
+        const XRec<R>* __restrict__ xr = S.fx + S.frow[i] - (i + 1);
+        R acc = 0;
+        for (int m = i + 1; m < mhi; ++m) {
+            R cm, sm;
+            cs_of(phase_of(S.ph[m], dxp, dyp, t), cm, sm);
+            const XRec<R> rec = xr[m];
+            const R cc = cr * cm, ss = sr * sm;
+            acc = fmaR(rec.xP, cc - ss, acc);
+            acc = fmaR(rec.xM, cc + ss, acc);
+        }
+        e2 += acc;
+    }
+    if (!live) return;
+    if (GRID) {
+        J.eta_grid[(int64_t)J.slot_dst[slot] * J.ncell + col] = (R)((double)e1 + (double)e2 + S.swl);
+    } else {
+        J.o_eta1[q] = e1; J.o_eta2[q] = e2;
+        J.o_gx[q] = gx; J.o_gy[q] = gy; J.o_gz[q] = gz; J.o_gp[q] = gp;
+    }
+}
+
+// ============================================================= level kernel
+// Per (column, chunk of LC levels): first order + second order + Taylor.
+// GRID (lsgrid_spline.cpp:708-721): z from the sigma map of the column's eta,
+//   profiles at min(0, z), Taylor with max(0, z - swl); filtered uvw2 terms.
+// POINTS: one level per point; flags select the terms (EV_*).
+template <typename R, int LC, bool GRID>
+__global__ __launch_bounds__(256) void k_levels(SeaDev<R> S, LevelJob<R> J) {
+    const int64_t tid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int nchunk = GRID ? (J.g.nl + LC - 1) / LC : 1;
+    const int64_t total = (int64_t)J.nslot * J.ncol * nchunk;
+    const bool live = tid < total;
+    const int64_t q = live ? tid : total - 1;
+    const uint32_t fl = J.flags;
+    double x, y, t;
+    int64_t col = 0;
+    int slot = 0, l0 = 0;
+    R zp[LC], zt[LC];
+    if (GRID) {
+        const int64_t cq = q / nchunk;
+        l0 = (int)(q % nchunk) * LC;
+        slot = (int)(cq / J.ncol);
+        col = J.cols ? J.cols[cq % J.ncol] : cq % J.ncol;
+        const int i = (int)(col / J.g.ny), j = (int)(col % J.g.ny);
+        x = J.g.x0 + J.g.dx * i;
+        y = J.g.y0 + J.g.dy * j;
+        t = J.slot_t[slot];
+        const double eta = (double)J.eta_grid[(int64_t)J.slot_dst[slot] * J.ncell + col];
+        const double e = eta - J.g.swl;
+#pragma unroll
+        for (int l = 0; l < LC; ++l) {
+            const int lv = l0 + l < J.g.nl ? l0 + l : J.g.nl - 1;
+            const double z = e + J.sigma[lv] * (J.g.h + e);           // s2z (lsgrid_spline.cpp:85-87)
+            zp[l] = (R)(z < 0. ? z : 0.);                               // min(0, z), no -swl (:711)
+            zt[l] = (R)(z - J.g.swl > 0. ? z - J.g.swl : 0.);          // max(0, z - swl)
+        }
+    } else {
+        x = J.x[q]; y = J.y[q]; t = J.t[q];
+        const double z = J.z[q];
+        double zz = z;
+        if (fl & EV_ZCLAMP) zz = (z - S.swl) < 0. ? (z - S.swl) : 0.;
+        else if (fl & EV_ZSHIFT) zz = z - S.swl;
+        zp[0] = (R)zz;
+        zt[0] = (R)((z - S.swl) > 0. ? (z - S.swl) : 0.);
+#pragma unroll
+        for (int l = 1; l < LC; ++l) { zp[l] = zp[0]; zt[l] = zt[0]; }
+    }
+    const bool direct = (fl & EV_DIRECT) != 0;
+    const bool wantP = (fl & EV_P) != 0;
+    const double dxp = x - S.x0, dyp = y - S.y0;
+    R U[LC], V[LC], W[LC], P[LC], P2[LC];
+#pragma unroll
+    for (int l = 0; l < LC; ++l) { U[l] = V[l] = W[l] = P[l] = P2[l] = 0; }
+    R gx = 0, gy = 0, gz = 0, gp = 0;
+    // kinetic running sums of the unfiltered dp2 (separable in i and m)
+    R kx[LC], ky[LC], kz[LC];
+#pragma unroll
+    for (int l = 0; l < LC; ++l) { kx[l] = ky[l] = kz[l] = 0; }
+
+    const UvwRec<R>* __restrict__ tab = direct ? S.duvw : S.fuvw;
+    // rows run backwards so that the dp2 kinetic sums over m > i are running sums
+    for (int i = S.n - 1; i >= 0; --i) {
+        const PhaseRec pi = S.ph[i];
+        const CompRec<R> c = S.cr[i];
+        R ci, si;
+        cs_of(phase_of(pi, dxp, dyp, t), ci, si);
+        const R ri = (R)freq_ramp(t, pi.rt, pi.ts, pi.du);
+        const R cr = ci * ri, sr = si * ri;
+        gx = fmaR(c.gxz, cr, gx);
+        gy = fmaR(c.gyz, cr, gy);
+        gz = fmaR(c.gzz, sr, gz);
+        gp = fmaR(c.gpz, cr, gp);
+        const bool sec = (fl & EV_SEC) && (direct || S.rowActive[i]);
+        if ((fl & EV_LIN) || (sec && direct && wantP)) {
+#pragma unroll
+            for (int l = 0; l < LC; ++l) {
+                R E, G;
+                prof(c.pk, c.pg, zp[l], E, G);
+                const R ch = E + G, sh = E - G;
+                if (fl & EV_LIN) {
+                    U[l] = fmaR(c.lu * cr, ch, U[l]);
+                    V[l] = fmaR(c.lv * cr, ch, V[l]);
+                    W[l] = fmaR(c.lw * sr, sh, W[l]);
+                    if (wantP) P[l] = fmaR(c.lp * cr, ch, P[l]);
+                }
+                if (sec && direct && wantP) {
+                    // -(Xx_i Xx_m + Xy_i Xy_m) px_i px_m + Z_i Z_m pz_i pz_m, summed over m > i
+                    P2[l] += ri * (-ch * (c.Xx * kx[l] + c.Xy * ky[l]) + sh * c.Zk * kz[l]);
+                    P2[l] += ri * (-c.kd1 * ch + c.kd2 * sh);     // diagonal "second term"
+                    kx[l] = fmaR(c.Xx, ch, kx[l]);
+                    ky[l] = fmaR(c.Xy, ch, ky[l]);
+                    kz[l] = fmaR(c.Zk, sh, kz[l]);
+                }
+            }
+        }
+        if (!sec) continue;
+        // diagonal self-interaction (profile of 2k)
+        {
+            const R c2 = (ci * ci - si * si) * ri, s2 = (R)2 * si * ci * ri;
+            const R du = direct ? c.ddu : c.dfu, dv = direct ? c.ddv : c.dfv, dw = direct ? c.ddw : c.dfw;
+#pragma unroll
+            for (int l = 0; l < LC; ++l) {
+                R E, G;
+                prof(c.p2k, c.p2g, zp[l], E, G);
+                const R ch = E + G, sh = E - G;
+                U[l] = fmaR(du * c2, ch, U[l]);
+                V[l] = fmaR(dv * c2, ch, V[l]);
+                W[l] = fmaR(dw * s2, sh, W[l]);
+                if (direct && wantP) P2[l] = fmaR(c.ddp * c2, ch, P2[l]);
+            }
+        }
+        // pair terms, m in (i, mhi)
+        const int mhi = direct ? S.n : S.fmhi[i];
+        const int64_t base = (direct ? S.drow[i] : S.frow[i]) - (i + 1);
+        const UvwRec<R>* __restrict__ rr = tab + base;
+        const XRec<R>* __restrict__ xr = S.dx + base;
+        for (int m = i + 1; m < mhi; ++m) {
+            R cm, sm;
+            cs_of(phase_of(S.ph[m], dxp, dyp, t), cm, sm);
+            const UvwRec<R> r = rr[m];
+            const R cc = cr * cm, ss = sr * sm, sc = sr * cm, cs = cr * sm;
+            const R cP = cc - ss, cM = cc + ss, sP = sc + cs, sM = sc - cs;
+            const R uP = r.cuP * cP, vP = r.cvP * cP, wP = r.cwP * sP;
+            const R uM = r.cuM * cM, vM = r.cvM * cM, wM = r.cwM * sM;
+            R pP = 0, pM = 0;
+            if (direct && wantP) {
+                const XRec<R> xq = xr[m];
+                pP = xq.xP * cP;
+                pM = xq.xM * cM;
+            }
+#pragma unroll
+            for (int l = 0; l < LC; ++l) {
+                R E, G;
+                prof(r.pkP, r.pgP, zp[l], E, G);
+                R ch = E + G, sh = E - G;
+                U[l] = fmaR(uP, ch, U[l]);
+                V[l] = fmaR(vP, ch, V[l]);
+                W[l] = fmaR(wP, sh, W[l]);
+                if (direct && wantP) P2[l] = fmaR(pP, ch, P2[l]);
+                prof(r.pkM, r.pgM, zp[l], E, G);
+                ch = E + G; sh = E - G;
+                U[l] = fmaR(uM, ch, U[l]);
+                V[l] = fmaR(vM, ch, V[l]);
+                W[l] = fmaR(wM, sh, W[l]);
+                if (direct && wantP) P2[l] = fmaR(pM, ch, P2[l]);
+            }
+        }
+    }
+    if (!live) return;
+    const bool tay = (fl & EV_TAYLOR) != 0;
+    if (GRID) {
+        const int64_t o = ((int64_t)J.slot_dst[slot] * J.ncell + col) * J.g.nl + l0;
+#pragma unroll
+        for (int l = 0; l < LC; ++l) {
+            if (l0 + l >= J.g.nl) break;
+            J.o_u[o + l] = (R)((double)U[l] + (double)gx * (double)zt[l]);
+            J.o_v[o + l] = (R)((double)V[l] + (double)gy * (double)zt[l]);
+            J.o_w[o + l] = (R)((double)W[l] + (double)gz * (double)zt[l]);
+        }
+    } else {
+        const double z1 = (double)zt[0];
+        J.p_u[q] = (double)U[0] + (tay ? (double)gx * z1 : 0.);
+        J.p_v[q] = (double)V[0] + (tay ? (double)gy * z1 : 0.);
+        J.p_w[q] = (double)W[0] + (tay ? (double)gz * z1 : 0.);
+        if (wantP) J.p_p[q] = (double)P[0] + S.rho * (double)P2[0] + (tay ? (double)gp * z1 : 0.);
+    }
+}
+
+// ======================================================== FD gradient kernels
+// update_gradient_eta_dxdy / update_gradient_dxdydz (lsgrid_spline.cpp:163-366):
+// central differences inside, one-sided at the edges, zero for a single point.
+// Output slot o (0, 1) holds ring slot tids[o].
+template <typename R>
+__global__ __launch_bounds__(256) void k_grad_xyz(const R* __restrict__ D, R* __restrict__ GX,
+                                                  R* __restrict__ GY, R* __restrict__ GS,
+                                                  int nx, int ny, int L, int tid0, int tid1,
+                                                  double dx, double dy, double ds) {
+    const int64_t per = (int64_t)nx * ny * L;
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= 2 * per) return;
+    const int o = (int)(e / per);
+    const int64_t qq = e % per;
+    const int m = (int)(qq % L);
+    const int j = (int)((qq / L) % ny);
+    const int i = (int)(qq / ((int64_t)L * ny));
+    const R* d = D + (int64_t)(o ? tid1 : tid0) * per;
+    const int64_t sx = (int64_t)ny * L, sy = L;
+    R vx, vy;
+    if (nx == 1) vx = 0;
+    else if (i == 0) vx = (R)((d[qq + sx] - d[qq]) / dx);
+    else if (i == nx - 1) vx = (R)((d[qq] - d[qq - sx]) / dx);
+    else vx = (R)((d[qq + sx] - d[qq - sx]) / (2 * dx));
+    if (ny == 1) vy = 0;
+    else if (j == 0) vy = (R)((d[qq + sy] - d[qq]) / dy);
+    else if (j == ny - 1) vy = (R)((d[qq] - d[qq - sy]) / dy);
+    else vy = (R)((d[qq + sy] - d[qq - sy]) / (2 * dy));
+    GX[o * per + qq] = vx;
+    GY[o * per + qq] = vy;
+    if (GS) {
+        R vs;
+        if (L == 1) vs = 0;
+        else if (m == 0) vs = (R)((d[qq + 1] - d[qq]) / ds);
+        else if (m == L - 1) vs = (R)((d[qq] - d[qq - 1]) / ds);
+        else vs = (R)((d[qq + 1] - d[qq - 1]) / (2 * ds));
+        GS[o * per + qq] = vs;
+    }
+}
+
+// update_gradient_dt / update_gradient_eta_dt (lsgrid_spline.cpp:124-161)
+template <typename R>
+__global__ __launch_bounds__(256) void k_grad_dt(const R* __restrict__ D, R* __restrict__ GD,
+                                                 int64_t per, int tstep, double dt) {
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= per) return;
+    if (tstep > 0) {
+        const int tid = (tstep + 2) % 4, t1 = (tstep + 3) % 4, t0 = (tstep + 1) % 4;
+        GD[tid * per + e] = (R)((D[t1 * per + e] - D[t0 * per + e]) / (2 * dt));
+    } else {
+        GD[1 * per + e] = (R)((D[2 * per + e] - D[1 * per + e]) / dt);
+        GD[2 * per + e] = (R)((D[3 * per + e] - D[1 * per + e]) / (2 * dt));
+    }
+}
+
+// ================================================================ spline query
+namespace {
+template <typename R>
+__device__ __forceinline__ R herm(R f0, R f1, R a, R b, R u) {
+    return f0 * ((R)1 - u) + f1 * u + u * ((R)1 - u) * (a * ((R)1 - u) + b * u);
+}
+template <typename R>
+__device__ __forceinline__ R lin(R f0, R f1, R u) { return f0 * ((R)1 - u) + f1 * u; }
+
+// the 8 corners (i|i+1, j|j+1, l|l+1) with edge clamps (lsgrid_spline.cpp:375-384)
+struct Corners {
+    int64_t a[8];
+};
+__device__ __forceinline__ Corners corners(int nx, int ny, int nl, int i, int j, int l) {
+    const int i1 = min(i + 1, nx - 1), j1 = min(j + 1, ny - 1), l1 = min(l + 1, nl - 1);
+    Corners c;
+    const int64_t I0 = (int64_t)i * ny, I1 = (int64_t)i1 * ny;
+    c.a[0] = (I0 + j) * nl + l;  c.a[1] = (I0 + j1) * nl + l;
+    c.a[2] = (I1 + j) * nl + l;  c.a[3] = (I1 + j1) * nl + l;
+    c.a[4] = (I0 + j) * nl + l1; c.a[5] = (I0 + j1) * nl + l1;
+    c.a[6] = (I1 + j) * nl + l1; c.a[7] = (I1 + j1) * nl + l1;
+    return c;
+}
+
+// cart2sigmaS (lsgrid_spline.cpp:107-112).  Points in (eta - swl, eta] give
+// sigma > 0 when swl > 0, where the reference indexes with a NaN; they are
+// mapped to the surface here.
+__device__ __forceinline__ double cart2sigmaS(double zpt, double e, double h, double swl,
+                                              double ta, double tb, double tanab) {
+    const double z = zpt < e ? zpt : e;
+    double sigma = (z - (e - swl)) / (h + (e - swl));
+    sigma = sigma < -1. ? -1. : sigma;
+    sigma = sigma > 0. ? 0. : sigma;
+    return 1 - atan(pow(-sigma * tanab, 1. / tb)) / ta;
+}
+
+// spline_interp_velo (lsgrid_spline.cpp:386-502)
+template <typename R>
+__device__ R spline_velo(const SplineField<R>& F, int nx, int ny, int nl, int64_t per, int i, int j,
+                         int l0, int l1, R xd, R yd, R s0, R s1, R td, int tid1, int tid2,
+                         R dx, R dy, R ds, R dt) {
+    const Corners A = corners(nx, ny, nl, i, j, l0), B = corners(nx, ny, nl, i, j, l1);
+    R C[8], D[8], Ct[8], Dt[8], Cx[8], Dx[8], Cy[8], Dy[8], Cs[8], Ds[8];
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+        C[q] = F.f[tid1 * per + A.a[q]];   D[q] = F.f[tid2 * per + B.a[q]];
+        Ct[q] = F.fdt[tid1 * per + A.a[q]]; Dt[q] = F.fdt[tid2 * per + B.a[q]];
+        Cx[q] = F.fdx[A.a[q]];             Dx[q] = F.fdx[per + B.a[q]];
+        Cy[q] = F.fdy[A.a[q]];             Dy[q] = F.fdy[per + B.a[q]];
+        Cs[q] = F.fds[A.a[q]];             Ds[q] = F.fds[per + B.a[q]];
+    }
+    const int lo[4] = {0, 1, 4, 5}, hi[4] = {2, 3, 6, 7};
+    R Cxr[4], Dxr[4], Cyr[4], Dyr[4], Ctr[4], Dtr[4], Csr[4], Dsr[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const R ac = dx * Cx[lo[q]] - (C[hi[q]] - C[lo[q]]);
+        const R bc = -dx * Cx[hi[q]] + (C[hi[q]] - C[lo[q]]);
+        const R ad = dx * Dx[lo[q]] - (D[hi[q]] - D[lo[q]]);
+        const R bd = -dx * Dx[hi[q]] + (D[hi[q]] - D[lo[q]]);
+        Cxr[q] = herm(C[lo[q]], C[hi[q]], ac, bc, xd);
+        Dxr[q] = herm(D[lo[q]], D[hi[q]], ad, bd, xd);
+        Cyr[q] = lin(Cy[lo[q]], Cy[hi[q]], xd); Dyr[q] = lin(Dy[lo[q]], Dy[hi[q]], xd);
+        Ctr[q] = lin(Ct[lo[q]], Ct[hi[q]], xd); Dtr[q] = lin(Dt[lo[q]], Dt[hi[q]], xd);
+        Csr[q] = lin(Cs[lo[q]], Cs[hi[q]], xd); Dsr[q] = lin(Ds[lo[q]], Ds[hi[q]], xd);
+    }
+    R Cyv[2], Dyv[2], Csv[2], Dsv[2], Ctv[2], Dtv[2];
+#pragma unroll
+    for (int q = 0; q < 2; ++q) {
+        const int a0 = 2 * q, a1 = 2 * q + 1;
+        const R ac = dy * Cyr[a0] - (Cxr[a1] - Cxr[a0]);
+        const R bc = -dy * Cyr[a1] + (Cxr[a1] - Cxr[a0]);
+        const R ad = dy * Dyr[a0] - (Dxr[a1] - Dxr[a0]);
+        const R bd = -dy * Dyr[a1] + (Dxr[a1] - Dxr[a0]);
+        Cyv[q] = herm(Cxr[a0], Cxr[a1], ac, bc, yd);
+        Dyv[q] = herm(Dxr[a0], Dxr[a1], ad, bd, yd);
+        Csv[q] = lin(Csr[a0], Csr[a1], yd); Dsv[q] = lin(Dsr[a0], Dsr[a1], yd);
+        Ctv[q] = lin(Ctr[a0], Ctr[a1], yd); Dtv[q] = lin(Dtr[a0], Dtr[a1], yd);
+    }
+    const R a0 = ds * Csv[0] - (Cyv[1] - Cyv[0]), b0 = -ds * Csv[1] + (Cyv[1] - Cyv[0]);
+    const R a1 = ds * Dsv[0] - (Dyv[1] - Dyv[0]), b1 = -ds * Dsv[1] + (Dyv[1] - Dyv[0]);
+    const R CC = herm(Cyv[0], Cyv[1], a0, b0, s0), DD = herm(Dyv[0], Dyv[1], a1, b1, s1);
+    const R CCt = lin(Ctv[0], Ctv[1], s0), DDt = lin(Dtv[0], Dtv[1], s1);
+    const R a = dt * CCt - (DD - CC), b = -dt * DDt + (DD - CC);
+    return herm(CC, DD, a, b, td);
+}
+}  // namespace
+
+// get_kinematics_at_point (lsgrid_spline.cpp:505-605).  The cell fractions
+// follow std::modf(double, float*): the coordinate is rounded to float and the
+// fraction taken in float.  TQ: coordinate/output type of the queries.
+template <typename R, typename TQ>
+__global__ __launch_bounds__(256) void k_query(QueryJob<R, TQ> J) {
+    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= J.n) return;
+    const GridGeom& g = J.g;
+    const double tpt = J.t ? (double)J.t[p] : J.tconst;
+    const double xpt = (double)J.x[p], ypt = (double)J.y[p], zpt = (double)J.z[p];
+    const int tid1 = (J.tstep + 1) % 4, tid2 = (J.tstep + 2) % 4;
+    double cx = (xpt - g.x0) / g.dx;
+    cx = cx < (g.nx - 1.) ? cx : (g.nx - 1.);
+    cx = cx > 0. ? cx : 0.;
+    double cy = (ypt - g.y0) / g.dy;
+    cy = cy < (g.ny - 1.) ? cy : (g.ny - 1.);
+    cy = cy > 0. ? cy : 0.;
+    const float fx = (float)cx, fy = (float)cy;
+    const float ix = truncf(fx), iy = truncf(fy);
+    const R xd = (R)(fx - ix), yd = (R)(fy - iy);
+    double td = (tpt - J.t0) / J.dt;
+    td = td > 0. ? td : 0.;
+    td = td < 1. ? td : 1.;
+    const int i = (int)ix, j = (int)iy;
+    const int nx = g.nx, ny = g.ny, nl = g.nl;
+    const int64_t pc = (int64_t)nx * ny;
+    const int j1 = min(j + 1, ny - 1), i1 = min(i + 1, nx - 1);
+    const int64_t a0 = (int64_t)i * ny + j, a1 = (int64_t)i * ny + j1;
+    const int64_t a2 = (int64_t)i1 * ny + j, a3 = (int64_t)i1 * ny + j1;
+    const R* E = J.eta.f;
+    const R C[4] = {E[tid1 * pc + a0], E[tid1 * pc + a1], E[tid1 * pc + a2], E[tid1 * pc + a3]};
+    const R D[4] = {E[tid2 * pc + a0], E[tid2 * pc + a1], E[tid2 * pc + a2], E[tid2 * pc + a3]};
+    const R* Et = J.eta.fdt;
+    const R Ct[4] = {Et[tid1 * pc + a0], Et[tid1 * pc + a1], Et[tid1 * pc + a2], Et[tid1 * pc + a3]};
+    const R Dt[4] = {Et[tid2 * pc + a0], Et[tid2 * pc + a1], Et[tid2 * pc + a2], Et[tid2 * pc + a3]};
+    const R* Ex = J.eta.fdx;
+    const R* Ey = J.eta.fdy;
+    const R Cx[4] = {Ex[a0], Ex[a1], Ex[a2], Ex[a3]}, Dx[4] = {Ex[pc + a0], Ex[pc + a1], Ex[pc + a2], Ex[pc + a3]};
+    const R Cy[4] = {Ey[a0], Ey[a1], Ey[a2], Ey[a3]}, Dy[4] = {Ey[pc + a0], Ey[pc + a1], Ey[pc + a2], Ey[pc + a3]};
+    const R dx = (R)g.dx, dy = (R)g.dy, dt = (R)J.dt;
+    const R a00 = dx * Cx[0] - (C[2] - C[0]), a01 = dx * Cx[1] - (C[3] - C[1]);
+    const R b00 = -dx * Cx[2] + (C[2] - C[0]), b01 = -dx * Cx[3] + (C[3] - C[1]);
+    const R a10 = dx * Dx[0] - (D[2] - D[0]), a11 = dx * Dx[1] - (D[3] - D[1]);
+    const R b10 = -dx * Dx[2] + (D[2] - D[0]), b11 = -dx * Dx[3] + (D[3] - D[1]);
+    const R C0 = herm(C[0], C[2], a00, b00, xd), C1 = herm(C[1], C[3], a01, b01, xd);
+    const R D0 = herm(D[0], D[2], a10, b10, xd), D1 = herm(D[1], D[3], a11, b11, xd);
+    const R C0y = lin(Cy[0], Cy[2], xd), C1y = lin(Cy[1], Cy[3], xd);
+    const R D0y = lin(Dy[0], Dy[2], xd), D1y = lin(Dy[1], Dy[3], xd);
+    const R C0t = lin(Ct[0], Ct[2], xd), C1t = lin(Ct[1], Ct[3], xd);
+    const R D0t = lin(Dt[0], Dt[2], xd), D1t = lin(Dt[1], Dt[3], xd);
+    const R ya0 = dy * C0y - (C1 - C0), yb0 = -dy * C1y + (C1 - C0);
+    const R ya1 = dy * D0y - (D1 - D0), yb1 = -dy * D1y + (D1 - D0);
+    const R e0 = herm(C0, C1, ya0, yb0, yd), e1 = herm(D0, D1, ya1, yb1, yd);
+    const R CCt = lin(C0t, C1t, yd), DDt = lin(D0t, D1t, yd);
+    const R ta = dt * CCt - (e1 - e0), tb = -dt * DDt + (e1 - e0);
+    const R tdR = (R)td;
+    const R eta = herm(e0, e1, ta, tb, tdR);
+    const double sp0 = cart2sigmaS(zpt, (double)e0, g.h, g.swl, J.tan_a, J.tan_b, J.tanab);
+    const double sp1 = cart2sigmaS(zpt, (double)e1, g.h, g.swl, J.tan_a, J.tan_b, J.tanab);
+    const float f0 = (float)(sp0 / g.ds), f1 = (float)(sp1 / g.ds);
+    const float k0 = truncf(f0), k1 = truncf(f1);
+    const R sd0 = (R)(f0 - k0), sd1 = (R)(f1 - k1);
+    int l0 = (int)k0, l1 = (int)k1;
+    l0 = l0 < 0 ? 0 : (l0 > nl - 1 ? nl - 1 : l0);
+    l1 = l1 < 0 ? 0 : (l1 > nl - 1 ? nl - 1 : l1);
+    const int64_t per = pc * nl;
+    const R ds = (R)g.ds;
+    const R ux = spline_velo(J.u, nx, ny, nl, per, i, j, l0, l1, xd, yd, sd0, sd1, tdR, tid1, tid2, dx, dy, ds, dt);
+    const R vy = spline_velo(J.v, nx, ny, nl, per, i, j, l0, l1, xd, yd, sd0, sd1, tdR, tid1, tid2, dx, dy, ds, dt);
+    const R wz = spline_velo(J.w, nx, ny, nl, per, i, j, l0, l1, xd, yd, sd0, sd1, tdR, tid1, tid2, dx, dy, ds, dt);
+    const double rmp = J.ramp_scale ? (double)J.ramp_scale[p] : 1.0;
+    J.o_eta[p] = (TQ)((double)eta * rmp);
+    J.o_u[p] = (TQ)((double)ux * rmp);
+    J.o_v[p] = (TQ)((double)vy * rmp);
+    J.o_w[p] = (TQ)((double)wz * rmp);
+}
+
+// ================================================================ launchers
+namespace {
+inline unsigned nblk(int64_t n, int b = 256) { return (unsigned)((n + b - 1) / b); }
+}
+
+template <typename R>
+hipError_t launch_columns(const SeaDev<R>& S, const ColumnJob<R>& J, bool grid, hipStream_t st) {
+    const int64_t n = J.nslot * J.ncol;
+    if (n <= 0) return hipSuccess;
+    if (grid) hipLaunchKernelGGL((k_columns<R, true>), dim3(nblk(n)), dim3(256), 0, st, S, J);
+    else hipLaunchKernelGGL((k_columns<R, false>), dim3(nblk(n)), dim3(256), 0, st, S, J);
+    return hipGetLastError();
+}
+
+template <typename R, int LC>
+static hipError_t launch_levels_lc(const SeaDev<R>& S, const LevelJob<R>& J, hipStream_t st) {
+    const int nchunk = (J.g.nl + LC - 1) / LC;
+    const int64_t n = (int64_t)J.nslot * J.ncol * nchunk;
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL((k_levels<R, LC, true>), dim3(nblk(n)), dim3(256), 0, st, S, J);
+    return hipGetLastError();
+}
+
+template <typename R>
+hipError_t launch_levels(const SeaDev<R>& S, const LevelJob<R>& J, bool grid, int lc, hipStream_t st) {
+    if (!grid) {
+        const int64_t n = (int64_t)J.nslot * J.ncol;
+        if (n <= 0) return hipSuccess;
+        hipLaunchKernelGGL((k_levels<R, 1, false>), dim3(nblk(n)), dim3(256), 0, st, S, J);
+        return hipGetLastError();
+    }
+    switch (lc) {
+    case 1: return launch_levels_lc<R, 1>(S, J, st);
+    case 2: return launch_levels_lc<R, 2>(S, J, st);
+    case 4: return launch_levels_lc<R, 4>(S, J, st);
+    case 8: return launch_levels_lc<R, 8>(S, J, st);
+    default: return launch_levels_lc<R, 16>(S, J, st);
+    }
+}
+
+template <typename R>
+hipError_t launch_grad_xyz(const R* D, R* GX, R* GY, R* GS, int nx, int ny, int L, int tid0, int tid1,
+                           double dx, double dy, double ds, hipStream_t st) {
+    const int64_t n = 2LL * nx * ny * L;
+    hipLaunchKernelGGL((k_grad_xyz<R>), dim3(nblk(n)), dim3(256), 0, st, D, GX, GY, GS, nx, ny, L,
+                       tid0, tid1, dx, dy, ds);
+    return hipGetLastError();
+}
+
+template <typename R>
+hipError_t launch_grad_dt(const R* D, R* GD, int64_t per, int tstep, double dt, hipStream_t st) {
+    hipLaunchKernelGGL((k_grad_dt<R>), dim3(nblk(per)), dim3(256), 0, st, D, GD, per, tstep, dt);
+    return hipGetLastError();
+}
+
+template <typename R, typename TQ>
+hipError_t launch_query(const QueryJob<R, TQ>& J, hipStream_t st) {
+    if (J.n <= 0) return hipSuccess;
+    hipLaunchKernelGGL((k_query<R, TQ>), dim3(nblk(J.n)), dim3(256), 0, st, J);
+    return hipGetLastError();
+}
+
+template hipError_t launch_columns<double>(const SeaDev<double>&, const ColumnJob<double>&, bool, hipStream_t);
+template hipError_t launch_columns<float>(const SeaDev<float>&, const ColumnJob<float>&, bool, hipStream_t);
+template hipError_t launch_levels<double>(const SeaDev<double>&, const LevelJob<double>&, bool, int, hipStream_t);
+template hipError_t launch_levels<float>(const SeaDev<float>&, const LevelJob<float>&, bool, int, hipStream_t);
+template hipError_t launch_grad_xyz<double>(const double*, double*, double*, double*, int, int, int, int, int, double, double, double, hipStream_t);
+template hipError_t launch_grad_xyz<float>(const float*, float*, float*, float*, int, int, int, int, int, double, double, double, hipStream_t);
+template hipError_t launch_grad_dt<double>(const double*, double*, int64_t, int, double, hipStream_t);
+template hipError_t launch_grad_dt<float>(const float*, float*, int64_t, int, double, hipStream_t);
+template hipError_t launch_query<double, double>(const QueryJob<double, double>&, hipStream_t);
+template hipError_t launch_query<float, double>(const QueryJob<float, double>&, hipStream_t);
+template hipError_t launch_query<double, float>(const QueryJob<double, float>&, hipStream_t);
+template hipError_t launch_query<float, float>(const QueryJob<float, float>&, hipStream_t);
+
+}  // namespace cfdwm

@@ -1,0 +1,74 @@
+// Device data layout shared by the HIP kernels (kernels.hip) and the engine.
+#pragma once
+#include <cstdint>
+
+namespace cfdwm {
+
+// Per-component phase data, FP64 in both precisions: the phase
+// k*(cth*(x-x0) + sth*(y-y0)) - omega*t + ph0 is always accumulated and
+// range-reduced in FP64 (src/Irregular.cpp:237-240).
+struct PhaseRec {
+    double k, cth, sth, omega, ph0, rt, ts, du;
+};
+
+// Per-component coefficients in the working precision R.  Depth profiles
+// are evaluated as E = e^{pk z}, G = e^{-pk z + pg} (FP64: natural base,
+// FP32: base 2 with pk = k log2 e), so cosh-ratio = C (E + G) and
+// sinh-ratio = C (E - G) with C folded into the coefficients.
+template <typename R>
+struct CompRec {
+    R lu, lv, lw, lp;          // u1, v1, w1, dp1
+    R gxz, gyz, gzz, gpz;      // Taylor gradients at z = 0
+    R A, e2d;                  // eta1 amplitude, filtered eta2 diagonal
+    R dfu, dfv, dfw;           // filtered (uvw2) diagonal
+    R ddu, ddv, ddw, ddp;      // unfiltered (u2..dp2) diagonal
+    R Xx, Xy, Zk, kd1, kd2;    // dp2 kinetic terms
+    R pk, pg, p2k, p2g;        // profile exponents for k and 2k
+    R pad[2];
+};
+
+// Per-pair velocity record (sum branch P, difference branch M).
+template <typename R>
+struct UvwRec {
+    R cuP, cvP, cwP, pkP, pgP, cuM, cvM, cwM, pkM, pgM;
+};
+// Per-pair scalar coefficients: eta (filtered) or pressure (unfiltered).
+template <typename R>
+struct XRec {
+    R xP, xM;
+};
+
+template <typename R>
+struct SeaDev {
+    int n;
+    double h, swl, x0, y0, rho;
+    const PhaseRec* __restrict__ ph;
+    const CompRec<R>* __restrict__ cr;
+    const uint8_t* __restrict__ rowActive;
+    const int64_t* __restrict__ frow;   // filtered table
+    const int32_t* __restrict__ fmhi;
+    const UvwRec<R>* __restrict__ fuvw;
+    const XRec<R>* __restrict__ fx;
+    const int64_t* __restrict__ drow;   // unfiltered table (rows cover m in [i+1, n))
+    const UvwRec<R>* __restrict__ duvw;
+    const XRec<R>* __restrict__ dx;
+};
+
+// Flags of the point evaluator.
+enum : uint32_t {
+    EV_LIN = 1u,        // first-order u1, v1, w1 (and dp1 with EV_P)
+    EV_SEC = 2u,        // second-order pair sums
+    EV_DIRECT = 4u,     // unfiltered u2/v2/w2/dp2 semantics (else filtered uvw2)
+    EV_TAYLOR = 8u,     // + gradient at z=0 times max(0, z - swl)
+    EV_P = 16u,         // pressure
+    EV_ZCLAMP = 32u,    // profiles at min(0, z - swl) (order 1 and 2)
+    EV_ZSHIFT = 64u,    // profiles at z - swl (order 0)
+};
+
+// s-grid geometry and ring-buffer slot bookkeeping (lsGridSpline).
+struct GridGeom {
+    int nx, ny, nl;
+    double x0, y0, dx, dy, ds, h, swl;
+};
+
+}  // namespace cfdwm

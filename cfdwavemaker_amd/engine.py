@@ -1,0 +1,172 @@
+"""Object interface over libcfdwm: sea states and s-grids resident on the GPU.
+
+``Sea`` mirrors the reference's ``Irregular`` (src/Irregular.h:47-193):
+``eval(kind, t, x, y, z)`` evaluates the member of that name (eta1, eta2, u1,
+..., uvw2) for arrays of points.  ``Grid`` mirrors ``lsGridSpline``
+(src/lsgrid_spline.h:15-129): construction runs initialize_kinematics,
+``force_update`` is wave_force_update, ``query`` is get_kinematics_at_point.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from ._lib import GridDescC, SeaParams, check, load
+
+KINDS = ["eta1", "eta2", "eta", "u1", "v1", "w1", "dp1", "phi1_dxdz", "phi1_dydz",
+         "phi1_dzdz", "dp1_dz", "u2", "v2", "w2", "dp2", "u", "v", "w", "dp",
+         "uvw2_u", "uvw2_v", "uvw2_w"]
+KIND = {k: i for i, k in enumerate(KINDS)}
+GRID_FIELDS = ["ETA", "ETAdx", "ETAdy", "ETAdt", "UX", "UY", "UZ", "Udt", "Vdt", "Wdt",
+               "Udx", "Udy", "Uds", "Vdx", "Vdy", "Vds", "Wdx", "Wdy", "Wds", "IGNORE"]
+FP64, FP32 = 0, 1
+
+
+def _f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+class Sea:
+    """A sea state (component arrays sorted by omega) on the GPU."""
+
+    def __init__(self, sea: dict, precision: int = FP64, with_direct: bool = True):
+        self.L = load()
+        self.precision = precision
+        n = len(sea["omega"])
+        arrs = [_f64(sea[k]) for k in ("omega", "A", "k", "phase", "theta")]
+        ramps = [None if sea.get(k) is None else _f64(sea[k]) for k in ("ramptime", "starttime", "duration")]
+        p = SeaParams()
+        p.depth = float(sea.get("depth", 300.0)); p.g = float(sea.get("g", 9.81))
+        p.rho = float(sea.get("rho", 1.0)); p.swl = float(sea.get("swl", 0.0))
+        p.mtheta = float(sea.get("mtheta", 0.0)); p.tofmax = float(sea.get("tofmax", 0.0))
+        p.x0 = float(sea.get("x0", 0.0)); p.y0 = float(sea.get("y0", 0.0))
+        p.bandwidth = float(sea.get("bandwidth", 100.0)); p.cutoff = float(sea.get("cutoff", 100000.0))
+        p.ampl = float(sea.get("ampl", 1.0)); p.order = int(sea.get("order", 2))
+        p.normalize = int(sea.get("normalize", 0)); p.bw_auto = int(sea.get("bw_auto", 0))
+        p.precision = int(precision); p.with_direct = int(bool(with_direct))
+        h = C.c_void_p()
+        check(self.L.cfdwm_sea_create(C.byref(h), n, *[_p(a) for a in arrs], *[_p(a) for a in ramps],
+                                      C.byref(p)), "cfdwm_sea_create")
+        self.h = h
+        self.n = n
+        self.depth = p.depth
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.cfdwm_sea_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def terms(self):
+        a, b, c = C.c_int64(), C.c_int64(), C.c_int64()
+        self.L.cfdwm_sea_terms(self.h, C.byref(a), C.byref(b), C.byref(c))
+        return a.value, b.value, c.value
+
+    def eval(self, kind, t, x, y, z=None):
+        t, x, y = np.broadcast_arrays(np.asarray(t, float), np.asarray(x, float), np.asarray(y, float))
+        shape = t.shape
+        t, x, y = (_f64(a.ravel()) for a in (t, x, y))
+        zz = None if z is None else _f64(np.broadcast_to(np.asarray(z, float), shape).ravel())
+        out = np.zeros(t.size)
+        k = KIND[kind] if isinstance(kind, str) else int(kind)
+        check(self.L.cfdwm_eval(self.h, k, t.size, _p(t), _p(x), _p(y), _p(zz), _p(out)), f"cfdwm_eval({kind})")
+        return out.reshape(shape)
+
+    def grid(self, domain, nx, ny, nl, t0, dt, tan_a, tan_b, water_depth, swl=0.0,
+             domain_ignore=(1e12, -1e12, 1e12, -1e12), ignore_at_init=0) -> "Grid":
+        return Grid(self, domain, nx, ny, nl, t0, dt, tan_a, tan_b, water_depth, swl, domain_ignore,
+                    ignore_at_init)
+
+
+class Grid:
+    def __init__(self, sea: Sea, domain, nx, ny, nl, t0, dt, tan_a, tan_b, water_depth, swl=0.0,
+                 domain_ignore=(1e12, -1e12, 1e12, -1e12), ignore_at_init=0):
+        self.sea, self.L = sea, sea.L
+        d = GridDescC()
+        for q in range(4):
+            d.domain[q] = float(domain[q])
+            d.domain_ignore[q] = float(domain_ignore[q])
+        d.nx, d.ny, d.nl = int(nx), int(ny), int(nl)
+        d.t0, d.dt, d.tan_a, d.tan_b = float(t0), float(dt), float(tan_a), float(tan_b)
+        d.water_depth, d.swl, d.ignore_at_init = float(water_depth), float(swl), int(ignore_at_init)
+        h = C.c_void_p()
+        check(self.L.cfdwm_grid_create(C.byref(h), sea.h, C.byref(d)), "cfdwm_grid_create")
+        self.h = h
+        self.nx, self.ny, self.nl = nx, ny, nl
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.cfdwm_grid_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def refill(self):
+        check(self.L.cfdwm_grid_refill(self.h), "cfdwm_grid_refill")
+
+    def force_update(self, t):
+        n = C.c_int()
+        check(self.L.cfdwm_grid_force_update(self.h, float(t), C.byref(n)), "cfdwm_grid_force_update")
+        return n.value
+
+    def state(self):
+        ts = C.c_int()
+        v = [C.c_double() for _ in range(4)]
+        self.L.cfdwm_grid_state(self.h, C.byref(ts), *[C.byref(q) for q in v])
+        return dict(tstep=ts.value, t0=v[0].value, dx=v[1].value, dy=v[2].value, ds=v[3].value)
+
+    def export(self, which):
+        w = GRID_FIELDS.index(which) if isinstance(which, str) else int(which)
+        n = self.L.cfdwm_grid_export(self.h, w, None)
+        out = np.zeros(n)
+        got = self.L.cfdwm_grid_export(self.h, w, _p(out))
+        if got != n:
+            raise RuntimeError("cfdwm_grid_export failed")
+        return out
+
+    def query(self, t, x, y, z):
+        t, x, y, z = (_f64(np.ravel(a)) for a in np.broadcast_arrays(t, x, y, z))
+        out = np.zeros((t.size, 4))
+        check(self.L.cfdwm_grid_query(self.h, t.size, _p(t), _p(x), _p(y), _p(z), _p(out)), "cfdwm_grid_query")
+        return out
+
+    def query_device_f32(self, n, t, x_ptr, y_ptr, z_ptr, eta_ptr, u_ptr, v_ptr, w_ptr):
+        """Device pointers (e.g. torch tensors' data_ptr()), enqueued on the library stream."""
+        check(self.L.cfdwm_grid_query_device_f32(self.h, int(n), float(t), x_ptr, y_ptr, z_ptr,
+                                                 eta_ptr, u_ptr, v_ptr, w_ptr), "cfdwm_grid_query_device_f32")
+
+
+def sync():
+    check(load().cfdwm_sync(), "cfdwm_sync")
+
+
+def prof_enable(on=True):
+    check(load().cfdwm_prof_enable(int(on)), "cfdwm_prof_enable")
+
+
+def prof_read(reset=True):
+    L = load()
+    cap = 64
+    names = C.create_string_buffer(64 * cap)
+    ms = np.zeros(cap)
+    cnt = np.zeros(cap, dtype=np.int64)
+    n = L.cfdwm_prof_read(names, _p(ms), cnt.ctypes.data_as(C.POINTER(C.c_int64)), cap, int(reset))
+    out = {}
+    for q in range(n):
+        nm = names.raw[64 * q:64 * q + 64].split(b"\0", 1)[0].decode()
+        out[nm] = (float(ms[q]), int(cnt[q]))
+    return out

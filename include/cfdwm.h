@@ -1,0 +1,183 @@
+/* libcfdwm: MI355X (gfx950) second-order irregular-wave kinematics behind the
+ * CFDwavemaker C ABI.
+ *
+ * Part 1 is the reference's exported API, same names, argument order and
+ * meaning (reference: src/CFDwavemaker.h:25-69), served for wave types 1
+ * (direct second-order irregular waves) and 5 (irregular waves on the
+ * lsgrid s-grid).  Other wave types of the reference (regular Stokes/Fenton,
+ * piston wavemaker, SWD, VTK, sloping water) answer 0.0, as the reference does
+ * for unsupported builds.
+ *
+ * Part 2 adds batched entries with no counterpart in the reference: point
+ * arrays evaluated in one launch, and explicit sea/grid objects for callers
+ * that do not go through waveinput.dat.
+ *
+ * All values are SI doubles; argument order is (x, y, z, t).  Batched entries
+ * return 0 on success and a negative code on error (never exit):
+ *   -1 invalid argument / state, -2 no GPU, -3 HIP runtime error;
+ * cfdwm_last_error() gives the message.
+ */
+#ifndef CFDWM_H
+#define CFDWM_H
+#include <stdint.h>
+
+#if defined(__GNUC__)
+#define CFDWM_EXPORT __attribute__((visibility("default")))
+#else
+#define CFDWM_EXPORT
+#endif
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ------------------------------------------------ Part 1: reference API */
+/* src/CFDwavemaker.h:25 -- 1 after wave_Initialize succeeded */
+CFDWM_EXPORT int CFDwavemaker_is_initialized(void);
+/* src/CFDwavemaker.h:27, CFDwavemaker.cpp:2065-2149 -- reads ./waveinput.dat
+ * (or ./constant/, ../input_files/, ../); 0 ok, -1 file not found.  Input
+ * errors print a message and exit the process, as the reference does. */
+CFDWM_EXPORT int wave_Initialize(void);
+/* src/CFDwavemaker.h:28 -- releases device memory; returns 0 */
+CFDWM_EXPORT int wave_Cleanup(void);
+/* src/CFDwavemaker.h:29-31, CFDwavemaker.cpp:1543-1797 -- velocity at (x,y,z,t),
+ * z clamped to >= -depth, multiplied by the ramps */
+CFDWM_EXPORT double wave_VeloX(double x, double y, double z, double t);
+CFDWM_EXPORT double wave_VeloY(double x, double y, double z, double t);
+CFDWM_EXPORT double wave_VeloZ(double x, double y, double z, double t);
+/* src/CFDwavemaker.h:32, CFDwavemaker.cpp:1800-1813 -- wave type 1 only */
+CFDWM_EXPORT double wave_DynPres(double x, double y, double z, double t);
+/* src/CFDwavemaker.h:33, CFDwavemaker.cpp:1815-1832 -- dp - rho g eta, type 1 */
+CFDWM_EXPORT double wave_nhPres(double x, double y, double z, double t);
+/* src/CFDwavemaker.h:34, CFDwavemaker.cpp:1894-1969 */
+CFDWM_EXPORT double wave_SurfElev(double x, double y, double t);
+/* src/CFDwavemaker.h:35, CFDwavemaker.cpp:1990-2009 */
+CFDWM_EXPORT double wave_VFrac(double x, double y, double z, double t, double delta_cell);
+/* src/CFDwavemaker.h:36, CFDwavemaker.cpp:1971-1986 -- -depth */
+CFDWM_EXPORT double wave_Seabed(double x, double y);
+/* src/CFDwavemaker.h:37, CFDwavemaker.cpp:1623-1667 -- {eta, u, v, w} x ramp.
+ * The pointer is thread-local storage valid until the calling thread's next
+ * call (the reference returns a fresh new[] allocation that is never freed). */
+CFDWM_EXPORT double* wave_Kinematics(double x, double y, double z, double t);
+/* src/CFDwavemaker.h:38 -- piston wavemaker: 0.0 here */
+CFDWM_EXPORT double wave_piston_position(double t);
+/* src/CFDwavemaker.h:41, CFDwavemaker.cpp:2253-2281 -- advances the s-grid
+ * one dt at a time while t > t0 + dt */
+CFDWM_EXPORT void wave_force_update(double t);
+/* src/CFDwavemaker.h:48 -- accepted and ignored (one GPU, no MPI) */
+CFDWM_EXPORT void wave_set_mpi_comm(int fortran_comm);
+/* src/CFDwavemaker.h:51 -- probes are never configured by the parser: no-op */
+CFDWM_EXPORT void update_probes(double t);
+/* src/CFDwavemaker.h:54-58, Irregular.cpp:1372-1436 */
+CFDWM_EXPORT double wave_phase_speed(int opt);
+CFDWM_EXPORT double wave_mean_period(int opt);
+CFDWM_EXPORT double wave_mean_length(int opt);
+CFDWM_EXPORT double wave_water_depth(void);
+/* src/CFDwavemaker.h:61-69, CFDwavemaker.cpp:2168-2198 -- COMFLOW aliases */
+CFDWM_EXPORT double VelocityX(int i, int j, int k, double x, double y, double z, double t);
+CFDWM_EXPORT double VelocityY(int i, int j, int k, double x, double y, double z, double t);
+CFDWM_EXPORT double VelocityZ(int i, int j, int k, double x, double y, double z, double t);
+CFDWM_EXPORT double DynamicPressure(int i, int j, int k, double x, double y, double z, double t);
+CFDWM_EXPORT double SurfaceElevation(int i, int j, double x, double y, double t);
+CFDWM_EXPORT double VolumeFraction(double x, double y, double z, double t, double delta_cell);
+CFDWM_EXPORT int Init(void* fptr);
+CFDWM_EXPORT int Prepare(double t, int mode);
+CFDWM_EXPORT int Cleanup(void);
+
+/* ------------------------------------------------ Part 2: batched entries */
+/* Output selection and precision flags for wave_EvalPoints. */
+#define CFDWM_OUT_ETA   0x1u
+#define CFDWM_OUT_U     0x2u
+#define CFDWM_OUT_V     0x4u
+#define CFDWM_OUT_W     0x8u
+#define CFDWM_OUT_P     0x10u
+#define CFDWM_FP32      0x100u   /* FP32 VALU sums with FP64 phase reduction */
+
+/* Batched point evaluation of the initialised wave field: for i < n,
+ * eta[i] = wave_SurfElev(x[i], y[i], t[i]), u[i] = wave_VeloX(x[i], y[i], z[i], t[i]),
+ * v, w likewise and p[i] = wave_DynPres(...), in one launch.  Output pointers
+ * may be NULL (not computed).  Host arrays.  Returns 0 or a negative code.
+ * No counterpart in the reference (which has only the per-point calls above). */
+CFDWM_EXPORT int wave_EvalPoints(int64_t n, const double* x, const double* y, const double* z,
+                                 const double* t, double* eta, double* u, double* v, double* w,
+                                 double* p, uint32_t flags);
+/* As wave_EvalPoints, all pointers in device memory (the call synchronises
+ * the library's stream before returning). */
+CFDWM_EXPORT int wave_EvalPointsDevice(int64_t n, const double* x, const double* y, const double* z,
+                                       const double* t, double* eta, double* u, double* v, double* w,
+                                       double* p, uint32_t flags);
+
+/* Explicit objects (no waveinput.dat, no process-global state). */
+typedef struct cfdwm_sea cfdwm_sea;
+typedef struct cfdwm_grid cfdwm_grid;
+
+typedef struct {
+    double depth, g, rho, swl, mtheta, tofmax, x0, y0, bandwidth, cutoff, ampl;
+    int order;        /* 2: second order (Irregular::order default), 0: linear */
+    int normalize;    /* Irregular::normalize_data: A = ampl A / sum(A), phases 0 */
+    int bw_auto;      /* Irregular::bandwidth_estimator */
+    int precision;    /* 0 FP64, 1 FP32 (FP64 phase reduction) */
+    int with_direct;  /* also build the unfiltered table of u2/v2/w2/dp2 */
+} cfdwm_sea_params;
+
+/* Components as Irregular holds them (sorted by omega).  ramptime, starttime,
+ * duration may be NULL (reference defaults 0, -1e-12, 1e12). */
+CFDWM_EXPORT int cfdwm_sea_create(cfdwm_sea** out, int n, const double* omega, const double* A,
+                                  const double* k, const double* phase, const double* theta,
+                                  const double* ramptime, const double* starttime,
+                                  const double* duration, const cfdwm_sea_params* params);
+CFDWM_EXPORT int cfdwm_sea_destroy(cfdwm_sea* sea);
+/* Number of second-order terms kept by the pair filters (plus+minus branches
+ * of the pair cache) and of diagonal terms: the algorithmic work per point. */
+CFDWM_EXPORT int cfdwm_sea_terms(cfdwm_sea* sea, int64_t* pair_terms, int64_t* diag_terms,
+                                 int64_t* direct_terms);
+
+/* Evaluation kinds, one per reference Irregular member:
+ *  0 eta1  1 eta2  2 eta  3 u1  4 v1  5 w1  6 dp1  7 phi1_dxdz  8 phi1_dydz  9 phi1_dzdz
+ * 10 dp1_dz  11 u2  12 v2  13 w2  14 dp2  15 u  16 v  17 w  18 dp  19 uvw2.u  20 uvw2.v
+ * 21 uvw2.w  (22 uvw2.p is computed by the reference but read by no API path: -1).
+ * Host arrays, n points; z ignored by the z-independent kinds. */
+CFDWM_EXPORT int cfdwm_eval(cfdwm_sea* sea, int kind, int64_t n, const double* t, const double* x,
+                            const double* y, const double* z, double* out);
+
+typedef struct {
+    double domain[4];            /* xmin xmax ymin ymax */
+    int nx, ny, nl;
+    double t0, dt, tan_a, tan_b, water_depth, swl;
+    double domain_ignore[4];     /* columns inside are skipped by updates */
+    int ignore_at_init;          /* initialize_kinematics_with_ignore */
+} cfdwm_grid_desc;
+
+/* lsGridSpline: allocate and fill (initialize_kinematics). */
+CFDWM_EXPORT int cfdwm_grid_create(cfdwm_grid** out, cfdwm_sea* sea, const cfdwm_grid_desc* desc);
+CFDWM_EXPORT int cfdwm_grid_destroy(cfdwm_grid* grid);
+/* Re-run initialize_kinematics (time ring back to t0). */
+CFDWM_EXPORT int cfdwm_grid_refill(cfdwm_grid* grid);
+/* wave_force_update semantics; *nupdates = time slices computed. */
+CFDWM_EXPORT int cfdwm_grid_force_update(cfdwm_grid* grid, double t, int* nupdates);
+CFDWM_EXPORT int cfdwm_grid_state(cfdwm_grid* grid, int* tstep, double* t0, double* dx, double* dy,
+                                  double* ds);
+/* Copy a field to host doubles (field numbering of the reference members:
+ * 0 ETA 1 ETAdx 2 ETAdy 3 ETAdt 4 UX 5 UY 6 UZ 7 Udt 8 Vdt 9 Wdt 10 Udx 11 Udy
+ * 12 Uds 13 Vdx 14 Vdy 15 Vds 16 Wdx 17 Wdy 18 Wds 19 IGNORE).  Returns the
+ * element count (out may be NULL to ask for it). */
+CFDWM_EXPORT int64_t cfdwm_grid_export(cfdwm_grid* grid, int which, double* out);
+/* get_kinematics_at_point for n host points -> out4[4n] = {eta, u, v, w} per point. */
+CFDWM_EXPORT int cfdwm_grid_query(cfdwm_grid* grid, int64_t n, const double* t, const double* x,
+                                  const double* y, const double* z, double* out4);
+/* Device-pointer query: FP32 coordinates at one time t, FP32 outputs.
+ * Enqueued on the library stream; cfdwm_sync() waits for it. */
+CFDWM_EXPORT int cfdwm_grid_query_device_f32(cfdwm_grid* grid, int64_t n, double t, const float* x,
+                                             const float* y, const float* z, float* eta, float* u,
+                                             float* v, float* w);
+CFDWM_EXPORT int cfdwm_sync(void);
+CFDWM_EXPORT int cfdwm_last_error(char* buf, int len);
+/* Kernel timing with HIP events on the library stream. */
+CFDWM_EXPORT int cfdwm_prof_enable(int on);
+/* Fills up to cap entries: name (64 bytes each), total ms, launch count. */
+CFDWM_EXPORT int cfdwm_prof_read(char* names, double* total_ms, int64_t* count, int cap, int reset);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
