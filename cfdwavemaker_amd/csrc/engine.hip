@@ -328,8 +328,12 @@ int Grid::init() {
         if (f_[f].alloc(b)) return -3;
         CFDWM_HIP(hipMemsetAsync(f_[f].get(), 0, b, st));
     }
-    level_chunk = d.nl >= 16 ? 16 : (d.nl > 8 ? 16 : (d.nl > 4 ? 8 : (d.nl > 2 ? 4 : (d.nl > 1 ? 2 : 1))));
+    // levels per thread: FP32 8 (124 VGPRs, 4 waves/SIMD), FP64 4 (139 VGPRs, 3 waves/SIMD)
+    const int cap = prec == kFP32 ? 8 : 4;
+    level_chunk = 1;
+    while (level_chunk < d.nl && level_chunk < cap) level_chunk *= 2;
     if (const char* e = std::getenv("CFDWM_LEVEL_CHUNK")) level_chunk = std::atoi(e);
+    if (const char* e = std::getenv("CFDWM_PRUNE")) prune_ = std::atoi(e) != 0;
     return refill();
 }
 
@@ -356,6 +360,14 @@ int Grid::fill_slots(int nslot, const double* slot_t, const int* slot_dst, bool 
     L.sigma = (const double*)sigma_d_.get();
     L.o_u = (R*)f_[4].get(); L.o_v = (R*)f_[5].get(); L.o_w = (R*)f_[6].get();
     L.flags = EV_LIN | EV_SEC | EV_TAYLOR;
+    if (prune_) {
+        const int64_t nrec = sea->nfilt();
+        if (lvl_.alloc(std::max<int64_t>(1, nrec) * sizeof(uint32_t)) || eb_.alloc(2 * sizeof(double))) return -3;
+        PROF("k_prune", launch_prune<R>((const R*)f_[0].get(), ncell, C.cols, C.ncol, nslot, slot_dst, g.swl,
+                                        (double*)eb_.get(), S.fuvw, nrec, (const double*)sigma_d_.get(), g.nl, g.h,
+                                        (uint32_t*)lvl_.get(), st));
+        L.lvl = (const uint32_t*)lvl_.get();
+    }
     PROF("k_levels_grid", launch_levels<R>(S, L, true, level_chunk, st));
     return 0;
 }

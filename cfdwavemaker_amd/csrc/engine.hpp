@@ -41,6 +41,7 @@ struct LevelJob {
     const double *x = nullptr, *y = nullptr, *t = nullptr, *z = nullptr;
     double *p_u = nullptr, *p_v = nullptr, *p_w = nullptr, *p_p = nullptr;
     uint32_t flags = 0;
+    const uint32_t* lvl = nullptr;        // GRID: per filtered pair record, pruning level bounds
 };
 
 template <typename R>
@@ -72,6 +73,12 @@ template <typename R>
 hipError_t launch_grad_dt(const R* D, R* GD, int64_t per, int tstep, double dt, hipStream_t st);
 template <typename R, typename TQ>
 hipError_t launch_query(const QueryJob<R, TQ>& J, hipStream_t st);
+// eta range of the columns of a fill (minus swl) -> eb[2]; then per filtered
+// pair record the levels where its profile exponentials exceed e^{-T}
+template <typename R>
+hipError_t launch_prune(const R* eta, int64_t ncell, const int32_t* cols, int64_t ncol, int nslot,
+                        const int* slot_dst, double swl, double* eb, const UvwRec<R>* rec, int64_t nrec,
+                        const double* sigma, int nl, double h, uint32_t* lvl, hipStream_t st);
 
 // ---------------------------------------------------------------- errors
 struct Error {
@@ -143,6 +150,7 @@ class Sea {
 public:
     SeaTables tab;
     int prec = kFP64;
+    int64_t nfilt() const { return (int64_t)tab.filt.rec.size(); }
     int upload(bool with_direct);
     template <typename R>
     const SeaDev<R>& dev() const;
@@ -196,6 +204,8 @@ private:
     std::vector<double> sigma_;
     DevBuf sigma_d_, cols_all_d_, cols_act_d_;
     int64_t nact_ = 0;
+    DevBuf lvl_, eb_;
+    bool prune_ = true;
     // fields: 0 ETA(4) 1 ETAdx(2) 2 ETAdy(2) 3 ETAdt(4) 4..6 U/V/W(4) 7..9 dt(4) 10..18 dx/dy/ds(2)
     DevBuf f_[19];
     DevBuf q_in_, q_out_;

@@ -58,6 +58,10 @@ __device__ __forceinline__ void prof(float pk, float pg, float z, float& E, floa
 template <typename R>
 __device__ __forceinline__ R fmaR(R a, R b, R c) { return a * b + c; }
 
+// the exponential of prof(): natural base in FP64, base 2 (hardware) in FP32
+__device__ __forceinline__ double pexp(double a) { return exp(a); }
+__device__ __forceinline__ float pexp(float a) { return __builtin_amdgcn_exp2f(a); }
+
 }  // namespace
 
 // ============================================================ column kernel
@@ -84,7 +88,7 @@ __global__ __launch_bounds__(256) void k_columns(SeaDev<R> S, ColumnJob<R> J) {
         x = J.x[q]; y = J.y[q]; t = J.t[q];
     }
     const double dxp = x - S.x0, dyp = y - S.y0;
-    R e1 = 0, e2 = 0, gx = 0, gy = 0, gz = 0, gp = 0;
+    double e1 = 0, e2 = 0, gx = 0, gy = 0, gz = 0, gp = 0;   // FP64 totals, rows in R
     for (int i = 0; i < S.n; ++i) {
         const PhaseRec pi = S.ph[i];
         const CompRec<R>& ci_ = S.cr[i];
@@ -92,13 +96,13 @@ __global__ __launch_bounds__(256) void k_columns(SeaDev<R> S, ColumnJob<R> J) {
         cs_of(phase_of(pi, dxp, dyp, t), ci, si);
         const R ri = (R)freq_ramp(t, pi.rt, pi.ts, pi.du);
         const R cr = ci * ri, sr = si * ri;
-        e1 = fmaR(ci_.A, cr, e1);
-        gx = fmaR(ci_.gxz, cr, gx);
-        gy = fmaR(ci_.gyz, cr, gy);
-        gz = fmaR(ci_.gzz, sr, gz);
-        gp = fmaR(ci_.gpz, cr, gp);
+        e1 += (double)(ci_.A * cr);
+        gx += (double)(ci_.gxz * cr);
+        gy += (double)(ci_.gyz * cr);
+        gz += (double)(ci_.gzz * sr);
+        gp += (double)(ci_.gpz * cr);
         if (!J.want_eta2) continue;
-        e2 = fmaR(ci_.e2d, (ci * ci - si * si) * ri, e2);
+        e2 += (double)(ci_.e2d * (ci * ci - si * si) * ri);
         const int mhi = S.fmhi[i];
         const XRec<R>* __restrict__ xr = S.fx + S.frow[i] - (i + 1);
         R acc = 0;
@@ -110,11 +114,11 @@ __global__ __launch_bounds__(256) void k_columns(SeaDev<R> S, ColumnJob<R> J) {
             acc = fmaR(rec.xP, cc - ss, acc);
             acc = fmaR(rec.xM, cc + ss, acc);
         }
-        e2 += acc;
+        e2 += (double)acc;
     }
     if (!live) return;
     if (GRID) {
-        J.eta_grid[(int64_t)J.slot_dst[slot] * J.ncell + col] = (R)((double)e1 + (double)e2 + S.swl);
+        J.eta_grid[(int64_t)J.slot_dst[slot] * J.ncell + col] = (R)(e1 + e2 + S.swl);
     } else {
         J.o_eta1[q] = e1; J.o_eta2[q] = e2;
         J.o_gx[q] = gx; J.o_gy[q] = gy; J.o_gz[q] = gz; J.o_gp[q] = gp;
@@ -123,38 +127,48 @@ __global__ __launch_bounds__(256) void k_columns(SeaDev<R> S, ColumnJob<R> J) {
 
 // ============================================================= level kernel
 // Per (column, chunk of LC levels): first order + second order + Taylor.
-// GRID (lsgrid_spline.cpp:708-721): z from the sigma map of the column's eta,
-//   profiles at min(0, z), Taylor with max(0, z - swl); filtered uvw2 terms.
-// POINTS: one level per point; flags select the terms (EV_*).
-template <typename R, int LC, bool GRID>
+// MODE 0 (GRID, lsgrid_spline.cpp:708-721): z from the sigma map of the
+//   column's eta, profiles at min(0, z), Taylor with max(0, z - swl), filtered
+//   uvw2 terms.  MODE 1/2 (POINTS, one level per point, 2 = with pressure):
+//   flags select the terms (EV_*).
+// Accumulation: each row i (pairs (i, m), m > i) is summed in the working
+// precision R and folded into FP64 totals, which keeps the FP32 error at the
+// level of the per-term rounding instead of growing over ~N^2/2 terms.
+template <typename R, int LC, int MODE>
 __global__ __launch_bounds__(256) void k_levels(SeaDev<R> S, LevelJob<R> J) {
+    constexpr bool GRID = MODE == 0;
+    constexpr bool WP = MODE == 2;
     const int64_t tid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    // GRID: level chunks are the slowest index and every chunk is padded to whole
+    // wavefronts, so the chunk (hence the absolute level of each unrolled slot)
+    // is wave-uniform and the pruning tests below are scalar branches.
+    const int64_t ncolt = (int64_t)J.nslot * J.ncol;
+    const int64_t ncolp = GRID ? (ncolt + 63) / 64 * 64 : ncolt;
     const int nchunk = GRID ? (J.g.nl + LC - 1) / LC : 1;
-    const int64_t total = (int64_t)J.nslot * J.ncol * nchunk;
-    const bool live = tid < total;
-    const int64_t q = live ? tid : total - 1;
-    const uint32_t fl = J.flags;
-    double x, y, t;
+    const int64_t chunk = tid / ncolp;
+    const int64_t cq0 = tid - chunk * ncolp;
+    const bool live = chunk < nchunk && cq0 < ncolt;
+    const int64_t cq = cq0 < ncolt ? cq0 : ncolt - 1;
+    const int64_t q = cq;
+    const uint32_t fl = GRID ? (uint32_t)(EV_LIN | EV_SEC | EV_TAYLOR) : J.flags;
+    double x, y, t, e = 0., z1 = 0.;
     int64_t col = 0;
     int slot = 0, l0 = 0;
-    R zp[LC], zt[LC];
+    R zp[LC];
     if (GRID) {
-        const int64_t cq = q / nchunk;
-        l0 = (int)(q % nchunk) * LC;
+        l0 = __builtin_amdgcn_readfirstlane((int)(chunk < nchunk ? chunk : nchunk - 1) * LC);
         slot = (int)(cq / J.ncol);
         col = J.cols ? J.cols[cq % J.ncol] : cq % J.ncol;
         const int i = (int)(col / J.g.ny), j = (int)(col % J.g.ny);
         x = J.g.x0 + J.g.dx * i;
         y = J.g.y0 + J.g.dy * j;
         t = J.slot_t[slot];
-        const double eta = (double)J.eta_grid[(int64_t)J.slot_dst[slot] * J.ncell + col];
-        const double e = eta - J.g.swl;
+        e = (double)J.eta_grid[(int64_t)J.slot_dst[slot] * J.ncell + col] - J.g.swl;
 #pragma unroll
         for (int l = 0; l < LC; ++l) {
             const int lv = l0 + l < J.g.nl ? l0 + l : J.g.nl - 1;
-            const double z = e + J.sigma[lv] * (J.g.h + e);           // s2z (lsgrid_spline.cpp:85-87)
-            zp[l] = (R)(z < 0. ? z : 0.);                               // min(0, z), no -swl (:711)
-            zt[l] = (R)(z - J.g.swl > 0. ? z - J.g.swl : 0.);          // max(0, z - swl)
+            const double z = e + J.sigma[lv] * (J.g.h + e);             // s2z (lsgrid_spline.cpp:85-87)
+            zp[l] = (R)(z < 0. ? z : 0.);                                 // min(0, z), no -swl (:711)
         }
     } else {
         x = J.x[q]; y = J.y[q]; t = J.t[q];
@@ -162,22 +176,18 @@ __global__ __launch_bounds__(256) void k_levels(SeaDev<R> S, LevelJob<R> J) {
         double zz = z;
         if (fl & EV_ZCLAMP) zz = (z - S.swl) < 0. ? (z - S.swl) : 0.;
         else if (fl & EV_ZSHIFT) zz = z - S.swl;
-        zp[0] = (R)zz;
-        zt[0] = (R)((z - S.swl) > 0. ? (z - S.swl) : 0.);
 #pragma unroll
-        for (int l = 1; l < LC; ++l) { zp[l] = zp[0]; zt[l] = zt[0]; }
+        for (int l = 0; l < LC; ++l) zp[l] = (R)zz;
+        z1 = (z - S.swl) > 0. ? (z - S.swl) : 0.;
     }
-    const bool direct = (fl & EV_DIRECT) != 0;
-    const bool wantP = (fl & EV_P) != 0;
+    const bool direct = !GRID && (fl & EV_DIRECT) != 0;
+    const bool lin = (fl & EV_LIN) != 0;
     const double dxp = x - S.x0, dyp = y - S.y0;
-    R U[LC], V[LC], W[LC], P[LC], P2[LC];
+    double U[LC], V[LC], W[LC];
 #pragma unroll
-    for (int l = 0; l < LC; ++l) { U[l] = V[l] = W[l] = P[l] = P2[l] = 0; }
-    R gx = 0, gy = 0, gz = 0, gp = 0;
-    // kinetic running sums of the unfiltered dp2 (separable in i and m)
-    R kx[LC], ky[LC], kz[LC];
-#pragma unroll
-    for (int l = 0; l < LC; ++l) { kx[l] = ky[l] = kz[l] = 0; }
+    for (int l = 0; l < LC; ++l) U[l] = V[l] = W[l] = 0.;
+    double P = 0., P2 = 0., kx = 0., ky = 0., kz = 0.;   // pressure (MODE 2, LC == 1)
+    double gx = 0, gy = 0, gz = 0, gp = 0;
 
     const UvwRec<R>* __restrict__ tab = direct ? S.duvw : S.fuvw;
     // rows run backwards so that the dp2 kinetic sums over m > i are running sums
@@ -188,34 +198,39 @@ __global__ __launch_bounds__(256) void k_levels(SeaDev<R> S, LevelJob<R> J) {
         cs_of(phase_of(pi, dxp, dyp, t), ci, si);
         const R ri = (R)freq_ramp(t, pi.rt, pi.ts, pi.du);
         const R cr = ci * ri, sr = si * ri;
-        gx = fmaR(c.gxz, cr, gx);
-        gy = fmaR(c.gyz, cr, gy);
-        gz = fmaR(c.gzz, sr, gz);
-        gp = fmaR(c.gpz, cr, gp);
+        gx += (double)(c.gxz * cr);
+        gy += (double)(c.gyz * cr);
+        gz += (double)(c.gzz * sr);
+        if (WP) gp += (double)(c.gpz * cr);
         const bool sec = (fl & EV_SEC) && (direct || S.rowActive[i]);
-        if ((fl & EV_LIN) || (sec && direct && wantP)) {
+        const bool kin = WP && sec && direct;
+        if (lin || kin) {
 #pragma unroll
             for (int l = 0; l < LC; ++l) {
                 R E, G;
                 prof(c.pk, c.pg, zp[l], E, G);
                 const R ch = E + G, sh = E - G;
-                if (fl & EV_LIN) {
-                    U[l] = fmaR(c.lu * cr, ch, U[l]);
-                    V[l] = fmaR(c.lv * cr, ch, V[l]);
-                    W[l] = fmaR(c.lw * sr, sh, W[l]);
-                    if (wantP) P[l] = fmaR(c.lp * cr, ch, P[l]);
+                if (lin) {
+                    U[l] += (double)(c.lu * cr * ch);
+                    V[l] += (double)(c.lv * cr * ch);
+                    W[l] += (double)(c.lw * sr * sh);
+                    if (WP) P += (double)(c.lp * cr * ch);
                 }
-                if (sec && direct && wantP) {
-                    // -(Xx_i Xx_m + Xy_i Xy_m) px_i px_m + Z_i Z_m pz_i pz_m, summed over m > i
-                    P2[l] += ri * (-ch * (c.Xx * kx[l] + c.Xy * ky[l]) + sh * c.Zk * kz[l]);
-                    P2[l] += ri * (-c.kd1 * ch + c.kd2 * sh);     // diagonal "second term"
-                    kx[l] = fmaR(c.Xx, ch, kx[l]);
-                    ky[l] = fmaR(c.Xy, ch, ky[l]);
-                    kz[l] = fmaR(c.Zk, sh, kz[l]);
+                if (kin) {
+                    // -(Xx_i Xx_m + Xy_i Xy_m) px_i px_m + Z_i Z_m pz_i pz_m over m > i,
+                    // and the diagonal "second term" (Irregular.cpp:755-756, 786-787)
+                    const double dch = (double)ch, dsh = (double)sh;
+                    P2 += (double)ri * (-dch * ((double)c.Xx * kx + (double)c.Xy * ky) + dsh * (double)c.Zk * kz);
+                    P2 += (double)ri * (-(double)c.kd1 * dch + (double)c.kd2 * dsh);
+                    kx += (double)c.Xx * dch;
+                    ky += (double)c.Xy * dch;
+                    kz += (double)c.Zk * dsh;
                 }
             }
         }
         if (!sec) continue;
+        R ur[LC], vr[LC], wr[LC];
+        R pr = 0;
         // diagonal self-interaction (profile of 2k)
         {
             const R c2 = (ci * ci - si * si) * ri, s2 = (R)2 * si * ci * ri;
@@ -225,10 +240,10 @@ __global__ __launch_bounds__(256) void k_levels(SeaDev<R> S, LevelJob<R> J) {
                 R E, G;
                 prof(c.p2k, c.p2g, zp[l], E, G);
                 const R ch = E + G, sh = E - G;
-                U[l] = fmaR(du * c2, ch, U[l]);
-                V[l] = fmaR(dv * c2, ch, V[l]);
-                W[l] = fmaR(dw * s2, sh, W[l]);
-                if (direct && wantP) P2[l] = fmaR(c.ddp * c2, ch, P2[l]);
+                ur[l] = du * c2 * ch;
+                vr[l] = dv * c2 * ch;
+                wr[l] = dw * s2 * sh;
+                if (WP && direct) pr = c.ddp * c2 * ch;
             }
         }
         // pair terms, m in (i, mhi)
@@ -236,55 +251,99 @@ __global__ __launch_bounds__(256) void k_levels(SeaDev<R> S, LevelJob<R> J) {
         const int64_t base = (direct ? S.drow[i] : S.frow[i]) - (i + 1);
         const UvwRec<R>* __restrict__ rr = tab + base;
         const XRec<R>* __restrict__ xr = S.dx + base;
+        const uint32_t* __restrict__ lvr = J.lvl + base;
         for (int m = i + 1; m < mhi; ++m) {
             R cm, sm;
             cs_of(phase_of(S.ph[m], dxp, dyp, t), cm, sm);
             const UvwRec<R> r = rr[m];
+            // levels at which e^{kz} / e^{-k(z+2h)} can exceed the pruning
+            // threshold (bounds from this launch's eta range); 255/0: none
+            uint32_t lv = 0x00ff00ffu;
+            if (GRID && J.lvl) lv = lvr[m];
             const R cc = cr * cm, ss = sr * sm, sc = sr * cm, cs = cr * sm;
             const R cP = cc - ss, cM = cc + ss, sP = sc + cs, sM = sc - cs;
             const R uP = r.cuP * cP, vP = r.cvP * cP, wP = r.cwP * sP;
             const R uM = r.cuM * cM, vM = r.cvM * cM, wM = r.cwM * sM;
             R pP = 0, pM = 0;
-            if (direct && wantP) {
+            if (WP && direct) {
                 const XRec<R> xq = xr[m];
                 pP = xq.xP * cP;
                 pM = xq.xM * cM;
+            }
+            if (GRID && J.lvl) {
+                const int lEP = (int)(lv & 255u), lGP = (int)((lv >> 8) & 255u);
+                const int lEM = (int)((lv >> 16) & 255u), lGM = (int)(lv >> 24);
+#pragma unroll
+                for (int l = 0; l < LC; ++l) {
+                    const int L = l0 + l;
+                    // separate blocks for e^{kz} and e^{-k(z+2h)}; the empty asm keeps
+                    // the compiler from speculating a skipped exponential into a select
+                    if (L >= lEP) {
+                        __asm__ volatile("");
+                        const R E = pexp(r.pkP * zp[l]);
+                        ur[l] = fmaR(uP, E, ur[l]); vr[l] = fmaR(vP, E, vr[l]); wr[l] = fmaR(wP, E, wr[l]);
+                    }
+                    if (L < lGP) {
+                        __asm__ volatile("");
+                        const R G = pexp(fmaR(-r.pkP, zp[l], r.pgP));
+                        ur[l] = fmaR(uP, G, ur[l]); vr[l] = fmaR(vP, G, vr[l]); wr[l] = fmaR(-wP, G, wr[l]);
+                    }
+                    if (L >= lEM) {
+                        __asm__ volatile("");
+                        const R E = pexp(r.pkM * zp[l]);
+                        ur[l] = fmaR(uM, E, ur[l]); vr[l] = fmaR(vM, E, vr[l]); wr[l] = fmaR(wM, E, wr[l]);
+                    }
+                    if (L < lGM) {
+                        __asm__ volatile("");
+                        const R G = pexp(fmaR(-r.pkM, zp[l], r.pgM));
+                        ur[l] = fmaR(uM, G, ur[l]); vr[l] = fmaR(vM, G, vr[l]); wr[l] = fmaR(-wM, G, wr[l]);
+                    }
+                }
+                continue;
             }
 #pragma unroll
             for (int l = 0; l < LC; ++l) {
                 R E, G;
                 prof(r.pkP, r.pgP, zp[l], E, G);
                 R ch = E + G, sh = E - G;
-                U[l] = fmaR(uP, ch, U[l]);
-                V[l] = fmaR(vP, ch, V[l]);
-                W[l] = fmaR(wP, sh, W[l]);
-                if (direct && wantP) P2[l] = fmaR(pP, ch, P2[l]);
+                ur[l] = fmaR(uP, ch, ur[l]);
+                vr[l] = fmaR(vP, ch, vr[l]);
+                wr[l] = fmaR(wP, sh, wr[l]);
+                if (WP && direct) pr = fmaR(pP, ch, pr);
                 prof(r.pkM, r.pgM, zp[l], E, G);
                 ch = E + G; sh = E - G;
-                U[l] = fmaR(uM, ch, U[l]);
-                V[l] = fmaR(vM, ch, V[l]);
-                W[l] = fmaR(wM, sh, W[l]);
-                if (direct && wantP) P2[l] = fmaR(pM, ch, P2[l]);
+                ur[l] = fmaR(uM, ch, ur[l]);
+                vr[l] = fmaR(vM, ch, vr[l]);
+                wr[l] = fmaR(wM, sh, wr[l]);
+                if (WP && direct) pr = fmaR(pM, ch, pr);
             }
         }
+#pragma unroll
+        for (int l = 0; l < LC; ++l) {
+            U[l] += (double)ur[l];
+            V[l] += (double)vr[l];
+            W[l] += (double)wr[l];
+        }
+        if (WP && direct) P2 += (double)pr;
     }
     if (!live) return;
-    const bool tay = (fl & EV_TAYLOR) != 0;
     if (GRID) {
         const int64_t o = ((int64_t)J.slot_dst[slot] * J.ncell + col) * J.g.nl + l0;
 #pragma unroll
         for (int l = 0; l < LC; ++l) {
             if (l0 + l >= J.g.nl) break;
-            J.o_u[o + l] = (R)((double)U[l] + (double)gx * (double)zt[l]);
-            J.o_v[o + l] = (R)((double)V[l] + (double)gy * (double)zt[l]);
-            J.o_w[o + l] = (R)((double)W[l] + (double)gz * (double)zt[l]);
+            const double z = e + J.sigma[l0 + l] * (J.g.h + e);
+            const double zt = z - J.g.swl > 0. ? z - J.g.swl : 0.;      // max(0, z - swl)
+            J.o_u[o + l] = (R)(U[l] + gx * zt);
+            J.o_v[o + l] = (R)(V[l] + gy * zt);
+            J.o_w[o + l] = (R)(W[l] + gz * zt);
         }
     } else {
-        const double z1 = (double)zt[0];
-        J.p_u[q] = (double)U[0] + (tay ? (double)gx * z1 : 0.);
-        J.p_v[q] = (double)V[0] + (tay ? (double)gy * z1 : 0.);
-        J.p_w[q] = (double)W[0] + (tay ? (double)gz * z1 : 0.);
-        if (wantP) J.p_p[q] = (double)P[0] + S.rho * (double)P2[0] + (tay ? (double)gp * z1 : 0.);
+        const bool tay = (fl & EV_TAYLOR) != 0;
+        J.p_u[q] = U[0] + (tay ? gx * z1 : 0.);
+        J.p_v[q] = V[0] + (tay ? gy * z1 : 0.);
+        J.p_w[q] = W[0] + (tay ? gz * z1 : 0.);
+        if (WP) J.p_p[q] = P + S.rho * P2 + (tay ? gp * z1 : 0.);
     }
 }
 
@@ -507,6 +566,72 @@ __global__ __launch_bounds__(256) void k_query(QueryJob<R, TQ> J) {
     J.o_w[p] = (TQ)((double)wz * rmp);
 }
 
+// ==================================================== profile-term pruning
+// eta range (minus swl) over the columns written by one fill
+template <typename R>
+__global__ __launch_bounds__(1024) void k_eta_bounds(const R* __restrict__ eta, int64_t ncell,
+                                                     const int32_t* __restrict__ cols, int64_t ncol,
+                                                     int nslot, int4 dst, double swl, double* __restrict__ eb) {
+    __shared__ double smin[1024], smax[1024];
+    double mn = 1e300, mx = -1e300;
+    const int64_t total = (int64_t)nslot * ncol;
+    for (int64_t q = threadIdx.x; q < total; q += blockDim.x) {
+        const int s = (int)(q / ncol);
+        const int64_t c = cols ? cols[q % ncol] : q % ncol;
+        const int d = s == 0 ? dst.x : (s == 1 ? dst.y : (s == 2 ? dst.z : dst.w));
+        const double v = (double)eta[(int64_t)d * ncell + c];
+        mn = fmin(mn, v);
+        mx = fmax(mx, v);
+    }
+    smin[threadIdx.x] = mn;
+    smax[threadIdx.x] = mx;
+    __syncthreads();
+    for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) {
+            smin[threadIdx.x] = fmin(smin[threadIdx.x], smin[threadIdx.x + w]);
+            smax[threadIdx.x] = fmax(smax[threadIdx.x], smax[threadIdx.x + w]);
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        eb[0] = smin[0] - swl;
+        eb[1] = smax[0] - swl;
+    }
+}
+
+// Level bounds per pair record.  Profile z of level l is min(0, e + sigma_l (h + e)),
+// increasing in e and in l, so over the fill's eta range:
+//   e^{k z} can exceed e^{-T} only for l >= lE (first such level),
+//   e^{-k(z+2h)} only for l < lG1.
+// Exponents are in the units of prof() (log2 in FP32), T likewise.
+template <typename R>
+__global__ __launch_bounds__(256) void k_prune(const UvwRec<R>* __restrict__ rec, int64_t nrec,
+                                               const double* __restrict__ eb, const double* __restrict__ sigma,
+                                               int nl, double h, double T, uint32_t* __restrict__ lvl) {
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= nrec) return;
+    const double emin = eb[0], emax = eb[1];
+    const UvwRec<R> r = rec[e];
+    uint32_t out = 0;
+    for (int b = 0; b < 2; ++b) {
+        const double pk = (double)(b ? r.pkM : r.pkP), pg = (double)(b ? r.pgM : r.pgP);
+        const bool present = b ? (r.cuM != 0 || r.cvM != 0 || r.cwM != 0) : (r.cuP != 0 || r.cvP != 0 || r.cwP != 0);
+        int lE = 255, lG1 = 0;
+        if (present) {
+            lE = nl;
+            for (int l = 0; l < nl; ++l) {
+                const double zx = emax + sigma[l] * (h + emax), zn = emin + sigma[l] * (h + emin);
+                const double zmx = zx < 0. ? zx : 0., zmn = zn < 0. ? zn : 0.;
+                if (lE == nl && pk * zmx >= -T) lE = l;
+                if (fma(-pk, zmn, pg) >= -T) lG1 = l + 1;
+            }
+            if (lE == nl) lE = 255;
+        }
+        out |= ((uint32_t)lE | ((uint32_t)lG1 << 8)) << (16 * b);
+    }
+    lvl[e] = out;
+}
+
 // ================================================================ launchers
 namespace {
 inline unsigned nblk(int64_t n, int b = 256) { return (unsigned)((n + b - 1) / b); }
@@ -524,9 +649,10 @@ hipError_t launch_columns(const SeaDev<R>& S, const ColumnJob<R>& J, bool grid, 
 template <typename R, int LC>
 static hipError_t launch_levels_lc(const SeaDev<R>& S, const LevelJob<R>& J, hipStream_t st) {
     const int nchunk = (J.g.nl + LC - 1) / LC;
-    const int64_t n = (int64_t)J.nslot * J.ncol * nchunk;
-    if (n <= 0) return hipSuccess;
-    hipLaunchKernelGGL((k_levels<R, LC, true>), dim3(nblk(n)), dim3(256), 0, st, S, J);
+    const int64_t ncolp = ((int64_t)J.nslot * J.ncol + 63) / 64 * 64;
+    const int64_t n = ncolp * nchunk;
+    if (J.nslot * J.ncol <= 0) return hipSuccess;
+    hipLaunchKernelGGL((k_levels<R, LC, 0>), dim3(nblk(n)), dim3(256), 0, st, S, J);
     return hipGetLastError();
 }
 
@@ -535,7 +661,8 @@ hipError_t launch_levels(const SeaDev<R>& S, const LevelJob<R>& J, bool grid, in
     if (!grid) {
         const int64_t n = (int64_t)J.nslot * J.ncol;
         if (n <= 0) return hipSuccess;
-        hipLaunchKernelGGL((k_levels<R, 1, false>), dim3(nblk(n)), dim3(256), 0, st, S, J);
+        if (J.flags & EV_P) hipLaunchKernelGGL((k_levels<R, 1, 2>), dim3(nblk(n)), dim3(256), 0, st, S, J);
+        else hipLaunchKernelGGL((k_levels<R, 1, 1>), dim3(nblk(n)), dim3(256), 0, st, S, J);
         return hipGetLastError();
     }
     switch (lc) {
@@ -568,6 +695,24 @@ hipError_t launch_query(const QueryJob<R, TQ>& J, hipStream_t st) {
     hipLaunchKernelGGL((k_query<R, TQ>), dim3(nblk(J.n)), dim3(256), 0, st, J);
     return hipGetLastError();
 }
+
+template <typename R>
+hipError_t launch_prune(const R* eta, int64_t ncell, const int32_t* cols, int64_t ncol, int nslot,
+                        const int* slot_dst, double swl, double* eb, const UvwRec<R>* rec, int64_t nrec,
+                        const double* sigma, int nl, double h, uint32_t* lvl, hipStream_t st) {
+    const int4 dst = make_int4(slot_dst[0], nslot > 1 ? slot_dst[1] : 0, nslot > 2 ? slot_dst[2] : 0,
+                               nslot > 3 ? slot_dst[3] : 0);
+    hipLaunchKernelGGL((k_eta_bounds<R>), dim3(1), dim3(1024), 0, st, eta, ncell, cols, ncol, nslot, dst, swl, eb);
+    if (hipGetLastError() != hipSuccess) return hipGetLastError();
+    if (nrec <= 0) return hipSuccess;
+    const double T = sizeof(R) == 4 ? 30.0 * 1.4426950408889634 : 40.0;
+    hipLaunchKernelGGL((k_prune<R>), dim3(nblk(nrec)), dim3(256), 0, st, rec, nrec, eb, sigma, nl, h, T, lvl);
+    return hipGetLastError();
+}
+template hipError_t launch_prune<double>(const double*, int64_t, const int32_t*, int64_t, int, const int*, double, double*,
+                                         const UvwRec<double>*, int64_t, const double*, int, double, uint32_t*, hipStream_t);
+template hipError_t launch_prune<float>(const float*, int64_t, const int32_t*, int64_t, int, const int*, double, double*,
+                                        const UvwRec<float>*, int64_t, const double*, int, double, uint32_t*, hipStream_t);
 
 template hipError_t launch_columns<double>(const SeaDev<double>&, const ColumnJob<double>&, bool, hipStream_t);
 template hipError_t launch_columns<float>(const SeaDev<float>&, const ColumnJob<float>&, bool, hipStream_t);
