@@ -85,6 +85,8 @@ def main():
     ap.add_argument("--cpu-cols", type=int, default=4)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-worker", nargs=2, default=None)
+    ap.add_argument("--calib-copy", action="store_true",
+                    help="run one plain 256 MiB device copy first (reference for PMC byte counters)")
     a = ap.parse_args()
     if a.cpu_worker:
         cpu_worker(int(a.cpu_worker[0]), a.cpu_worker[1])
@@ -133,6 +135,13 @@ def main():
         grid.refill()
         query()
         engine.sync()
+
+    if a.calib_copy:
+        src = torch.ones(64 * 1024 * 1024, dtype=torch.float32, device=dev)   # 256 MiB
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
 
     for _ in range(a.warmup):
         step()

@@ -388,6 +388,25 @@ int Grid::gradients() {
     return 0;
 }
 
+template <typename R>
+int Grid::pack() {
+    const size_t esz = sizeof(R);
+    const int64_t c = (int64_t)g.nx * g.ny, v = c * g.nl;
+    if (pk_eta_.alloc(8 * c * esz) || pk_C_.alloc(16 * v * esz) || pk_D_.alloc(16 * v * esz)) return -3;
+    PackJob<R> J;
+    J.nx = g.nx; J.ny = g.ny; J.nl = g.nl;
+    J.tid1 = (tstep + 1) % 4; J.tid2 = (tstep + 2) % 4;
+    auto P = [&](int f) { return (const R*)f_[f].get(); };
+    J.E = P(0); J.Ex = P(1); J.Ey = P(2); J.Et = P(3);
+    for (int f = 0; f < 3; ++f) {
+        J.F[f] = P(4 + f); J.Ft[f] = P(7 + f);
+        J.Fx[f] = P(10 + 3 * f); J.Fy[f] = P(11 + 3 * f); J.Fs[f] = P(12 + 3 * f);
+    }
+    J.pk_eta = (R*)pk_eta_.get(); J.pk_C = (R*)pk_C_.get(); J.pk_D = (R*)pk_D_.get();
+    PROF("k_pack", launch_pack<R>(J, Engine::get().stream()));
+    return 0;
+}
+
 int Grid::refill() {
     tstep = 0;
     t0 = d.t0;
@@ -400,6 +419,8 @@ int Grid::refill() {
                            : fill_slots<double>(4, ign ? tb : ta, dst, ign);
     if (rc) return rc;
     rc = prec == kFP32 ? gradients<float>() : gradients<double>();
+    if (rc) return rc;
+    rc = prec == kFP32 ? pack<float>() : pack<double>();
     if (rc) return rc;
     CFDWM_HIP(hipStreamSynchronize(Engine::get().stream()));
     return 0;
@@ -416,6 +437,8 @@ int Grid::force_update(double t, int* n) {
         int rc = prec == kFP32 ? fill_slots<float>(1, &t2, &slot, true) : fill_slots<double>(1, &t2, &slot, true);
         if (rc) return rc;
         rc = prec == kFP32 ? gradients<float>() : gradients<double>();
+        if (rc) return rc;
+        rc = prec == kFP32 ? pack<float>() : pack<double>();
         if (rc) return rc;
         ++k;
     }
@@ -450,11 +473,9 @@ int Grid::run_query(int64_t n, const TQ* t, double tconst, const TQ* x, const TQ
     QueryJob<R, TQ> J;
     J.n = n; J.g = g; J.tstep = tstep; J.t0 = t0; J.dt = d.dt;
     J.tan_a = d.tan_a; J.tan_b = d.tan_b; J.tanab = std::pow(std::tan(d.tan_a), d.tan_b);
-    auto P = [&](int f) { return (const R*)f_[f].get(); };
-    J.eta = SplineField<R>{P(0), P(3), P(1), P(2), nullptr};
-    J.u = SplineField<R>{P(4), P(7), P(10), P(11), P(12)};
-    J.v = SplineField<R>{P(5), P(8), P(13), P(14), P(15)};
-    J.w = SplineField<R>{P(6), P(9), P(16), P(17), P(18)};
+    J.pk_eta = (const R*)pk_eta_.get();
+    J.pk_C = (const R*)pk_C_.get();
+    J.pk_D = (const R*)pk_D_.get();
     J.t = t; J.tconst = tconst; J.x = x; J.y = y; J.z = z;
     J.ramp_scale = ramp;
     J.o_eta = eta; J.o_u = u; J.o_v = v; J.o_w = w;

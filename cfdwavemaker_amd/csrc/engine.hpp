@@ -45,8 +45,11 @@ struct LevelJob {
 };
 
 template <typename R>
-struct SplineField {
-    const R *f, *fdt, *fdx, *fdy, *fds;
+struct PackJob {
+    int nx, ny, nl, tid1, tid2;
+    const R *E, *Et, *Ex, *Ey;
+    const R *F[3], *Ft[3], *Fx[3], *Fy[3], *Fs[3];
+    R *pk_eta, *pk_C, *pk_D;
 };
 
 template <typename R, typename TQ>
@@ -55,7 +58,7 @@ struct QueryJob {
     GridGeom g{};
     int tstep = 0;
     double t0 = 0, dt = 1, tan_a = 0, tan_b = 1, tanab = 1;
-    SplineField<R> eta{}, u{}, v{}, w{};
+    const R *pk_eta = nullptr, *pk_C = nullptr, *pk_D = nullptr;
     const TQ *t = nullptr, *x = nullptr, *y = nullptr, *z = nullptr;
     double tconst = 0;
     const double* ramp_scale = nullptr;
@@ -73,6 +76,8 @@ template <typename R>
 hipError_t launch_grad_dt(const R* D, R* GD, int64_t per, int tstep, double dt, hipStream_t st);
 template <typename R, typename TQ>
 hipError_t launch_query(const QueryJob<R, TQ>& J, hipStream_t st);
+template <typename R>
+hipError_t launch_pack(const PackJob<R>& J, hipStream_t st);
 // eta range of the columns of a fill (minus swl) -> eb[2]; then per filtered
 // pair record the levels where its profile exponentials exceed e^{-T}
 template <typename R>
@@ -209,6 +214,9 @@ private:
     // fields: 0 ETA(4) 1 ETAdx(2) 2 ETAdy(2) 3 ETAdt(4) 4..6 U/V/W(4) 7..9 dt(4) 10..18 dx/dy/ds(2)
     DevBuf f_[19];
     DevBuf q_in_, q_out_;
+    DevBuf pk_eta_, pk_C_, pk_D_;             // query records of the two active slots
+    template <typename R>
+    int pack();
     template <typename R>
     int fill_slots(int nslot, const double* slot_t, const int* slot_dst, bool use_ignore);
     template <typename R>

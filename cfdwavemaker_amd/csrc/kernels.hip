@@ -403,6 +403,49 @@ __global__ __launch_bounds__(256) void k_grad_dt(const R* __restrict__ D, R* __r
 }
 
 // ================================================================ spline query
+// The two time slots a query reads, (tstep+1)%4 and (tstep+2)%4, are packed
+// after every fill/update by k_pack:
+//   column record (8 R): eta(tid1) eta(tid2) eta_t(tid1) eta_t(tid2) eta_x(0) eta_x(1) eta_y(0) eta_y(1)
+//   node records C and D (16 R each): for u, v, w in turn f, f_t, f_x, f_y, f_s,
+//     C from time slot tid1 and derivative slot 0, D from tid2 and derivative slot 1.
+// Levels are the fastest index, so the corners l and l+1 of a query share a
+// cache line and one query touches 8 node lines instead of 8 x 30 arrays.
+template <typename R>
+__global__ __launch_bounds__(256) void k_pack(PackJob<R> J) {
+    const int64_t per = (int64_t)J.nx * J.ny * J.nl;
+    const int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= per) return;
+    R* c = J.pk_C + 16 * n;
+    R* d = J.pk_D + 16 * n;
+#pragma unroll
+    for (int f = 0; f < 3; ++f) {
+        c[5 * f + 0] = J.F[f][J.tid1 * per + n];
+        c[5 * f + 1] = J.Ft[f][J.tid1 * per + n];
+        c[5 * f + 2] = J.Fx[f][n];
+        c[5 * f + 3] = J.Fy[f][n];
+        c[5 * f + 4] = J.Fs[f][n];
+        d[5 * f + 0] = J.F[f][J.tid2 * per + n];
+        d[5 * f + 1] = J.Ft[f][J.tid2 * per + n];
+        d[5 * f + 2] = J.Fx[f][per + n];
+        d[5 * f + 3] = J.Fy[f][per + n];
+        d[5 * f + 4] = J.Fs[f][per + n];
+    }
+    c[15] = 0;
+    d[15] = 0;
+    if (n % J.nl == 0) {
+        const int64_t pc = (int64_t)J.nx * J.ny, col = n / J.nl;
+        R* e = J.pk_eta + 8 * col;
+        e[0] = J.E[J.tid1 * pc + col];
+        e[1] = J.E[J.tid2 * pc + col];
+        e[2] = J.Et[J.tid1 * pc + col];
+        e[3] = J.Et[J.tid2 * pc + col];
+        e[4] = J.Ex[col];
+        e[5] = J.Ex[pc + col];
+        e[6] = J.Ey[col];
+        e[7] = J.Ey[pc + col];
+    }
+}
+
 namespace {
 template <typename R>
 __device__ __forceinline__ R herm(R f0, R f1, R a, R b, R u) {
@@ -411,19 +454,14 @@ __device__ __forceinline__ R herm(R f0, R f1, R a, R b, R u) {
 template <typename R>
 __device__ __forceinline__ R lin(R f0, R f1, R u) { return f0 * ((R)1 - u) + f1 * u; }
 
-// the 8 corners (i|i+1, j|j+1, l|l+1) with edge clamps (lsgrid_spline.cpp:375-384)
-struct Corners {
-    int64_t a[8];
-};
-__device__ __forceinline__ Corners corners(int nx, int ny, int nl, int i, int j, int l) {
+// node index of the 8 corners (i|i+1, j|j+1, l|l+1) with edge clamps (lsgrid_spline.cpp:375-384)
+__device__ __forceinline__ void corners(int64_t* a, int nx, int ny, int nl, int i, int j, int l) {
     const int i1 = min(i + 1, nx - 1), j1 = min(j + 1, ny - 1), l1 = min(l + 1, nl - 1);
-    Corners c;
     const int64_t I0 = (int64_t)i * ny, I1 = (int64_t)i1 * ny;
-    c.a[0] = (I0 + j) * nl + l;  c.a[1] = (I0 + j1) * nl + l;
-    c.a[2] = (I1 + j) * nl + l;  c.a[3] = (I1 + j1) * nl + l;
-    c.a[4] = (I0 + j) * nl + l1; c.a[5] = (I0 + j1) * nl + l1;
-    c.a[6] = (I1 + j) * nl + l1; c.a[7] = (I1 + j1) * nl + l1;
-    return c;
+    a[0] = (I0 + j) * nl + l;  a[1] = (I0 + j1) * nl + l;
+    a[2] = (I1 + j) * nl + l;  a[3] = (I1 + j1) * nl + l;
+    a[4] = (I0 + j) * nl + l1; a[5] = (I0 + j1) * nl + l1;
+    a[6] = (I1 + j) * nl + l1; a[7] = (I1 + j1) * nl + l1;
 }
 
 // cart2sigmaS (lsgrid_spline.cpp:107-112).  Points in (eta - swl, eta] give
@@ -438,20 +476,21 @@ __device__ __forceinline__ double cart2sigmaS(double zpt, double e, double h, do
     return 1 - atan(pow(-sigma * tanab, 1. / tb)) / ta;
 }
 
-// spline_interp_velo (lsgrid_spline.cpp:386-502)
+// spline_interp_velo (lsgrid_spline.cpp:386-502) for field f of the packed records
 template <typename R>
-__device__ R spline_velo(const SplineField<R>& F, int nx, int ny, int nl, int64_t per, int i, int j,
-                         int l0, int l1, R xd, R yd, R s0, R s1, R td, int tid1, int tid2,
+__device__ R spline_velo(const R* __restrict__ pC, const R* __restrict__ pD, int nx, int ny, int nl,
+                         int i, int j, int l0, int l1, int f, R xd, R yd, R s0, R s1, R td,
                          R dx, R dy, R ds, R dt) {
-    const Corners A = corners(nx, ny, nl, i, j, l0), B = corners(nx, ny, nl, i, j, l1);
+    int64_t A[8], B[8];
+    corners(A, nx, ny, nl, i, j, l0);
+    corners(B, nx, ny, nl, i, j, l1);
     R C[8], D[8], Ct[8], Dt[8], Cx[8], Dx[8], Cy[8], Dy[8], Cs[8], Ds[8];
 #pragma unroll
     for (int q = 0; q < 8; ++q) {
-        C[q] = F.f[tid1 * per + A.a[q]];   D[q] = F.f[tid2 * per + B.a[q]];
-        Ct[q] = F.fdt[tid1 * per + A.a[q]]; Dt[q] = F.fdt[tid2 * per + B.a[q]];
-        Cx[q] = F.fdx[A.a[q]];             Dx[q] = F.fdx[per + B.a[q]];
-        Cy[q] = F.fdy[A.a[q]];             Dy[q] = F.fdy[per + B.a[q]];
-        Cs[q] = F.fds[A.a[q]];             Ds[q] = F.fds[per + B.a[q]];
+        const R* a = pC + 16 * A[q] + 5 * f;
+        const R* b = pD + 16 * B[q] + 5 * f;
+        C[q] = a[0]; Ct[q] = a[1]; Cx[q] = a[2]; Cy[q] = a[3]; Cs[q] = a[4];
+        D[q] = b[0]; Dt[q] = b[1]; Dx[q] = b[2]; Dy[q] = b[3]; Ds[q] = b[4];
     }
     const int lo[4] = {0, 1, 4, 5}, hi[4] = {2, 3, 6, 7};
     R Cxr[4], Dxr[4], Cyr[4], Dyr[4], Ctr[4], Dtr[4], Csr[4], Dsr[4];
@@ -499,7 +538,6 @@ __global__ __launch_bounds__(256) void k_query(QueryJob<R, TQ> J) {
     const GridGeom& g = J.g;
     const double tpt = J.t ? (double)J.t[p] : J.tconst;
     const double xpt = (double)J.x[p], ypt = (double)J.y[p], zpt = (double)J.z[p];
-    const int tid1 = (J.tstep + 1) % 4, tid2 = (J.tstep + 2) % 4;
     double cx = (xpt - g.x0) / g.dx;
     cx = cx < (g.nx - 1.) ? cx : (g.nx - 1.);
     cx = cx > 0. ? cx : 0.;
@@ -514,20 +552,15 @@ __global__ __launch_bounds__(256) void k_query(QueryJob<R, TQ> J) {
     td = td < 1. ? td : 1.;
     const int i = (int)ix, j = (int)iy;
     const int nx = g.nx, ny = g.ny, nl = g.nl;
-    const int64_t pc = (int64_t)nx * ny;
     const int j1 = min(j + 1, ny - 1), i1 = min(i + 1, nx - 1);
-    const int64_t a0 = (int64_t)i * ny + j, a1 = (int64_t)i * ny + j1;
-    const int64_t a2 = (int64_t)i1 * ny + j, a3 = (int64_t)i1 * ny + j1;
-    const R* E = J.eta.f;
-    const R C[4] = {E[tid1 * pc + a0], E[tid1 * pc + a1], E[tid1 * pc + a2], E[tid1 * pc + a3]};
-    const R D[4] = {E[tid2 * pc + a0], E[tid2 * pc + a1], E[tid2 * pc + a2], E[tid2 * pc + a3]};
-    const R* Et = J.eta.fdt;
-    const R Ct[4] = {Et[tid1 * pc + a0], Et[tid1 * pc + a1], Et[tid1 * pc + a2], Et[tid1 * pc + a3]};
-    const R Dt[4] = {Et[tid2 * pc + a0], Et[tid2 * pc + a1], Et[tid2 * pc + a2], Et[tid2 * pc + a3]};
-    const R* Ex = J.eta.fdx;
-    const R* Ey = J.eta.fdy;
-    const R Cx[4] = {Ex[a0], Ex[a1], Ex[a2], Ex[a3]}, Dx[4] = {Ex[pc + a0], Ex[pc + a1], Ex[pc + a2], Ex[pc + a3]};
-    const R Cy[4] = {Ey[a0], Ey[a1], Ey[a2], Ey[a3]}, Dy[4] = {Ey[pc + a0], Ey[pc + a1], Ey[pc + a2], Ey[pc + a3]};
+    const int64_t ac[4] = {(int64_t)i * ny + j, (int64_t)i * ny + j1, (int64_t)i1 * ny + j, (int64_t)i1 * ny + j1};
+    R C[4], D[4], Ct[4], Dt[4], Cx[4], Dx[4], Cy[4], Dy[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const R* e = J.pk_eta + 8 * ac[q];
+        C[q] = e[0]; D[q] = e[1]; Ct[q] = e[2]; Dt[q] = e[3];
+        Cx[q] = e[4]; Dx[q] = e[5]; Cy[q] = e[6]; Dy[q] = e[7];
+    }
     const R dx = (R)g.dx, dy = (R)g.dy, dt = (R)J.dt;
     const R a00 = dx * Cx[0] - (C[2] - C[0]), a01 = dx * Cx[1] - (C[3] - C[1]);
     const R b00 = -dx * Cx[2] + (C[2] - C[0]), b01 = -dx * Cx[3] + (C[3] - C[1]);
@@ -554,16 +587,15 @@ __global__ __launch_bounds__(256) void k_query(QueryJob<R, TQ> J) {
     int l0 = (int)k0, l1 = (int)k1;
     l0 = l0 < 0 ? 0 : (l0 > nl - 1 ? nl - 1 : l0);
     l1 = l1 < 0 ? 0 : (l1 > nl - 1 ? nl - 1 : l1);
-    const int64_t per = pc * nl;
     const R ds = (R)g.ds;
-    const R ux = spline_velo(J.u, nx, ny, nl, per, i, j, l0, l1, xd, yd, sd0, sd1, tdR, tid1, tid2, dx, dy, ds, dt);
-    const R vy = spline_velo(J.v, nx, ny, nl, per, i, j, l0, l1, xd, yd, sd0, sd1, tdR, tid1, tid2, dx, dy, ds, dt);
-    const R wz = spline_velo(J.w, nx, ny, nl, per, i, j, l0, l1, xd, yd, sd0, sd1, tdR, tid1, tid2, dx, dy, ds, dt);
     const double rmp = J.ramp_scale ? (double)J.ramp_scale[p] : 1.0;
     J.o_eta[p] = (TQ)((double)eta * rmp);
-    J.o_u[p] = (TQ)((double)ux * rmp);
-    J.o_v[p] = (TQ)((double)vy * rmp);
-    J.o_w[p] = (TQ)((double)wz * rmp);
+    TQ* outs[3] = {J.o_u, J.o_v, J.o_w};
+#pragma unroll 1
+    for (int f = 0; f < 3; ++f) {
+        const R v = spline_velo(J.pk_C, J.pk_D, nx, ny, nl, i, j, l0, l1, f, xd, yd, sd0, sd1, tdR, dx, dy, ds, dt);
+        outs[f][p] = (TQ)((double)v * rmp);
+    }
 }
 
 // ==================================================== profile-term pruning
@@ -688,6 +720,15 @@ hipError_t launch_grad_dt(const R* D, R* GD, int64_t per, int tstep, double dt, 
     hipLaunchKernelGGL((k_grad_dt<R>), dim3(nblk(per)), dim3(256), 0, st, D, GD, per, tstep, dt);
     return hipGetLastError();
 }
+
+template <typename R>
+hipError_t launch_pack(const PackJob<R>& J, hipStream_t st) {
+    const int64_t n = (int64_t)J.nx * J.ny * J.nl;
+    hipLaunchKernelGGL((k_pack<R>), dim3(nblk(n)), dim3(256), 0, st, J);
+    return hipGetLastError();
+}
+template hipError_t launch_pack<double>(const PackJob<double>&, hipStream_t);
+template hipError_t launch_pack<float>(const PackJob<float>&, hipStream_t);
 
 template <typename R, typename TQ>
 hipError_t launch_query(const QueryJob<R, TQ>& J, hipStream_t st) {
