@@ -75,6 +75,33 @@ def cpu_baseline(ncols: int, threads: int):
             return json.load(f)
 
 
+def rocprof_summary(db_path: str, command: str) -> str:
+    """Markdown summary of a rocprofv3 results database: per kernel and launch
+    size the dispatch count and durations, and the mean of any PMC counter
+    (KiB per dispatch)."""
+    import sqlite3
+    con = sqlite3.connect(db_path)
+    lines = [f"command: `{command}`", ""]
+    rows = con.execute("select name, grid_x, count(*), sum(duration), avg(duration), min(duration), max(duration) "
+                       "from kernels group by name, grid_x order by sum(duration) desc").fetchall()
+    if rows:
+        lines += ["| kernel | grid_x | calls | total ms | avg ms | min ms | max ms |", "|---|---|---|---|---|---|---|"]
+        for nm, gx, n, tot, avg, mn, mx in rows:
+            if tot < 1e4:
+                continue
+            short = nm.split("(")[0].replace("void ", "")
+            lines.append(f"| `{short}` | {gx} | {n} | {tot / 1e6:.3f} | {avg / 1e6:.3f} | {mn / 1e6:.3f} | {mx / 1e6:.3f} |")
+    pmc = con.execute("select kernel_name, grid_size, counter_name, count(*), avg(value), avg(duration) "
+                      "from counters_collection group by kernel_name, grid_size, counter_name "
+                      "order by avg(duration) desc").fetchall()
+    if pmc:
+        lines += ["", "| kernel | grid_size | counter | dispatches | mean KiB/dispatch | mean ms |", "|---|---|---|---|---|---|"]
+        for nm, gs, cn, n, v, d in pmc:
+            short = nm.split("(")[0].replace("void ", "")
+            lines.append(f"| `{short}` | {gs} | {cn} | {n} | {v:.1f} | {d / 1e6:.3f} |")
+    return "\n".join(lines) + "\n"
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -85,9 +112,14 @@ def main():
     ap.add_argument("--cpu-cols", type=int, default=4)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-worker", nargs=2, default=None)
+    ap.add_argument("--rocprof-summary", nargs=2, metavar=("DB", "COMMAND"), default=None,
+                    help="print a markdown summary of a rocprofv3 results database and exit")
     ap.add_argument("--calib-copy", action="store_true",
                     help="run one plain 256 MiB device copy first (reference for PMC byte counters)")
     a = ap.parse_args()
+    if a.rocprof_summary:
+        print(rocprof_summary(*a.rocprof_summary), end="")
+        return
     if a.cpu_worker:
         cpu_worker(int(a.cpu_worker[0]), a.cpu_worker[1])
         return

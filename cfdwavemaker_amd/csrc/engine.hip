@@ -2,6 +2,7 @@
 // launch orchestration for the column / level / gradient / query kernels.
 #include "engine.hpp"
 
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 
@@ -205,6 +206,56 @@ int Sea::upload(bool with_direct) {
     return prec == kFP32 ? upload_t<float>(with_direct) : upload_t<double>(with_direct);
 }
 
+const Segment* Sea::segments(bool direct, int nseg) {
+    const auto key = std::make_pair(direct, nseg);
+    auto it = seg_cache_.find(key);
+    if (it != seg_cache_.end()) return (const Segment*)it->second->get();
+    const PairTable& P = direct ? tab.direct : tab.filt;
+    const int N = tab.comp.n;
+    const std::vector<int64_t>& ro = P.rowoff;      // N + 1 entries, ro[N] = total
+    const int64_t total = ro[N];
+    // row holding record e < total: the last i with ro[i] <= e
+    auto row_of = [&](int64_t e) { return (int)(std::upper_bound(ro.begin(), ro.begin() + N + 1, e) - ro.begin()) - 1; };
+    auto first_at_least = [&](int64_t e) { return (int)(std::lower_bound(ro.begin(), ro.begin() + N, e) - ro.begin()); };
+    std::vector<Segment> sg(nseg);
+    for (int s = 0; s < nseg; ++s) {
+        const int64_t e0 = total * s / nseg, e1 = total * (s + 1) / nseg;
+        Segment& g = sg[s];
+        g.own_lo = s == 0 ? 0 : first_at_least(e0);
+        g.own_hi = s == nseg - 1 ? N : first_at_least(e1);
+        if (e1 > e0) {
+            g.rlo = row_of(e0);
+            g.mstart = g.rlo + 1 + (int)(e0 - ro[g.rlo]);
+            const int rl = row_of(e1 - 1);
+            g.rhi = rl + 1;
+            g.mend = rl + 1 + (int)(e1 - ro[rl]);
+        } else {
+            g.rlo = g.rhi = g.own_lo;
+            g.mstart = g.mend = 0;
+        }
+        g.pad0 = g.pad1 = 0;
+    }
+    auto buf = std::make_unique<DevBuf>();
+    if (buf->upload(sg.data(), sg.size() * sizeof(Segment), Engine::get().stream())) return nullptr;
+    if (hipStreamSynchronize(Engine::get().stream()) != hipSuccess) return nullptr;
+    const Segment* r = (const Segment*)buf->get();
+    seg_cache_[key] = std::move(buf);
+    return r;
+}
+
+namespace {
+// Segments so that about 256 CUs x 4 SIMDs x 4 waves of threads are busy,
+// with at least 64 pair records per segment.
+int pick_nseg(int64_t threads, int64_t nrec) {
+    if (const char* e = std::getenv("CFDWM_NSEG")) return std::max(1, std::atoi(e));
+    const int64_t target = 256LL * 4 * 4 * 64;
+    int64_t s = (target + threads - 1) / std::max<int64_t>(threads, 1);
+    s = std::min<int64_t>(s, std::max<int64_t>(1, nrec / 64));
+    s = std::min<int64_t>(s, 4096);
+    return (int)std::max<int64_t>(1, s);
+}
+}  // namespace
+
 template <>
 const SeaDev<double>& Sea::dev<double>() const { return d64_; }
 template <>
@@ -222,6 +273,13 @@ int run_points(Sea& s, int64_t n, const double* dx, const double* dy, const doub
         J.x = dx; J.y = dy; J.t = dt;
         J.want_eta2 = want_eta2 ? 1 : 0;
         J.o_eta1 = o; J.o_eta2 = o + n; J.o_gx = o + 2 * n; J.o_gy = o + 3 * n; J.o_gz = o + 4 * n; J.o_gp = o + 5 * n;
+        J.nseg = want_eta2 ? pick_nseg(n, s.table_records(false)) : 1;
+        J.seg = s.segments(false, J.nseg);
+        if (!J.seg) return -3;
+        if (J.nseg > 1) {
+            if (s.partial.alloc((size_t)J.nseg * n * 6 * sizeof(double))) return -3;
+            J.partial = (double*)s.partial.get();
+        }
         PROF("k_columns_points", launch_columns<R>(S, J, false, st));
     }
     if (want_lev) {
@@ -230,6 +288,14 @@ int run_points(Sea& s, int64_t n, const double* dx, const double* dy, const doub
         J.x = dx; J.y = dy; J.z = dz; J.t = dt;
         J.flags = flags;
         J.p_u = o + 6 * n; J.p_v = o + 7 * n; J.p_w = o + 8 * n; J.p_p = o + 9 * n;
+        const bool direct = (flags & EV_DIRECT) != 0;
+        J.nseg = (flags & EV_SEC) ? pick_nseg(n, s.table_records(direct)) : 1;
+        J.seg = s.segments(direct, J.nseg);
+        if (!J.seg) return -3;
+        if (J.nseg > 1) {
+            if (s.partial.alloc((size_t)J.nseg * n * 9 * sizeof(double))) return -3;
+            J.partial = (double*)s.partial.get();
+        }
         PROF("k_levels_points", launch_levels<R>(S, J, false, 1, st));
     }
     return 0;
@@ -244,6 +310,10 @@ int Sea::eval_points(int64_t n, const double* x, const double* y, const double* 
     hipStream_t st = Engine::get().stream();
     const bool want_cols = eta1 || eta2 || grads4;
     const bool want_lev = u || v || w || p;
+    if (want_lev && (flags & EV_DIRECT) && tab.direct.rowoff.empty()) {
+        set_error("the unfiltered (direct) pair table was not built for this sea");
+        return -1;
+    }
     const double *dx = x, *dy = y, *dz = z, *dt = t;
     if (!device_ptrs) {
         if (pts_in.alloc(4 * n * sizeof(double))) return -3;
@@ -351,6 +421,17 @@ int Grid::fill_slots(int nslot, const double* slot_t, const int* slot_dst, bool 
     for (int s = 0; s < nslot; ++s) { C.slot_t[s] = slot_t[s]; C.slot_dst[s] = slot_dst[s]; }
     C.eta_grid = (R*)f_[0].get();
     C.want_eta2 = 1;
+    const int64_t nrec = sea->nfilt();
+    C.nseg = pick_nseg(C.ncol * nslot, nrec);
+    C.seg = sea->segments(false, C.nseg);
+    if (!C.seg) return -3;
+    const int64_t lthr = level_threads(nslot, C.ncol, g.nl, level_chunk, true);
+    const int lseg = pick_nseg(lthr, nrec);
+    {
+        const size_t need = std::max((size_t)C.nseg * nslot * C.ncol * 6, (size_t)lseg * lthr * (3 * level_chunk + 6));
+        if ((C.nseg > 1 || lseg > 1) && partial_.alloc(need * sizeof(double))) return -3;
+    }
+    if (C.nseg > 1) C.partial = (double*)partial_.get();
     // initialize_kinematics: ETA = eta1 + eta2 + swl (lsgrid_spline.cpp:700)
     PROF("k_columns_grid", launch_columns<R>(S, C, true, st));
     LevelJob<R> L;
@@ -360,8 +441,11 @@ int Grid::fill_slots(int nslot, const double* slot_t, const int* slot_dst, bool 
     L.sigma = (const double*)sigma_d_.get();
     L.o_u = (R*)f_[4].get(); L.o_v = (R*)f_[5].get(); L.o_w = (R*)f_[6].get();
     L.flags = EV_LIN | EV_SEC | EV_TAYLOR;
+    L.nseg = lseg;
+    L.seg = sea->segments(false, lseg);
+    if (!L.seg) return -3;
+    if (lseg > 1) L.partial = (double*)partial_.get();
     if (prune_) {
-        const int64_t nrec = sea->nfilt();
         if (lvl_.alloc(std::max<int64_t>(1, nrec) * sizeof(uint32_t)) || eb_.alloc(2 * sizeof(double))) return -3;
         PROF("k_prune", launch_prune<R>((const R*)f_[0].get(), ncell, C.cols, C.ncol, nslot, slot_dst, g.swl,
                                         (double*)eb_.get(), S.fuvw, nrec, (const double*)sigma_d_.get(), g.nl, g.h,

@@ -26,6 +26,9 @@ struct ColumnJob {
     const double *x = nullptr, *y = nullptr, *t = nullptr;
     int want_eta2 = 1;
     double *o_eta1 = nullptr, *o_eta2 = nullptr, *o_gx = nullptr, *o_gy = nullptr, *o_gz = nullptr, *o_gp = nullptr;
+    const Segment* seg = nullptr;         // nseg segments of the filtered table (blockIdx.y)
+    int nseg = 1;
+    double* partial = nullptr;            // nseg > 1: [seg][thread][6] partial sums
 };
 
 template <typename R>
@@ -42,6 +45,9 @@ struct LevelJob {
     double *p_u = nullptr, *p_v = nullptr, *p_w = nullptr, *p_p = nullptr;
     uint32_t flags = 0;
     const uint32_t* lvl = nullptr;        // GRID: per filtered pair record, pruning level bounds
+    const Segment* seg = nullptr;         // nseg segments of the table in use (blockIdx.y)
+    int nseg = 1;
+    double* partial = nullptr;            // nseg > 1: [seg][thread][3 LC + 6] partial sums
 };
 
 template <typename R>
@@ -69,6 +75,9 @@ template <typename R>
 hipError_t launch_columns(const SeaDev<R>& S, const ColumnJob<R>& J, bool grid, hipStream_t st);
 template <typename R>
 hipError_t launch_levels(const SeaDev<R>& S, const LevelJob<R>& J, bool grid, int lc, hipStream_t st);
+// thread counts of one launch (partial buffers hold nseg x threads records)
+int64_t column_threads(int64_t nslot, int64_t ncol);
+int64_t level_threads(int64_t nslot, int64_t ncol, int nl, int lc, bool grid);
 template <typename R>
 hipError_t launch_grad_xyz(const R* D, R* GX, R* GY, R* GS, int nx, int ny, int L, int tid0, int tid1,
                            double dx, double dy, double ds, hipStream_t st);
@@ -159,6 +168,12 @@ public:
     int upload(bool with_direct);
     template <typename R>
     const SeaDev<R>& dev() const;
+    // balanced split of a pair table into nseg segments (cached on the device)
+    const Segment* segments(bool direct, int nseg);
+    int64_t table_records(bool direct) const {
+        return direct ? (int64_t)tab.direct.rec.size() : (int64_t)tab.filt.rec.size();
+    }
+    DevBuf partial;                          // split-K partial sums of point batches
     std::mutex mu;                           // serialises the per-point scratch use
     // scratch for point batches (inputs 4n doubles, outputs 6n doubles)
     DevBuf pts_in, pts_out;
@@ -167,6 +182,7 @@ public:
                     double* u, double* v, double* w, double* p, bool device_ptrs);
 private:
     DevBuf ph_, cr_, act_, frow_, fmhi_, fuvw_, fx_, drow_, duvw_, dx_;
+    std::map<std::pair<bool, int>, std::unique_ptr<DevBuf>> seg_cache_;
     SeaDev<double> d64_{};
     SeaDev<float> d32_{};
     template <typename R>
@@ -209,7 +225,7 @@ private:
     std::vector<double> sigma_;
     DevBuf sigma_d_, cols_all_d_, cols_act_d_;
     int64_t nact_ = 0;
-    DevBuf lvl_, eb_;
+    DevBuf lvl_, eb_, partial_;
     bool prune_ = true;
     // fields: 0 ETA(4) 1 ETAdx(2) 2 ETAdy(2) 3 ETAdt(4) 4..6 U/V/W(4) 7..9 dt(4) 10..18 dx/dy/ds(2)
     DevBuf f_[19];

@@ -64,49 +64,83 @@ __device__ __forceinline__ float pexp(float a) { return __builtin_amdgcn_exp2f(a
 
 }  // namespace
 
+// =================================================== thread -> column mapping
+// Column kernel: one thread per (slot, active column) or per point.
+template <typename R>
+struct ColPos {
+    int64_t q, col;
+    int slot;
+    double x, y, t;
+};
+template <typename R, bool GRID>
+__device__ __forceinline__ ColPos<R> col_pos(const ColumnJob<R>& J, int64_t tid) {
+    const int64_t total = J.nslot * J.ncol;
+    ColPos<R> P;
+    P.q = tid < total ? tid : total - 1;
+    P.col = 0;
+    P.slot = 0;
+    if (GRID) {
+        P.slot = (int)(P.q / J.ncol);
+        P.col = J.cols ? J.cols[P.q % J.ncol] : P.q % J.ncol;
+        const int i = (int)(P.col / J.g.ny), j = (int)(P.col % J.g.ny);
+        P.x = J.g.x0 + J.g.dx * i;
+        P.y = J.g.y0 + J.g.dy * j;
+        P.t = J.slot_t[P.slot];
+    } else {
+        P.x = J.x[P.q]; P.y = J.y[P.q]; P.t = J.t[P.q];
+    }
+    return P;
+}
+
+// final column outputs (eta of the slot, or the point arrays)
+template <typename R, bool GRID>
+__device__ __forceinline__ void col_store(const SeaDev<R>& S, const ColumnJob<R>& J, const ColPos<R>& P,
+                                          const double* v /* e1 e2 gx gy gz gp */) {
+    if (GRID) {
+        J.eta_grid[(int64_t)J.slot_dst[P.slot] * J.ncell + P.col] = (R)(v[0] + v[1] + S.swl);
+    } else {
+        J.o_eta1[P.q] = v[0]; J.o_eta2[P.q] = v[1];
+        J.o_gx[P.q] = v[2]; J.o_gy[P.q] = v[3]; J.o_gz[P.q] = v[4]; J.o_gp[P.q] = v[5];
+    }
+}
+
 // ============================================================ column kernel
-// Per column: eta1, filtered eta2, Taylor gradients (Irregular.cpp:178-293, 1125-1192).
-// GRID: columns are (slot, active column) of the s-grid; writes ETA of the slot.
-// POINTS: columns are points; writes eta1, eta2, gxz, gyz, gzz, gpz (FP64).
+// Per column: eta1, filtered eta2, Taylor gradients (Irregular.cpp:178-293, 1125-1192),
+// over the pair records of segment blockIdx.y (rows it owns for the per-row terms).
 template <typename R, bool GRID>
 __global__ __launch_bounds__(256) void k_columns(SeaDev<R> S, ColumnJob<R> J) {
     const int64_t tid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const int64_t total = J.nslot * J.ncol;
     const bool live = tid < total;
-    const int64_t q = live ? tid : total - 1;
-    double x, y, t;
-    int64_t col = 0;
-    int slot = 0;
-    if (GRID) {
-        slot = (int)(q / J.ncol);
-        col = J.cols ? J.cols[q % J.ncol] : q % J.ncol;
-        const int i = (int)(col / J.g.ny), j = (int)(col % J.g.ny);
-        x = J.g.x0 + J.g.dx * i;
-        y = J.g.y0 + J.g.dy * j;
-        t = J.slot_t[slot];
-    } else {
-        x = J.x[q]; y = J.y[q]; t = J.t[q];
-    }
-    const double dxp = x - S.x0, dyp = y - S.y0;
-    double e1 = 0, e2 = 0, gx = 0, gy = 0, gz = 0, gp = 0;   // FP64 totals, rows in R
-    for (int i = 0; i < S.n; ++i) {
+    const ColPos<R> P = col_pos<R, GRID>(J, tid);
+    const Segment sg = J.seg[blockIdx.y];
+    const double dxp = P.x - S.x0, dyp = P.y - S.y0, t = P.t;
+    double v[6] = {0, 0, 0, 0, 0, 0};     // e1 e2 gx gy gz gp: FP64 totals, rows in R
+    const int ia = min(sg.rlo, sg.own_lo), ib = max(sg.rhi, sg.own_hi);
+    for (int i = ia; i < ib; ++i) {
+        const bool own = i >= sg.own_lo && i < sg.own_hi;
+        const bool prs = J.want_eta2 && i >= sg.rlo && i < sg.rhi;
+        if (!own && !prs) continue;
         const PhaseRec pi = S.ph[i];
         const CompRec<R>& ci_ = S.cr[i];
         R ci, si;
         cs_of(phase_of(pi, dxp, dyp, t), ci, si);
         const R ri = (R)freq_ramp(t, pi.rt, pi.ts, pi.du);
         const R cr = ci * ri, sr = si * ri;
-        e1 += (double)(ci_.A * cr);
-        gx += (double)(ci_.gxz * cr);
-        gy += (double)(ci_.gyz * cr);
-        gz += (double)(ci_.gzz * sr);
-        gp += (double)(ci_.gpz * cr);
-        if (!J.want_eta2) continue;
-        e2 += (double)(ci_.e2d * (ci * ci - si * si) * ri);
-        const int mhi = S.fmhi[i];
+        if (own) {
+            v[0] += (double)(ci_.A * cr);
+            v[2] += (double)(ci_.gxz * cr);
+            v[3] += (double)(ci_.gyz * cr);
+            v[4] += (double)(ci_.gzz * sr);
+            v[5] += (double)(ci_.gpz * cr);
+            if (J.want_eta2) v[1] += (double)(ci_.e2d * (ci * ci - si * si) * ri);
+        }
+        if (!prs) continue;
+        const int m0 = i == sg.rlo ? sg.mstart : i + 1;
+        const int m1 = i == sg.rhi - 1 ? sg.mend : S.fmhi[i];
         const XRec<R>* __restrict__ xr = S.fx + S.frow[i] - (i + 1);
         R acc = 0;
-        for (int m = i + 1; m < mhi; ++m) {
+        for (int m = m0; m < m1; ++m) {
             R cm, sm;
             cs_of(phase_of(S.ph[m], dxp, dyp, t), cm, sm);
             const XRec<R> rec = xr[m];
@@ -114,19 +148,115 @@ __global__ __launch_bounds__(256) void k_columns(SeaDev<R> S, ColumnJob<R> J) {
             acc = fmaR(rec.xP, cc - ss, acc);
             acc = fmaR(rec.xM, cc + ss, acc);
         }
-        e2 += (double)acc;
+        v[1] += (double)acc;
     }
     if (!live) return;
-    if (GRID) {
-        J.eta_grid[(int64_t)J.slot_dst[slot] * J.ncell + col] = (R)(e1 + e2 + S.swl);
+    if (J.partial) {
+        double* o = J.partial + ((int64_t)blockIdx.y * total + tid) * 6;
+#pragma unroll
+        for (int k = 0; k < 6; ++k) o[k] = v[k];
     } else {
-        J.o_eta1[q] = e1; J.o_eta2[q] = e2;
-        J.o_gx[q] = gx; J.o_gy[q] = gy; J.o_gz[q] = gz; J.o_gp[q] = gp;
+        col_store<R, GRID>(S, J, P, v);
     }
 }
 
+// sum of the segment partials of k_columns
+template <typename R, bool GRID>
+__global__ __launch_bounds__(256) void k_columns_reduce(SeaDev<R> S, ColumnJob<R> J) {
+    const int64_t tid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t total = J.nslot * J.ncol;
+    if (tid >= total) return;
+    const ColPos<R> P = col_pos<R, GRID>(J, tid);
+    double v[6] = {0, 0, 0, 0, 0, 0};
+    for (int sgi = 0; sgi < J.nseg; ++sgi) {
+        const double* o = J.partial + ((int64_t)sgi * total + tid) * 6;
+#pragma unroll
+        for (int k = 0; k < 6; ++k) v[k] += o[k];
+    }
+    col_store<R, GRID>(S, J, P, v);
+}
+
 // ============================================================= level kernel
-// Per (column, chunk of LC levels): first order + second order + Taylor.
+// Thread -> (column, chunk of LC levels).  GRID: level chunks are the slowest
+// index and every chunk is padded to whole wavefronts, so the chunk (hence
+// the absolute level of each unrolled slot) is wave-uniform and the pruning
+// tests are scalar branches.  POINTS: one level per point.
+template <typename R, int LC, bool GRID>
+struct LevPos {
+    int64_t cq, col;
+    int slot, l0;
+    bool live;
+    double x, y, t, e, z1;
+    R zp[LC];
+};
+template <typename R, int LC, bool GRID>
+__device__ __forceinline__ LevPos<R, LC, GRID> lev_pos(const SeaDev<R>& S, const LevelJob<R>& J, int64_t tid,
+                                                       uint32_t fl) {
+    LevPos<R, LC, GRID> P;
+    const int64_t ncolt = (int64_t)J.nslot * J.ncol;
+    const int64_t ncolp = GRID ? (ncolt + 63) / 64 * 64 : ncolt;
+    const int nchunk = GRID ? (J.g.nl + LC - 1) / LC : 1;
+    const int64_t chunk = tid / ncolp;
+    const int64_t cq0 = tid - chunk * ncolp;
+    P.live = chunk < nchunk && cq0 < ncolt;
+    P.cq = cq0 < ncolt ? cq0 : ncolt - 1;
+    P.col = 0; P.slot = 0; P.l0 = 0; P.e = 0.; P.z1 = 0.;
+    if (GRID) {
+        P.l0 = __builtin_amdgcn_readfirstlane((int)(chunk < nchunk ? chunk : nchunk - 1) * LC);
+        P.slot = (int)(P.cq / J.ncol);
+        P.col = J.cols ? J.cols[P.cq % J.ncol] : P.cq % J.ncol;
+        const int i = (int)(P.col / J.g.ny), j = (int)(P.col % J.g.ny);
+        P.x = J.g.x0 + J.g.dx * i;
+        P.y = J.g.y0 + J.g.dy * j;
+        P.t = J.slot_t[P.slot];
+        P.e = (double)J.eta_grid[(int64_t)J.slot_dst[P.slot] * J.ncell + P.col] - J.g.swl;
+#pragma unroll
+        for (int l = 0; l < LC; ++l) {
+            const int lv = P.l0 + l < J.g.nl ? P.l0 + l : J.g.nl - 1;
+            const double z = P.e + J.sigma[lv] * (J.g.h + P.e);          // s2z (lsgrid_spline.cpp:85-87)
+            P.zp[l] = (R)(z < 0. ? z : 0.);                                // min(0, z), no -swl (:711)
+        }
+    } else {
+        P.x = J.x[P.cq]; P.y = J.y[P.cq]; P.t = J.t[P.cq];
+        const double z = J.z[P.cq];
+        double zz = z;
+        if (fl & EV_ZCLAMP) zz = (z - S.swl) < 0. ? (z - S.swl) : 0.;
+        else if (fl & EV_ZSHIFT) zz = z - S.swl;
+#pragma unroll
+        for (int l = 0; l < LC; ++l) P.zp[l] = (R)zz;
+        P.z1 = (z - S.swl) > 0. ? (z - S.swl) : 0.;
+    }
+    return P;
+}
+
+// final level outputs.  a = {U[LC], V[LC], W[LC], gx, gy, gz, gp, P, P2}
+template <typename R, int LC, bool GRID, bool WP>
+__device__ __forceinline__ void lev_store(const SeaDev<R>& S, const LevelJob<R>& J, uint32_t fl,
+                                          const LevPos<R, LC, GRID>& P, const double* a) {
+    const double gx = a[3 * LC], gy = a[3 * LC + 1], gz = a[3 * LC + 2], gp = a[3 * LC + 3];
+    if (GRID) {
+        const int64_t o = ((int64_t)J.slot_dst[P.slot] * J.ncell + P.col) * J.g.nl + P.l0;
+#pragma unroll
+        for (int l = 0; l < LC; ++l) {
+            if (P.l0 + l >= J.g.nl) break;
+            const double z = P.e + J.sigma[P.l0 + l] * (J.g.h + P.e);
+            const double zt = z - J.g.swl > 0. ? z - J.g.swl : 0.;      // max(0, z - swl)
+            J.o_u[o + l] = (R)(a[l] + gx * zt);
+            J.o_v[o + l] = (R)(a[LC + l] + gy * zt);
+            J.o_w[o + l] = (R)(a[2 * LC + l] + gz * zt);
+        }
+    } else {
+        const bool tay = (fl & EV_TAYLOR) != 0;
+        const double z1 = P.z1;
+        J.p_u[P.cq] = a[0] + (tay ? gx * z1 : 0.);
+        J.p_v[P.cq] = a[LC] + (tay ? gy * z1 : 0.);
+        J.p_w[P.cq] = a[2 * LC] + (tay ? gz * z1 : 0.);
+        if (WP) J.p_p[P.cq] = a[3 * LC + 4] + S.rho * a[3 * LC + 5] + (tay ? gp * z1 : 0.);
+    }
+}
+
+// Per (column, chunk of LC levels): first order + second order + Taylor over
+// the pair records of segment blockIdx.y.
 // MODE 0 (GRID, lsgrid_spline.cpp:708-721): z from the sigma map of the
 //   column's eta, profiles at min(0, z), Taylor with max(0, z - swl), filtered
 //   uvw2 terms.  MODE 1/2 (POINTS, one level per point, 2 = with pressure):
@@ -138,184 +268,168 @@ template <typename R, int LC, int MODE>
 __global__ __launch_bounds__(256) void k_levels(SeaDev<R> S, LevelJob<R> J) {
     constexpr bool GRID = MODE == 0;
     constexpr bool WP = MODE == 2;
+    constexpr int RS = 3 * LC + 6;
     const int64_t tid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    // GRID: level chunks are the slowest index and every chunk is padded to whole
-    // wavefronts, so the chunk (hence the absolute level of each unrolled slot)
-    // is wave-uniform and the pruning tests below are scalar branches.
-    const int64_t ncolt = (int64_t)J.nslot * J.ncol;
-    const int64_t ncolp = GRID ? (ncolt + 63) / 64 * 64 : ncolt;
-    const int nchunk = GRID ? (J.g.nl + LC - 1) / LC : 1;
-    const int64_t chunk = tid / ncolp;
-    const int64_t cq0 = tid - chunk * ncolp;
-    const bool live = chunk < nchunk && cq0 < ncolt;
-    const int64_t cq = cq0 < ncolt ? cq0 : ncolt - 1;
-    const int64_t q = cq;
     const uint32_t fl = GRID ? (uint32_t)(EV_LIN | EV_SEC | EV_TAYLOR) : J.flags;
-    double x, y, t, e = 0., z1 = 0.;
-    int64_t col = 0;
-    int slot = 0, l0 = 0;
+    const LevPos<R, LC, GRID> PP = lev_pos<R, LC, GRID>(S, J, tid, fl);
+    const int l0 = PP.l0;
     R zp[LC];
-    if (GRID) {
-        l0 = __builtin_amdgcn_readfirstlane((int)(chunk < nchunk ? chunk : nchunk - 1) * LC);
-        slot = (int)(cq / J.ncol);
-        col = J.cols ? J.cols[cq % J.ncol] : cq % J.ncol;
-        const int i = (int)(col / J.g.ny), j = (int)(col % J.g.ny);
-        x = J.g.x0 + J.g.dx * i;
-        y = J.g.y0 + J.g.dy * j;
-        t = J.slot_t[slot];
-        e = (double)J.eta_grid[(int64_t)J.slot_dst[slot] * J.ncell + col] - J.g.swl;
 #pragma unroll
-        for (int l = 0; l < LC; ++l) {
-            const int lv = l0 + l < J.g.nl ? l0 + l : J.g.nl - 1;
-            const double z = e + J.sigma[lv] * (J.g.h + e);             // s2z (lsgrid_spline.cpp:85-87)
-            zp[l] = (R)(z < 0. ? z : 0.);                                 // min(0, z), no -swl (:711)
-        }
-    } else {
-        x = J.x[q]; y = J.y[q]; t = J.t[q];
-        const double z = J.z[q];
-        double zz = z;
-        if (fl & EV_ZCLAMP) zz = (z - S.swl) < 0. ? (z - S.swl) : 0.;
-        else if (fl & EV_ZSHIFT) zz = z - S.swl;
-#pragma unroll
-        for (int l = 0; l < LC; ++l) zp[l] = (R)zz;
-        z1 = (z - S.swl) > 0. ? (z - S.swl) : 0.;
-    }
+    for (int l = 0; l < LC; ++l) zp[l] = PP.zp[l];
     const bool direct = !GRID && (fl & EV_DIRECT) != 0;
     const bool lin = (fl & EV_LIN) != 0;
-    const double dxp = x - S.x0, dyp = y - S.y0;
+    const bool secf = (fl & EV_SEC) != 0;
+    const double dxp = PP.x - S.x0, dyp = PP.y - S.y0, t = PP.t;
+    const Segment sg = J.seg[blockIdx.y];
     double U[LC], V[LC], W[LC];
 #pragma unroll
     for (int l = 0; l < LC; ++l) U[l] = V[l] = W[l] = 0.;
-    double P = 0., P2 = 0., kx = 0., ky = 0., kz = 0.;   // pressure (MODE 2, LC == 1)
+    double P = 0., P2 = 0.;   // pressure (MODE 2, LC == 1)
     double gx = 0, gy = 0, gz = 0, gp = 0;
 
+    // dp2 kinetic terms (Irregular.cpp:755-756, 786-787): phase-free and
+    // separable, -(Xx_i Xx_m + Xy_i Xy_m) px_i px_m + Z_i Z_m pz_i pz_m over
+    // m > i, summed with running sums over descending i; plus the diagonal
+    // "second term".  Done once per point, by segment 0.
+    if (WP && direct && secf && blockIdx.y == 0) {
+        double kx = 0., ky = 0., kz = 0.;
+        for (int i = S.n - 1; i >= 0; --i) {
+            const PhaseRec pi = S.ph[i];
+            const CompRec<R> c = S.cr[i];
+            const double ri = freq_ramp(t, pi.rt, pi.ts, pi.du);
+            R E, G;
+            prof(c.pk, c.pg, zp[0], E, G);
+            const double dch = (double)(E + G), dsh = (double)(E - G);
+            P2 += ri * (-dch * ((double)c.Xx * kx + (double)c.Xy * ky) + dsh * (double)c.Zk * kz);
+            P2 += ri * (-(double)c.kd1 * dch + (double)c.kd2 * dsh);
+            kx += (double)c.Xx * dch;
+            ky += (double)c.Xy * dch;
+            kz += (double)c.Zk * dsh;
+        }
+    }
+
     const UvwRec<R>* __restrict__ tab = direct ? S.duvw : S.fuvw;
-    // rows run backwards so that the dp2 kinetic sums over m > i are running sums
-    for (int i = S.n - 1; i >= 0; --i) {
+    const int ia = min(sg.rlo, sg.own_lo), ib = max(sg.rhi, sg.own_hi);
+    for (int i = ia; i < ib; ++i) {
+        const bool own = i >= sg.own_lo && i < sg.own_hi;
+        const bool sec = secf && (direct || S.rowActive[i]);
+        const bool prs = sec && i >= sg.rlo && i < sg.rhi;
+        if (!own && !prs) continue;
         const PhaseRec pi = S.ph[i];
         const CompRec<R> c = S.cr[i];
         R ci, si;
         cs_of(phase_of(pi, dxp, dyp, t), ci, si);
         const R ri = (R)freq_ramp(t, pi.rt, pi.ts, pi.du);
         const R cr = ci * ri, sr = si * ri;
-        gx += (double)(c.gxz * cr);
-        gy += (double)(c.gyz * cr);
-        gz += (double)(c.gzz * sr);
-        if (WP) gp += (double)(c.gpz * cr);
-        const bool sec = (fl & EV_SEC) && (direct || S.rowActive[i]);
-        const bool kin = WP && sec && direct;
-        if (lin || kin) {
+        R ur[LC], vr[LC], wr[LC];
+        R pr = 0;
 #pragma unroll
-            for (int l = 0; l < LC; ++l) {
-                R E, G;
-                prof(c.pk, c.pg, zp[l], E, G);
-                const R ch = E + G, sh = E - G;
-                if (lin) {
+        for (int l = 0; l < LC; ++l) ur[l] = vr[l] = wr[l] = 0;
+        if (own) {
+            gx += (double)(c.gxz * cr);
+            gy += (double)(c.gyz * cr);
+            gz += (double)(c.gzz * sr);
+            if (WP) gp += (double)(c.gpz * cr);
+            if (lin) {
+#pragma unroll
+                for (int l = 0; l < LC; ++l) {
+                    R E, G;
+                    prof(c.pk, c.pg, zp[l], E, G);
+                    const R ch = E + G, sh = E - G;
                     U[l] += (double)(c.lu * cr * ch);
                     V[l] += (double)(c.lv * cr * ch);
                     W[l] += (double)(c.lw * sr * sh);
                     if (WP) P += (double)(c.lp * cr * ch);
                 }
-                if (kin) {
-                    // -(Xx_i Xx_m + Xy_i Xy_m) px_i px_m + Z_i Z_m pz_i pz_m over m > i,
-                    // and the diagonal "second term" (Irregular.cpp:755-756, 786-787)
-                    const double dch = (double)ch, dsh = (double)sh;
-                    P2 += (double)ri * (-dch * ((double)c.Xx * kx + (double)c.Xy * ky) + dsh * (double)c.Zk * kz);
-                    P2 += (double)ri * (-(double)c.kd1 * dch + (double)c.kd2 * dsh);
-                    kx += (double)c.Xx * dch;
-                    ky += (double)c.Xy * dch;
-                    kz += (double)c.Zk * dsh;
-                }
             }
-        }
-        if (!sec) continue;
-        R ur[LC], vr[LC], wr[LC];
-        R pr = 0;
-        // diagonal self-interaction (profile of 2k)
-        {
-            const R c2 = (ci * ci - si * si) * ri, s2 = (R)2 * si * ci * ri;
-            const R du = direct ? c.ddu : c.dfu, dv = direct ? c.ddv : c.dfv, dw = direct ? c.ddw : c.dfw;
-#pragma unroll
-            for (int l = 0; l < LC; ++l) {
-                R E, G;
-                prof(c.p2k, c.p2g, zp[l], E, G);
-                const R ch = E + G, sh = E - G;
-                ur[l] = du * c2 * ch;
-                vr[l] = dv * c2 * ch;
-                wr[l] = dw * s2 * sh;
-                if (WP && direct) pr = c.ddp * c2 * ch;
-            }
-        }
-        // pair terms, m in (i, mhi)
-        const int mhi = direct ? S.n : S.fmhi[i];
-        const int64_t base = (direct ? S.drow[i] : S.frow[i]) - (i + 1);
-        const UvwRec<R>* __restrict__ rr = tab + base;
-        const XRec<R>* __restrict__ xr = S.dx + base;
-        const uint32_t* __restrict__ lvr = J.lvl + base;
-        for (int m = i + 1; m < mhi; ++m) {
-            R cm, sm;
-            cs_of(phase_of(S.ph[m], dxp, dyp, t), cm, sm);
-            const UvwRec<R> r = rr[m];
-            // levels at which e^{kz} / e^{-k(z+2h)} can exceed the pruning
-            // threshold (bounds from this launch's eta range); 255/0: none
-            uint32_t lv = 0x00ff00ffu;
-            if (GRID && J.lvl) lv = lvr[m];
-            const R cc = cr * cm, ss = sr * sm, sc = sr * cm, cs = cr * sm;
-            const R cP = cc - ss, cM = cc + ss, sP = sc + cs, sM = sc - cs;
-            const R uP = r.cuP * cP, vP = r.cvP * cP, wP = r.cwP * sP;
-            const R uM = r.cuM * cM, vM = r.cvM * cM, wM = r.cwM * sM;
-            R pP = 0, pM = 0;
-            if (WP && direct) {
-                const XRec<R> xq = xr[m];
-                pP = xq.xP * cP;
-                pM = xq.xM * cM;
-            }
-            if (GRID && J.lvl) {
-                const int lEP = (int)(lv & 255u), lGP = (int)((lv >> 8) & 255u);
-                const int lEM = (int)((lv >> 16) & 255u), lGM = (int)(lv >> 24);
+            if (sec) {
+                // diagonal self-interaction (profile of 2k)
+                const R c2 = (ci * ci - si * si) * ri, s2 = (R)2 * si * ci * ri;
+                const R du = direct ? c.ddu : c.dfu, dv = direct ? c.ddv : c.dfv, dw = direct ? c.ddw : c.dfw;
 #pragma unroll
                 for (int l = 0; l < LC; ++l) {
-                    const int L = l0 + l;
-                    // separate blocks for e^{kz} and e^{-k(z+2h)}; the empty asm keeps
-                    // the compiler from speculating a skipped exponential into a select
-                    if (L >= lEP) {
-                        __asm__ volatile("");
-                        const R E = pexp(r.pkP * zp[l]);
-                        ur[l] = fmaR(uP, E, ur[l]); vr[l] = fmaR(vP, E, vr[l]); wr[l] = fmaR(wP, E, wr[l]);
-                    }
-                    if (L < lGP) {
-                        __asm__ volatile("");
-                        const R G = pexp(fmaR(-r.pkP, zp[l], r.pgP));
-                        ur[l] = fmaR(uP, G, ur[l]); vr[l] = fmaR(vP, G, vr[l]); wr[l] = fmaR(-wP, G, wr[l]);
-                    }
-                    if (L >= lEM) {
-                        __asm__ volatile("");
-                        const R E = pexp(r.pkM * zp[l]);
-                        ur[l] = fmaR(uM, E, ur[l]); vr[l] = fmaR(vM, E, vr[l]); wr[l] = fmaR(wM, E, wr[l]);
-                    }
-                    if (L < lGM) {
-                        __asm__ volatile("");
-                        const R G = pexp(fmaR(-r.pkM, zp[l], r.pgM));
-                        ur[l] = fmaR(uM, G, ur[l]); vr[l] = fmaR(vM, G, vr[l]); wr[l] = fmaR(-wM, G, wr[l]);
-                    }
+                    R E, G;
+                    prof(c.p2k, c.p2g, zp[l], E, G);
+                    const R ch = E + G, sh = E - G;
+                    ur[l] = du * c2 * ch;
+                    vr[l] = dv * c2 * ch;
+                    wr[l] = dw * s2 * sh;
+                    if (WP && direct) pr = c.ddp * c2 * ch;
                 }
-                continue;
             }
+        }
+        if (prs) {
+            // pair terms, m in [m0, m1)
+            const int m0 = i == sg.rlo ? sg.mstart : i + 1;
+            const int m1 = i == sg.rhi - 1 ? sg.mend : (direct ? S.n : S.fmhi[i]);
+            const int64_t base = (direct ? S.drow[i] : S.frow[i]) - (i + 1);
+            const UvwRec<R>* __restrict__ rr = tab + base;
+            const XRec<R>* __restrict__ xr = S.dx + base;
+            const uint32_t* __restrict__ lvr = J.lvl + base;
+            for (int m = m0; m < m1; ++m) {
+                R cm, sm;
+                cs_of(phase_of(S.ph[m], dxp, dyp, t), cm, sm);
+                const UvwRec<R> r = rr[m];
+                const R cc = cr * cm, ss = sr * sm, sc = sr * cm, cs = cr * sm;
+                const R cP = cc - ss, cM = cc + ss, sP = sc + cs, sM = sc - cs;
+                const R uP = r.cuP * cP, vP = r.cvP * cP, wP = r.cwP * sP;
+                const R uM = r.cuM * cM, vM = r.cvM * cM, wM = r.cwM * sM;
+                if (GRID && J.lvl) {
+                    // levels at which e^{kz} / e^{-k(z+2h)} can exceed the pruning
+                    // threshold (bounds from this launch's eta range)
+                    const uint32_t lv = lvr[m];
+                    const int lEP = (int)(lv & 255u), lGP = (int)((lv >> 8) & 255u);
+                    const int lEM = (int)((lv >> 16) & 255u), lGM = (int)(lv >> 24);
 #pragma unroll
-            for (int l = 0; l < LC; ++l) {
-                R E, G;
-                prof(r.pkP, r.pgP, zp[l], E, G);
-                R ch = E + G, sh = E - G;
-                ur[l] = fmaR(uP, ch, ur[l]);
-                vr[l] = fmaR(vP, ch, vr[l]);
-                wr[l] = fmaR(wP, sh, wr[l]);
-                if (WP && direct) pr = fmaR(pP, ch, pr);
-                prof(r.pkM, r.pgM, zp[l], E, G);
-                ch = E + G; sh = E - G;
-                ur[l] = fmaR(uM, ch, ur[l]);
-                vr[l] = fmaR(vM, ch, vr[l]);
-                wr[l] = fmaR(wM, sh, wr[l]);
-                if (WP && direct) pr = fmaR(pM, ch, pr);
+                    for (int l = 0; l < LC; ++l) {
+                        const int L = l0 + l;
+                        // separate blocks for e^{kz} and e^{-k(z+2h)}; the empty asm keeps
+                        // the compiler from speculating a skipped exponential into a select
+                        if (L >= lEP) {
+                            __asm__ volatile("");
+                            const R E = pexp(r.pkP * zp[l]);
+                            ur[l] = fmaR(uP, E, ur[l]); vr[l] = fmaR(vP, E, vr[l]); wr[l] = fmaR(wP, E, wr[l]);
+                        }
+                        if (L < lGP) {
+                            __asm__ volatile("");
+                            const R G = pexp(fmaR(-r.pkP, zp[l], r.pgP));
+                            ur[l] = fmaR(uP, G, ur[l]); vr[l] = fmaR(vP, G, vr[l]); wr[l] = fmaR(-wP, G, wr[l]);
+                        }
+                        if (L >= lEM) {
+                            __asm__ volatile("");
+                            const R E = pexp(r.pkM * zp[l]);
+                            ur[l] = fmaR(uM, E, ur[l]); vr[l] = fmaR(vM, E, vr[l]); wr[l] = fmaR(wM, E, wr[l]);
+                        }
+                        if (L < lGM) {
+                            __asm__ volatile("");
+                            const R G = pexp(fmaR(-r.pkM, zp[l], r.pgM));
+                            ur[l] = fmaR(uM, G, ur[l]); vr[l] = fmaR(vM, G, vr[l]); wr[l] = fmaR(-wM, G, wr[l]);
+                        }
+                    }
+                    continue;
+                }
+                R pP = 0, pM = 0;
+                if (WP && direct) {
+                    const XRec<R> xq = xr[m];
+                    pP = xq.xP * cP;
+                    pM = xq.xM * cM;
+                }
+#pragma unroll
+                for (int l = 0; l < LC; ++l) {
+                    R E, G;
+                    prof(r.pkP, r.pgP, zp[l], E, G);
+                    R ch = E + G, sh = E - G;
+                    ur[l] = fmaR(uP, ch, ur[l]);
+                    vr[l] = fmaR(vP, ch, vr[l]);
+                    wr[l] = fmaR(wP, sh, wr[l]);
+                    if (WP && direct) pr = fmaR(pP, ch, pr);
+                    prof(r.pkM, r.pgM, zp[l], E, G);
+                    ch = E + G; sh = E - G;
+                    ur[l] = fmaR(uM, ch, ur[l]);
+                    vr[l] = fmaR(vM, ch, vr[l]);
+                    wr[l] = fmaR(wM, sh, wr[l]);
+                    if (WP && direct) pr = fmaR(pM, ch, pr);
+                }
             }
         }
 #pragma unroll
@@ -326,25 +440,45 @@ __global__ __launch_bounds__(256) void k_levels(SeaDev<R> S, LevelJob<R> J) {
         }
         if (WP && direct) P2 += (double)pr;
     }
-    if (!live) return;
-    if (GRID) {
-        const int64_t o = ((int64_t)J.slot_dst[slot] * J.ncell + col) * J.g.nl + l0;
+    if (!PP.live) return;
+    double a[RS];
 #pragma unroll
-        for (int l = 0; l < LC; ++l) {
-            if (l0 + l >= J.g.nl) break;
-            const double z = e + J.sigma[l0 + l] * (J.g.h + e);
-            const double zt = z - J.g.swl > 0. ? z - J.g.swl : 0.;      // max(0, z - swl)
-            J.o_u[o + l] = (R)(U[l] + gx * zt);
-            J.o_v[o + l] = (R)(V[l] + gy * zt);
-            J.o_w[o + l] = (R)(W[l] + gz * zt);
-        }
+    for (int l = 0; l < LC; ++l) { a[l] = U[l]; a[LC + l] = V[l]; a[2 * LC + l] = W[l]; }
+    a[3 * LC] = gx; a[3 * LC + 1] = gy; a[3 * LC + 2] = gz; a[3 * LC + 3] = gp;
+    a[3 * LC + 4] = P; a[3 * LC + 5] = P2;
+    if (J.partial) {
+        const int64_t nthr = (GRID ? ((int64_t)J.nslot * J.ncol + 63) / 64 * 64 * ((J.g.nl + LC - 1) / LC)
+                                   : (int64_t)J.nslot * J.ncol);
+        double* o = J.partial + ((int64_t)blockIdx.y * nthr + tid) * RS;
+#pragma unroll
+        for (int k = 0; k < RS; ++k) o[k] = a[k];
     } else {
-        const bool tay = (fl & EV_TAYLOR) != 0;
-        J.p_u[q] = U[0] + (tay ? gx * z1 : 0.);
-        J.p_v[q] = V[0] + (tay ? gy * z1 : 0.);
-        J.p_w[q] = W[0] + (tay ? gz * z1 : 0.);
-        if (WP) J.p_p[q] = P + S.rho * P2 + (tay ? gp * z1 : 0.);
+        lev_store<R, LC, GRID, WP>(S, J, fl, PP, a);
     }
+}
+
+// sum of the segment partials of k_levels
+template <typename R, int LC, int MODE>
+__global__ __launch_bounds__(256) void k_levels_reduce(SeaDev<R> S, LevelJob<R> J) {
+    constexpr bool GRID = MODE == 0;
+    constexpr bool WP = MODE == 2;
+    constexpr int RS = 3 * LC + 6;
+    const int64_t tid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t nthr = (GRID ? ((int64_t)J.nslot * J.ncol + 63) / 64 * 64 * ((J.g.nl + LC - 1) / LC)
+                               : (int64_t)J.nslot * J.ncol);
+    if (tid >= nthr) return;
+    const uint32_t fl = GRID ? (uint32_t)(EV_LIN | EV_SEC | EV_TAYLOR) : J.flags;
+    const LevPos<R, LC, GRID> PP = lev_pos<R, LC, GRID>(S, J, tid, fl);
+    if (!PP.live) return;
+    double a[RS];
+#pragma unroll
+    for (int k = 0; k < RS; ++k) a[k] = 0.;
+    for (int sgi = 0; sgi < J.nseg; ++sgi) {
+        const double* o = J.partial + ((int64_t)sgi * nthr + tid) * RS;
+#pragma unroll
+        for (int k = 0; k < RS; ++k) a[k] += o[k];
+    }
+    lev_store<R, LC, GRID, WP>(S, J, fl, PP, a);
 }
 
 // ======================================================== FD gradient kernels
@@ -669,40 +803,44 @@ namespace {
 inline unsigned nblk(int64_t n, int b = 256) { return (unsigned)((n + b - 1) / b); }
 }
 
+int64_t column_threads(int64_t nslot, int64_t ncol) { return nslot * ncol; }
+int64_t level_threads(int64_t nslot, int64_t ncol, int nl, int lc, bool grid) {
+    if (!grid) return nslot * ncol;
+    return (nslot * ncol + 63) / 64 * 64 * ((nl + lc - 1) / lc);
+}
+
 template <typename R>
 hipError_t launch_columns(const SeaDev<R>& S, const ColumnJob<R>& J, bool grid, hipStream_t st) {
     const int64_t n = J.nslot * J.ncol;
     if (n <= 0) return hipSuccess;
-    if (grid) hipLaunchKernelGGL((k_columns<R, true>), dim3(nblk(n)), dim3(256), 0, st, S, J);
-    else hipLaunchKernelGGL((k_columns<R, false>), dim3(nblk(n)), dim3(256), 0, st, S, J);
+    const dim3 g1(nblk(n), J.nseg);
+    if (grid) hipLaunchKernelGGL((k_columns<R, true>), g1, dim3(256), 0, st, S, J);
+    else hipLaunchKernelGGL((k_columns<R, false>), g1, dim3(256), 0, st, S, J);
+    if (hipGetLastError() != hipSuccess || !J.partial) return hipGetLastError();
+    if (grid) hipLaunchKernelGGL((k_columns_reduce<R, true>), dim3(nblk(n)), dim3(256), 0, st, S, J);
+    else hipLaunchKernelGGL((k_columns_reduce<R, false>), dim3(nblk(n)), dim3(256), 0, st, S, J);
     return hipGetLastError();
 }
 
-template <typename R, int LC>
-static hipError_t launch_levels_lc(const SeaDev<R>& S, const LevelJob<R>& J, hipStream_t st) {
-    const int nchunk = (J.g.nl + LC - 1) / LC;
-    const int64_t ncolp = ((int64_t)J.nslot * J.ncol + 63) / 64 * 64;
-    const int64_t n = ncolp * nchunk;
+template <typename R, int LC, int MODE>
+static hipError_t launch_levels_t(const SeaDev<R>& S, const LevelJob<R>& J, hipStream_t st) {
+    const int64_t n = level_threads(J.nslot, J.ncol, J.g.nl, LC, MODE == 0);
     if (J.nslot * J.ncol <= 0) return hipSuccess;
-    hipLaunchKernelGGL((k_levels<R, LC, 0>), dim3(nblk(n)), dim3(256), 0, st, S, J);
+    hipLaunchKernelGGL((k_levels<R, LC, MODE>), dim3(nblk(n), J.nseg), dim3(256), 0, st, S, J);
+    if (hipGetLastError() != hipSuccess || !J.partial) return hipGetLastError();
+    hipLaunchKernelGGL((k_levels_reduce<R, LC, MODE>), dim3(nblk(n)), dim3(256), 0, st, S, J);
     return hipGetLastError();
 }
 
 template <typename R>
 hipError_t launch_levels(const SeaDev<R>& S, const LevelJob<R>& J, bool grid, int lc, hipStream_t st) {
-    if (!grid) {
-        const int64_t n = (int64_t)J.nslot * J.ncol;
-        if (n <= 0) return hipSuccess;
-        if (J.flags & EV_P) hipLaunchKernelGGL((k_levels<R, 1, 2>), dim3(nblk(n)), dim3(256), 0, st, S, J);
-        else hipLaunchKernelGGL((k_levels<R, 1, 1>), dim3(nblk(n)), dim3(256), 0, st, S, J);
-        return hipGetLastError();
-    }
+    if (!grid) return (J.flags & EV_P) ? launch_levels_t<R, 1, 2>(S, J, st) : launch_levels_t<R, 1, 1>(S, J, st);
     switch (lc) {
-    case 1: return launch_levels_lc<R, 1>(S, J, st);
-    case 2: return launch_levels_lc<R, 2>(S, J, st);
-    case 4: return launch_levels_lc<R, 4>(S, J, st);
-    case 8: return launch_levels_lc<R, 8>(S, J, st);
-    default: return launch_levels_lc<R, 16>(S, J, st);
+    case 1: return launch_levels_t<R, 1, 0>(S, J, st);
+    case 2: return launch_levels_t<R, 2, 0>(S, J, st);
+    case 4: return launch_levels_t<R, 4, 0>(S, J, st);
+    case 8: return launch_levels_t<R, 8, 0>(S, J, st);
+    default: return launch_levels_t<R, 16, 0>(S, J, st);
     }
 }
 

@@ -65,6 +65,14 @@ enum : uint32_t {
     EV_ZSHIFT = 64u,    // profiles at z - swl (order 0)
 };
 
+// A contiguous range of pair records (split-K over the pair sum).  Rows
+// [rlo, rhi) have pairs in it (row rlo from m = mstart, row rhi-1 up to
+// m = mend); rows [own_lo, own_hi) get their per-row terms (linear,
+// diagonal, Taylor gradients) from this segment.
+struct Segment {
+    int rlo, rhi, mstart, mend, own_lo, own_hi, pad0, pad1;
+};
+
 // s-grid geometry and ring-buffer slot bookkeeping (lsGridSpline).
 struct GridGeom {
     int nx, ny, nl;

@@ -123,3 +123,22 @@ def test_config3_columns_fp64_fp32_vs_oracle(eng, oracle):
             assert err <= tol, (prec, f, err)
         # elevation gate of the benchmark: |eta error| <= 1e-3 Hs
         assert np.max(np.abs(grid.export("ETA") - R["ETA"])) <= 1e-3 * hs
+
+
+def test_split_k_matches_single_segment(eng, monkeypatch):
+    """The pair sum split over segments (split-K) equals the single-segment sum."""
+    g = golden("points.npz")
+    s = sea_from_golden(g, "B")
+    t, x, y, z = g["B__pts"]
+    out = {}
+    for nseg in ("1", "7", "64"):
+        monkeypatch.setenv("CFDWM_NSEG", nseg)
+        sea = eng.Sea(s, precision=eng.FP64)
+        out[nseg] = {k: sea.eval(k, t, x, y, z) for k in ("eta", "u", "dp", "uvw2_w")}
+        grid = sea.grid((0.0, 120.0, -20.0, 20.0), 7, 5, 6, 0.0, 0.5, 0.7, 1.5, 30.0)
+        out[nseg]["UX"] = grid.export("UX")
+        out[nseg]["ETA"] = grid.export("ETA")
+    for k in out["1"]:
+        for nseg in ("7", "64"):
+            err = scaled_err(out[nseg][k], out["1"][k])
+            assert err <= 1e-12, (k, nseg, err)
