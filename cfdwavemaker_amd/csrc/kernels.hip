@@ -11,6 +11,8 @@
 // phi1_d*dz), src/lsgrid_spline.cpp (column fill, FD gradients, query).
 #include <hip/hip_runtime.h>
 
+#include <cstdlib>
+
 #include "engine.hpp"
 #include "kernels.hpp"
 
@@ -265,7 +267,7 @@ __device__ __forceinline__ void lev_store(const SeaDev<R>& S, const LevelJob<R>&
 // precision R and folded into FP64 totals, which keeps the FP32 error at the
 // level of the per-term rounding instead of growing over ~N^2/2 terms.
 template <typename R, int LC, int MODE>
-__global__ __launch_bounds__(256) void k_levels(SeaDev<R> S, LevelJob<R> J) {
+__device__ __forceinline__ void levels_body(const SeaDev<R>& S, const LevelJob<R>& J) {
     constexpr bool GRID = MODE == 0;
     constexpr bool WP = MODE == 2;
     constexpr int RS = 3 * LC + 6;
@@ -455,6 +457,20 @@ __global__ __launch_bounds__(256) void k_levels(SeaDev<R> S, LevelJob<R> J) {
     } else {
         lev_store<R, LC, GRID, WP>(S, J, fl, PP, a);
     }
+}
+
+template <typename R, int LC, int MODE>
+__global__ __launch_bounds__(256) void k_levels(SeaDev<R> S, LevelJob<R> J) {
+    levels_body<R, LC, MODE>(S, J);
+}
+
+// FP32 s-grid fill: the register budget is steered to 128 VGPRs so that four
+// waves share each SIMD (the kernel is issue-bound; more waves hide the
+// scalar-load and transcendental latencies).
+template <int LC>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8)))
+void k_levels_grid32(SeaDev<float> S, LevelJob<float> J) {
+    levels_body<float, LC, 0>(S, J);
 }
 
 // sum of the segment partials of k_levels
@@ -826,7 +842,14 @@ template <typename R, int LC, int MODE>
 static hipError_t launch_levels_t(const SeaDev<R>& S, const LevelJob<R>& J, hipStream_t st) {
     const int64_t n = level_threads(J.nslot, J.ncol, J.g.nl, LC, MODE == 0);
     if (J.nslot * J.ncol <= 0) return hipSuccess;
-    hipLaunchKernelGGL((k_levels<R, LC, MODE>), dim3(nblk(n), J.nseg), dim3(256), 0, st, S, J);
+    if constexpr (MODE == 0 && sizeof(R) == 4 && (LC == 8 || LC == 16)) {
+        if (!std::getenv("CFDWM_NO_WAVES_HINT"))
+            hipLaunchKernelGGL((k_levels_grid32<LC>), dim3(nblk(n), J.nseg), dim3(256), 0, st, S, J);
+        else
+            hipLaunchKernelGGL((k_levels<R, LC, MODE>), dim3(nblk(n), J.nseg), dim3(256), 0, st, S, J);
+    } else {
+        hipLaunchKernelGGL((k_levels<R, LC, MODE>), dim3(nblk(n), J.nseg), dim3(256), 0, st, S, J);
+    }
     if (hipGetLastError() != hipSuccess || !J.partial) return hipGetLastError();
     hipLaunchKernelGGL((k_levels_reduce<R, LC, MODE>), dim3(nblk(n)), dim3(256), 0, st, S, J);
     return hipGetLastError();
