@@ -31,7 +31,7 @@ BATCH_API = [
     "cfdwm_sea_terms", "cfdwm_eval", "cfdwm_grid_create", "cfdwm_grid_destroy",
     "cfdwm_grid_refill", "cfdwm_grid_force_update", "cfdwm_grid_state", "cfdwm_grid_export",
     "cfdwm_grid_query", "cfdwm_grid_query_device_f32", "cfdwm_sync", "cfdwm_last_error",
-    "cfdwm_prof_enable", "cfdwm_prof_read",
+    "cfdwm_prof_enable", "cfdwm_prof_read", "cfdwm_grid_qform_info", "cfdwm_depth_basis",
 ]
 
 
@@ -119,6 +119,10 @@ def load(path: str | None = None) -> C.CDLL:
     L.cfdwm_prof_enable.argtypes = [C.c_int]
     L.cfdwm_prof_read.restype = C.c_int
     L.cfdwm_prof_read.argtypes = [C.c_char_p, _dp, C.POINTER(_i64), C.c_int, C.c_int]
+    L.cfdwm_grid_qform_info.restype = C.c_int
+    L.cfdwm_grid_qform_info.argtypes = [C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_int), _dp, _dp]
+    L.cfdwm_depth_basis.restype = C.c_int
+    L.cfdwm_depth_basis.argtypes = [_d, _d, _d, C.c_int, _dp, _dp, _dp]
     if path is None:
         _LIB = L
     return L

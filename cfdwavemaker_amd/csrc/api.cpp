@@ -16,6 +16,7 @@
 
 #include "../../include/cfdwm.h"
 #include "engine.hpp"
+#include "depth_basis.hpp"
 #include "sea_tables.hpp"
 
 struct cfdwm_sea {
@@ -763,6 +764,26 @@ int cfdwm_grid_query_device_f32(cfdwm_grid* g, int64_t n, double t, const float*
                                 const float* z, float* eta, float* u, float* v, float* w) {
     if (!g || n < 0 || !x || !y || !z || !eta || !u || !v || !w) { set_error("bad arguments"); return -1; }
     return g->g->query_device_f32(n, t, x, y, z, eta, u, v, w);
+}
+int cfdwm_grid_qform_info(cfdwm_grid* grid, int* nform, int* rank, double* mfma_flop, double* basis_err) {
+    if (!grid) return -1;
+    const Grid& g = *grid->g;
+    if (nform) *nform = g.qf_nform;
+    if (rank) *rank = g.qf_rank;
+    if (mfma_flop) *mfma_flop = g.qf_flop;
+    if (basis_err) *basis_err = g.qf_basis_err;
+    return g.qf_used ? 1 : 0;
+}
+int cfdwm_depth_basis(double kmax, double tmax, double tol, int rmax, double* mu, double* tau, double* err) {
+    if (rmax < 1 || rmax > 64 || !(kmax > 0) || !(tmax > 0)) return 0;
+    DepthBasis B;
+    const int rc = build_depth_basis(kmax, tmax, tol, rmax, B);
+    for (int q = 0; q < B.R; ++q) {
+        if (mu) mu[q] = B.mu[q];
+        if (tau) tau[q] = B.tau[q];
+    }
+    if (err) *err = B.err;
+    return rc == 0 ? B.R : -B.R;
 }
 int cfdwm_sync(void) {
     if (Engine::get().init()) return -2;

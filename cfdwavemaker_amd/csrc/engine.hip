@@ -1,10 +1,12 @@
 // Device engine: HIP stream, device-resident sea tables and s-grids,
 // launch orchestration for the column / level / gradient / query kernels.
 #include "engine.hpp"
+#include "depth_basis.hpp"
 
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <thread>
 
 namespace cfdwm {
 
@@ -244,6 +246,116 @@ const Segment* Sea::segments(bool direct, int nseg) {
 }
 
 namespace {
+// bf16 bits of a float, round to nearest even
+uint16_t bf16_bits(float f) {
+    uint32_t u;
+    std::memcpy(&u, &f, 4);
+    u += 0x7FFFu + ((u >> 16) & 1u);
+    return (uint16_t)(u >> 16);
+}
+float bf16_val(uint16_t b) {
+    const uint32_t u = (uint32_t)b << 16;
+    float f;
+    std::memcpy(&f, &u, 4);
+    return f;
+}
+// Irregular::freq_ramp (Irregular.cpp:200-204), host side
+double freq_ramp_host(double t, double rt, double ts, double du) {
+    const double a = 1. - ((t - ts - du + rt) / (rt + 1e-12));
+    const double b = (t - ts) / (rt + 1e-12);
+    const double down = a < 1. ? (a < 0. ? 0. : a) : 1.;
+    const double up = b < 1. ? (b < 0. ? 0. : b) : 1.;
+    return up * down;
+}
+}  // namespace
+
+// Tables of the quadratic forms over the filtered pairs (i, m), m in
+// [i+1, mhi[i]), split into bf16 hi + lo in the tiled operand layout (rows i,
+// k = m):  form 0 (eta2):  B1 = xP + xM, B2 = xM - xP;  per depth-basis rank q
+// (coefficients cP = c_q(kP), cM = c_q(kM)):  u: B1 = cuP cP + cuM cM,
+// B2 = cuM cM - cuP cP;  v likewise with cv;  w (sin-type): B1 = cwP cP + cwM cM,
+// B2 = cwP cP - cwM cM.
+const Sea::QfTables* Sea::qform_tables() {
+    if (qf_) return qf_.get();
+    const PairTable& P = tab.filt;
+    const int N = tab.comp.n;
+    const double h = tab.in.depth;
+    double kmx = 0;
+    for (const PairRec64& r : P.rec) kmx = std::max(kmx, std::max(std::fabs(r.kP), std::fabs(r.kM)));
+    double tol = 1e-6;
+    if (const char* e = std::getenv("CFDWM_QF_TOL")) tol = std::atof(e);
+    DepthBasis DB;
+    build_depth_basis(kmx * (1.0 + 1e-9) + 1e-12, 2.0 * h, tol, 32, DB);
+    const int R = DB.R;
+    const int nform = 1 + 3 * R;
+    const int npad = (N + 127) / 128 * 128, nkb = npad / 16;
+    const size_t nn = (size_t)npad * npad;
+    std::vector<uint16_t> b((size_t)nform * 4 * nn, 0);
+    std::vector<int8_t> ftype(nform, 0);
+    for (int q = 0; q < R; ++q) ftype[1 + 2 * R + q] = 1;
+    std::vector<int32_t> kend(npad / 64, 0);
+    for (int i = 0; i < N; ++i) {
+        int& ke = kend[i / 64];
+        ke = std::max(ke, std::min(npad, (P.mhi[i] + 15) / 16 * 16));
+    }
+    auto put = [&](int f, int k, size_t o, double v) {
+        uint16_t* d = b.data() + ((size_t)f * 4 + 2 * k) * nn;
+        const float x = (float)v;
+        const uint16_t hi = bf16_bits(x);
+        d[o] = hi;
+        d[o + nn] = bf16_bits(x - bf16_val(hi));
+    };
+    auto rows = [&](int i0, int i1) {
+        std::vector<double> cp(R), cm(R);
+        for (int i = i0; i < i1; ++i) {
+            const int64_t base = P.rowoff[i] - (i + 1);
+            for (int m = i + 1; m < P.mhi[i]; ++m) {
+                const PairRec64& r = P.rec[base + m];
+                const size_t o = ((size_t)(i >> 5) * nkb + (m >> 4)) * 512 + ((m >> 3) & 1) * 256 + (i & 31) * 8 + (m & 7);
+                put(0, 0, o, r.xP + r.xM);
+                put(0, 1, o, r.xM - r.xP);
+                DB.coeffs(r.kP, cp.data());
+                DB.coeffs(r.kM, cm.data());
+                for (int q = 0; q < R; ++q) {
+                    put(1 + q, 0, o, r.cuP * cp[q] + r.cuM * cm[q]);
+                    put(1 + q, 1, o, r.cuM * cm[q] - r.cuP * cp[q]);
+                    put(1 + R + q, 0, o, r.cvP * cp[q] + r.cvM * cm[q]);
+                    put(1 + R + q, 1, o, r.cvM * cm[q] - r.cvP * cp[q]);
+                    put(1 + 2 * R + q, 0, o, r.cwP * cp[q] + r.cwM * cm[q]);
+                    put(1 + 2 * R + q, 1, o, r.cwP * cp[q] - r.cwM * cm[q]);
+                }
+            }
+        }
+    };
+    {
+        // rows are disjoint tile rows of every table: split them over threads
+        const int nt = (int)std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+        std::vector<std::thread> th;
+        std::vector<int64_t> cum(N + 1, 0);
+        for (int i = 0; i < N; ++i) cum[i + 1] = cum[i] + std::max(0, P.mhi[i] - i - 1);
+        int i0 = 0;
+        for (int k = 0; k < nt; ++k) {
+            const int64_t goal = cum[N] * (k + 1) / nt;
+            int i1 = i0;
+            while (i1 < N && (cum[i1] < goal || k == nt - 1)) ++i1;
+            th.emplace_back(rows, i0, i1);
+            i0 = i1;
+        }
+        for (auto& t : th) t.join();
+    }
+    auto T = std::make_unique<QfTables>();
+    T->npad = npad; T->nform = nform; T->rank = R; T->mu = DB.mu; T->basis_err = DB.err;
+    T->kend_h = kend;
+    hipStream_t st = Engine::get().stream();
+    if (T->b.upload(b.data(), b.size() * sizeof(uint16_t), st)) return nullptr;
+    if (T->ftype.upload(ftype.data(), ftype.size(), st)) return nullptr;
+    if (T->kend.upload(kend.data(), kend.size() * sizeof(int32_t), st)) return nullptr;
+    if (hipStreamSynchronize(st) != hipSuccess) return nullptr;
+    qf_ = std::move(T);
+    return qf_.get();
+}
+
+namespace {
 // Segments so that about 256 CUs x 4 SIMDs x 4 waves of threads are busy,
 // with at least 64 pair records per segment.
 int pick_nseg(int64_t threads, int64_t nrec) {
@@ -404,6 +516,7 @@ int Grid::init() {
     while (level_chunk < d.nl && level_chunk < cap) level_chunk *= 2;
     if (const char* e = std::getenv("CFDWM_LEVEL_CHUNK")) level_chunk = std::atoi(e);
     if (const char* e = std::getenv("CFDWM_PRUNE")) prune_ = std::atoi(e) != 0;
+    if (const char* e = std::getenv("CFDWM_QFORM")) qform_ = std::atoi(e) != 0;
     return refill();
 }
 
@@ -423,10 +536,18 @@ int Grid::fill_slots(int nslot, const double* slot_t, const int* slot_dst, bool 
     C.want_eta2 = 1;
     const int64_t nrec = sea->nfilt();
     C.nseg = pick_nseg(C.ncol * nslot, nrec);
+    LevelJob<R> L;
+    bool qf_used = false;
+    if constexpr (sizeof(R) == 4) {
+        const int rc = qform_fill(nslot, slot_t, C.ncol, C.cols, C, L);
+        if (rc < 0) return rc;
+        qf_used = rc == 1;
+        if (qf_used) C.nseg = 1;             // pair sums done on the matrix cores
+    }
     C.seg = sea->segments(false, C.nseg);
     if (!C.seg) return -3;
     const int64_t lthr = level_threads(nslot, C.ncol, g.nl, level_chunk, true);
-    const int lseg = pick_nseg(lthr, nrec);
+    const int lseg = qf_used ? 1 : pick_nseg(lthr, nrec);
     {
         const size_t need = std::max((size_t)C.nseg * nslot * C.ncol * 6, (size_t)lseg * lthr * (3 * level_chunk + 6));
         if ((C.nseg > 1 || lseg > 1) && partial_.alloc(need * sizeof(double))) return -3;
@@ -434,7 +555,6 @@ int Grid::fill_slots(int nslot, const double* slot_t, const int* slot_dst, bool 
     if (C.nseg > 1) C.partial = (double*)partial_.get();
     // initialize_kinematics: ETA = eta1 + eta2 + swl (lsgrid_spline.cpp:700)
     PROF("k_columns_grid", launch_columns<R>(S, C, true, st));
-    LevelJob<R> L;
     L.nslot = nslot; L.ncol = C.ncol; L.ncell = ncell; L.cols = C.cols; L.g = g;
     for (int s = 0; s < nslot; ++s) { L.slot_t[s] = slot_t[s]; L.slot_dst[s] = slot_dst[s]; }
     L.eta_grid = (const R*)f_[0].get();
@@ -445,7 +565,7 @@ int Grid::fill_slots(int nslot, const double* slot_t, const int* slot_dst, bool 
     L.seg = sea->segments(false, lseg);
     if (!L.seg) return -3;
     if (lseg > 1) L.partial = (double*)partial_.get();
-    if (prune_) {
+    if (prune_ && !qf_used) {
         if (lvl_.alloc(std::max<int64_t>(1, nrec) * sizeof(uint32_t)) || eb_.alloc(2 * sizeof(double))) return -3;
         PROF("k_prune", launch_prune<R>((const R*)f_[0].get(), ncell, C.cols, C.ncol, nslot, slot_dst, g.swl,
                                         (double*)eb_.get(), S.fuvw, nrec, (const double*)sigma_d_.get(), g.nl, g.h,
@@ -454,6 +574,87 @@ int Grid::fill_slots(int nslot, const double* slot_t, const int* slot_dst, bool 
     }
     PROF("k_levels_grid", launch_levels<R>(S, L, true, level_chunk, st));
     return 0;
+}
+
+// Second-order pair sums of an FP32 fill as MFMA quadratic forms: the
+// filtered eta2 for the column pass and, through the separable depth basis,
+// the u/v/w pair sums for the level pass.  Returns 1 when used, 0 when not
+// applicable (switched off, no pairs, or frequency ramps that differ between
+// components at one of the slot times), < 0 on error.
+int Grid::qform_fill(int nslot, const double* slot_t, int64_t ncol, const int32_t* cols, ColumnJob<float>& C,
+                     LevelJob<float>& L) {
+    qf_used = false;
+    if (!qform_ || sea->nfilt() == 0) return 0;
+    const CompTable& c = sea->tab.comp;
+    double scale[4] = {1, 1, 1, 1};
+    for (int s = 0; s < nslot; ++s) {
+        const double r0 = freq_ramp_host(slot_t[s], c.rt[0], c.ts[0], c.du[0]);
+        for (int i = 1; i < c.n; ++i)
+            if (freq_ramp_host(slot_t[s], c.rt[i], c.ts[i], c.du[i]) != r0) return 0;
+        scale[s] = r0;
+    }
+    const Sea::QfTables* T = sea->qform_tables();
+    if (!T) return -3;
+    const int npad = T->npad, R = T->rank;
+    const double hs = sea->tab.in.depth;
+    // the depth basis covers t = -z in [0, 2 hs] and t = 2 hs + z: the grid's levels must lie above -2 hs
+    if (R > 32 || g.h > 2.0 * hs) return 0;
+    const int64_t P = (int64_t)nslot * ncol;
+    const int64_t ppad_all = (P + 127) / 128 * 128;
+    // Z operands of one chunk of points: 8 bytes per (point, component), at most 2 GiB
+    const int64_t cap = std::max<int64_t>(128, ((int64_t)1 << 31) / ((int64_t)npad * 8) / 128 * 128);
+    const int64_t chunk = std::min(ppad_all, cap);
+    const size_t zn = (size_t)chunk * npad;
+    if (zbuf_.alloc(zn * 4 * sizeof(uint16_t))) return -3;
+    if (qout_.alloc((size_t)T->nform * ppad_all * sizeof(float))) return -3;
+    QfJob Q;
+    Q.n = c.n; Q.npad = npad;
+    Q.grid = 1; Q.ncol = ncol; Q.cols = cols; Q.g = g;
+    for (int s = 0; s < nslot; ++s) Q.slot_t[s] = slot_t[s];
+    Q.x0 = sea->tab.in.x0; Q.y0 = sea->tab.in.y0;
+    Q.ph = sea->dev<float>().ph;
+    uint16_t* z = (uint16_t*)zbuf_.get();
+    Q.zc_hi = z; Q.zc_lo = z + zn; Q.zs_hi = z + 2 * zn; Q.zs_lo = z + 3 * zn;
+    Q.b = (const uint16_t*)T->b.get();
+    Q.ftype = (const int8_t*)T->ftype.get();
+    Q.nform = T->nform;
+    Q.kend = (const int32_t*)T->kend.get();
+    Q.out = (float*)qout_.get();
+    Q.ostride = ppad_all;
+    hipStream_t st = Engine::get().stream();
+    // executed MFMA flops per padded point and form: per 64-wide i tile,
+    // 64 x (kend - i0) products, two operand pairs (c and s), three bf16 products
+    double fpp = 0;
+    for (size_t t = 0; t < T->kend_h.size(); ++t)
+        fpp += 64.0 * std::max(0, T->kend_h[t] - (int)(64 * t)) * 2 * 3 * 2;
+    qf_flop = 0;
+    for (int64_t p0 = 0; p0 < P; p0 += chunk) {
+        Q.p0 = p0;
+        Q.P = std::min(chunk, P - p0);
+        Q.ppad = (Q.P + 127) / 128 * 128;
+        PROF("k_qf_zgen", launch_qf_zgen(Q, st));
+        PROF("k_qf_forms", launch_qf_forms(Q, st));
+        qf_flop += fpp * (double)Q.ppad * T->nform;
+    }
+    qf_used = true;
+    qf_nform = T->nform;
+    qf_rank = R;
+    qf_basis_err = T->basis_err;
+    C.pairs = 0;
+    C.q2 = (const float*)qout_.get();
+    C.nq2 = 1;
+    C.q2stride = ppad_all;
+    for (int s = 0; s < 4; ++s) C.q2scale[s] = scale[s];
+    L.qf = (const float*)qout_.get() + ppad_all;
+    L.qf_rank = R;
+    L.qf_stride = ppad_all;
+    constexpr double kL2e = 1.4426950408889634074;
+    for (int q = 0; q < R; ++q) {
+        L.qf_mu2[q] = (float)(T->mu[q] * kL2e);
+        L.qf_mg[q] = (float)(-2.0 * hs * T->mu[q] * kL2e);
+    }
+    for (int s = 0; s < 4; ++s) L.qf_scale[s] = scale[s];
+    return 1;
 }
 
 template <typename R>

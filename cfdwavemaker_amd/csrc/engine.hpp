@@ -25,6 +25,11 @@ struct ColumnJob {
     R* eta_grid = nullptr;
     const double *x = nullptr, *y = nullptr, *t = nullptr;
     int want_eta2 = 1;
+    int pairs = 1;                        // 0: eta2 pair sum comes from the MFMA partials q2
+    const float* q2 = nullptr;            // [nq2][q2stride] per-point partials (GRID)
+    int nq2 = 0;
+    int64_t q2stride = 0;
+    double q2scale[4] = {1, 1, 1, 1};     // per slot ramp factor of the partials
     double *o_eta1 = nullptr, *o_eta2 = nullptr, *o_gx = nullptr, *o_gy = nullptr, *o_gz = nullptr, *o_gp = nullptr;
     const Segment* seg = nullptr;         // nseg segments of the filtered table (blockIdx.y)
     int nseg = 1;
@@ -45,6 +50,14 @@ struct LevelJob {
     double *p_u = nullptr, *p_v = nullptr, *p_w = nullptr, *p_p = nullptr;
     uint32_t flags = 0;
     const uint32_t* lvl = nullptr;        // GRID: per filtered pair record, pruning level bounds
+    // GRID: pair sums from the MFMA quadratic forms instead of the pair loop:
+    // u, v, w += scale * sum_q (e^{mu_q z} +- e^{-mu_q (z + 2h)}) Q_{f,q}(column)
+    const float* qf = nullptr;            // [3 * qf_rank][qf_stride]
+    int qf_rank = 0;
+    int64_t qf_stride = 0;
+    float qf_mu2[32] = {};                // mu_q log2 e
+    float qf_mg[32] = {};                 // -2 h mu_q log2 e
+    double qf_scale[4] = {1, 1, 1, 1};
     const Segment* seg = nullptr;         // nseg segments of the table in use (blockIdx.y)
     int nseg = 1;
     double* partial = nullptr;            // nseg > 1: [seg][thread][3 LC + 6] partial sums
@@ -87,6 +100,9 @@ template <typename R, typename TQ>
 hipError_t launch_query(const QueryJob<R, TQ>& J, hipStream_t st);
 template <typename R>
 hipError_t launch_pack(const PackJob<R>& J, hipStream_t st);
+// Z operands, then the MFMA quadratic form of the filtered eta2 pair sum
+hipError_t launch_qf_zgen(const QfJob& J, hipStream_t st);
+hipError_t launch_qf_forms(const QfJob& J, hipStream_t st);
 // eta range of the columns of a fill (minus swl) -> eb[2]; then per filtered
 // pair record the levels where its profile exponentials exceed e^{-T}
 template <typename R>
@@ -173,6 +189,19 @@ public:
     int64_t table_records(bool direct) const {
         return direct ? (int64_t)tab.direct.rec.size() : (int64_t)tab.filt.rec.size();
     }
+    // bf16 hi/lo tables of the second-order quadratic forms (built on first
+    // use): form 0 the filtered eta2, then per depth-basis rank q the u, v
+    // (cos-type) and w (sin-type) forms
+    struct QfTables {
+        int npad = 0, nform = 0, rank = 0;
+        double basis_err = 0;
+        std::vector<double> mu;              // skeleton wavenumbers of the depth basis
+        DevBuf b;                            // [nform][4][npad x npad] tiled
+        DevBuf ftype;                        // [nform] int8
+        DevBuf kend;                         // per 64-wide tile of i
+        std::vector<int32_t> kend_h;
+    };
+    const QfTables* qform_tables();
     DevBuf partial;                          // split-K partial sums of point batches
     std::mutex mu;                           // serialises the per-point scratch use
     // scratch for point batches (inputs 4n doubles, outputs 6n doubles)
@@ -183,6 +212,7 @@ public:
 private:
     DevBuf ph_, cr_, act_, frow_, fmhi_, fuvw_, fx_, drow_, duvw_, dx_;
     std::map<std::pair<bool, int>, std::unique_ptr<DevBuf>> seg_cache_;
+    std::unique_ptr<QfTables> qf_;
     SeaDev<double> d64_{};
     SeaDev<float> d32_{};
     template <typename R>
@@ -219,6 +249,10 @@ public:
     double t0 = 0;
     int level_chunk = 16;
     int64_t points_per_fill() const { return 4LL * g.nx * g.ny * g.nl; }
+    // quadratic forms of the last fill (measurement)
+    bool qf_used = false;
+    int qf_nform = 0, qf_rank = 0;
+    double qf_flop = 0, qf_basis_err = 0;
 private:
     int prec;
     std::vector<int32_t> ignore_;
@@ -227,6 +261,11 @@ private:
     int64_t nact_ = 0;
     DevBuf lvl_, eb_, partial_;
     bool prune_ = true;
+    bool qform_ = true;                       // FP32: eta2 pair sum as an MFMA quadratic form
+    DevBuf zbuf_;                             // its Z operands (one chunk of points)
+    DevBuf qout_;                             // form values [nform][points of the fill]
+    int qform_fill(int nslot, const double* slot_t, int64_t ncol, const int32_t* cols, ColumnJob<float>& C,
+                   LevelJob<float>& L);
     // fields: 0 ETA(4) 1 ETAdx(2) 2 ETAdy(2) 3 ETAdt(4) 4..6 U/V/W(4) 7..9 dt(4) 10..18 dx/dy/ds(2)
     DevBuf f_[19];
     DevBuf q_in_, q_out_;

@@ -99,7 +99,13 @@ template <typename R, bool GRID>
 __device__ __forceinline__ void col_store(const SeaDev<R>& S, const ColumnJob<R>& J, const ColPos<R>& P,
                                           const double* v /* e1 e2 gx gy gz gp */) {
     if (GRID) {
-        J.eta_grid[(int64_t)J.slot_dst[P.slot] * J.ncell + P.col] = (R)(v[0] + v[1] + S.swl);
+        double e2 = v[1];
+        if (J.q2) {
+            double q = 0.;
+            for (int k = 0; k < J.nq2; ++k) q += (double)J.q2[(int64_t)k * J.q2stride + P.q];
+            e2 += J.q2scale[P.slot] * q;
+        }
+        J.eta_grid[(int64_t)J.slot_dst[P.slot] * J.ncell + P.col] = (R)(v[0] + e2 + S.swl);
     } else {
         J.o_eta1[P.q] = v[0]; J.o_eta2[P.q] = v[1];
         J.o_gx[P.q] = v[2]; J.o_gy[P.q] = v[3]; J.o_gz[P.q] = v[4]; J.o_gp[P.q] = v[5];
@@ -121,7 +127,7 @@ __global__ __launch_bounds__(256) void k_columns(SeaDev<R> S, ColumnJob<R> J) {
     const int ia = min(sg.rlo, sg.own_lo), ib = max(sg.rhi, sg.own_hi);
     for (int i = ia; i < ib; ++i) {
         const bool own = i >= sg.own_lo && i < sg.own_hi;
-        const bool prs = J.want_eta2 && i >= sg.rlo && i < sg.rhi;
+        const bool prs = J.want_eta2 && J.pairs && i >= sg.rlo && i < sg.rhi;
         if (!own && !prs) continue;
         const PhaseRec pi = S.ph[i];
         const CompRec<R>& ci_ = S.cr[i];
@@ -315,7 +321,7 @@ __device__ __forceinline__ void levels_body(const SeaDev<R>& S, const LevelJob<R
     for (int i = ia; i < ib; ++i) {
         const bool own = i >= sg.own_lo && i < sg.own_hi;
         const bool sec = secf && (direct || S.rowActive[i]);
-        const bool prs = sec && i >= sg.rlo && i < sg.rhi;
+        const bool prs = sec && !(GRID && J.qf) && i >= sg.rlo && i < sg.rhi;
         if (!own && !prs) continue;
         const PhaseRec pi = S.ph[i];
         const CompRec<R> c = S.cr[i];
@@ -441,6 +447,37 @@ __device__ __forceinline__ void levels_body(const SeaDev<R>& S, const LevelJob<R
             W[l] += (double)wr[l];
         }
         if (WP && direct) P2 += (double)pr;
+    }
+    if constexpr (GRID) {
+        if (J.qf) {
+            // pair sums from the quadratic forms of the separable depth basis
+            R su[LC], sv[LC], sw[LC];
+#pragma unroll
+            for (int l = 0; l < LC; ++l) su[l] = sv[l] = sw[l] = 0;
+            const int64_t cq = PP.cq;
+            const int Rk = J.qf_rank;
+            for (int q = 0; q < Rk; ++q) {
+                const R Qu = (R)J.qf[(int64_t)q * J.qf_stride + cq];
+                const R Qv = (R)J.qf[(int64_t)(Rk + q) * J.qf_stride + cq];
+                const R Qw = (R)J.qf[(int64_t)(2 * Rk + q) * J.qf_stride + cq];
+                const R m2 = (R)J.qf_mu2[q], mg = (R)J.qf_mg[q];
+#pragma unroll
+                for (int l = 0; l < LC; ++l) {
+                    R E, G;
+                    prof(m2, mg, zp[l], E, G);
+                    su[l] = fmaR(Qu, E + G, su[l]);
+                    sv[l] = fmaR(Qv, E + G, sv[l]);
+                    sw[l] = fmaR(Qw, E - G, sw[l]);
+                }
+            }
+            const double sc = J.qf_scale[PP.slot];
+#pragma unroll
+            for (int l = 0; l < LC; ++l) {
+                U[l] += sc * (double)su[l];
+                V[l] += sc * (double)sv[l];
+                W[l] += sc * (double)sw[l];
+            }
+        }
     }
     if (!PP.live) return;
     double a[RS];

@@ -79,4 +79,28 @@ struct GridGeom {
     double x0, y0, dx, dy, ds, h, swl;
 };
 
+// Second-order pair sums as quadratic forms on MFMA (qform.hip).  Operands
+// are bf16 hi/lo pairs in a tiled layout: rows in blocks of 32, k in blocks
+// of 16, a 32 x 16 tile stored as 512 contiguous values in two 32 x 8 halves
+// (k 0-7, then 8-15), each row's 8 values contiguous.
+struct QfJob {
+    int64_t p0 = 0, P = 0, ppad = 0;     // chunk of points [p0, p0 + P), padded to 128
+    int n = 0, npad = 0;                  // components, padded to 128
+    int grid = 1;                         // points: slot-major s-grid columns, else arrays
+    int64_t ncol = 0;
+    const int32_t* cols = nullptr;
+    GridGeom g{};
+    double slot_t[4] = {0, 0, 0, 0};
+    const double *x = nullptr, *y = nullptr, *t = nullptr;
+    double x0 = 0, y0 = 0;                // wave reference point
+    const PhaseRec* ph = nullptr;
+    uint16_t *zc_hi = nullptr, *zc_lo = nullptr, *zs_hi = nullptr, *zs_lo = nullptr;   // ppad x npad
+    const uint16_t* b = nullptr;          // [nform][B1 hi, B1 lo, B2 hi, B2 lo][npad x npad]
+    const int8_t* ftype = nullptr;        // per form: 0 cos-type, 1 sin-type
+    int nform = 0;
+    const int32_t* kend = nullptr;        // per 64-wide tile of i: end of its m range (multiple of 16)
+    float* out = nullptr;                 // [nform][ostride]
+    int64_t ostride = 0;
+};
+
 }  // namespace cfdwm

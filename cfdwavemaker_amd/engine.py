@@ -144,6 +144,15 @@ class Grid:
         check(self.L.cfdwm_grid_query(self.h, t.size, _p(t), _p(x), _p(y), _p(z), _p(out)), "cfdwm_grid_query")
         return out
 
+    def qform_info(self):
+        """Quadratic forms of the last fill: None when the pair loop ran."""
+        nf, rk = C.c_int(), C.c_int()
+        fl, er = C.c_double(), C.c_double()
+        used = self.L.cfdwm_grid_qform_info(self.h, C.byref(nf), C.byref(rk), C.byref(fl), C.byref(er))
+        if used != 1:
+            return None
+        return dict(nform=nf.value, rank=rk.value, mfma_flop=fl.value, basis_err=er.value)
+
     def query_device_f32(self, n, t, x_ptr, y_ptr, z_ptr, eta_ptr, u_ptr, v_ptr, w_ptr):
         """Device pointers (e.g. torch tensors' data_ptr()), enqueued on the library stream."""
         check(self.L.cfdwm_grid_query_device_f32(self.h, int(n), float(t), x_ptr, y_ptr, z_ptr,
@@ -170,3 +179,16 @@ def prof_read(reset=True):
         nm = names.raw[64 * q:64 * q + 64].split(b"\0", 1)[0].decode()
         out[nm] = (float(ms[q]), int(cnt[q]))
     return out
+
+
+def depth_basis(kmax, tmax, tol=1e-6, rmax=32):
+    """Skeleton-exponential depth basis (host only): dict(rank, mu, tau, err, ok)."""
+    L = load()
+    mu = np.zeros(rmax)
+    tau = np.zeros(rmax)
+    err = C.c_double()
+    r = L.cfdwm_depth_basis(float(kmax), float(tmax), float(tol), int(rmax), _p(mu), _p(tau), C.byref(err))
+    if r == 0:
+        raise ValueError("cfdwm_depth_basis: invalid arguments")
+    k = abs(r)
+    return dict(rank=k, mu=mu[:k].copy(), tau=tau[:k].copy(), err=err.value, ok=r > 0)

@@ -170,6 +170,20 @@ CFDWM_EXPORT int cfdwm_grid_query(cfdwm_grid* grid, int64_t n, const double* t, 
 CFDWM_EXPORT int cfdwm_grid_query_device_f32(cfdwm_grid* grid, int64_t n, double t, const float* x,
                                              const float* y, const float* z, float* eta, float* u,
                                              float* v, float* w);
+/* Second-order quadratic forms of the last FP32 fill (no reference
+ * counterpart: measurement hook).  Returns 1 when the fill ran the pair sums
+ * as MFMA quadratic forms (then nform, the depth-basis rank, the executed MFMA
+ * flops of that fill and the basis' max error are filled in), 0 when it ran
+ * the pair loop. */
+CFDWM_EXPORT int cfdwm_grid_qform_info(cfdwm_grid* grid, int* nform, int* rank, double* mfma_flop,
+                                       double* basis_err);
+/* Skeleton-exponential depth basis e^{-kappa t} ~ sum_q c_q(kappa) e^{-mu_q t}
+ * over kappa in [0, kmax], t in [0, tmax] (host only, no GPU needed): the
+ * smallest rank <= rmax whose max error is <= tol.  mu and tau (capacity
+ * rmax) receive the skeleton wavenumbers and depths, err the measured max
+ * error.  Returns the rank, or -(rank) when tol was not reached. */
+CFDWM_EXPORT int cfdwm_depth_basis(double kmax, double tmax, double tol, int rmax, double* mu, double* tau,
+                                   double* err);
 CFDWM_EXPORT int cfdwm_sync(void);
 CFDWM_EXPORT int cfdwm_last_error(char* buf, int len);
 /* Kernel timing with HIP events on the library stream. */

@@ -142,3 +142,23 @@ def test_split_k_matches_single_segment(eng, monkeypatch):
         for nseg in ("7", "64"):
             err = scaled_err(out[nseg][k], out["1"][k])
             assert err <= 1e-12, (k, nseg, err)
+
+
+def test_eta2_quadratic_form_matches_pair_loop(eng, oracle, monkeypatch):
+    """FP32 s-grid elevation with the filtered eta2 pair sum on the matrix
+    cores (bf16 hi/lo split, FP32 accumulation) against the FP32 pair loop and
+    the FP64 oracle: stated bound 1e-5 of the elevation scale."""
+    from cfdwavemaker_amd.seastate import config_sea
+    s = config_sea(3).as_dict()
+    dom = (0.0, 1000.0 * 20 / 255, 0.0, 1000.0 * 12 / 255)   # 21 x 13 columns of the 256^2 grid
+    ref = oracle.sea(s).grid(dom, 21, 13, 16, 0.0, 0.5, 0.7, 1.5, 100.0)
+    eta_ref = ref.export("ETA")
+    out = {}
+    for qf in ("1", "0"):
+        monkeypatch.setenv("CFDWM_QFORM", qf)
+        grid = eng.Sea(s, precision=eng.FP32, with_direct=False).grid(dom, 21, 13, 16, 0.0, 0.5, 0.7, 1.5, 100.0)
+        out[qf] = grid.export("ETA")
+        err = scaled_err(out[qf], eta_ref)
+        print(f"qform {qf} ETA err vs oracle {err:.2e}", flush=True)
+        assert err <= 1e-5, (qf, err)
+    assert scaled_err(out["1"], out["0"]) <= 1e-5
