@@ -615,6 +615,7 @@ int Grid::qform_fill(int nslot, const double* slot_t, int64_t ncol, const int32_
     Q.ph = sea->dev<float>().ph;
     uint16_t* z = (uint16_t*)zbuf_.get();
     Q.zc_hi = z; Q.zc_lo = z + zn; Q.zs_hi = z + 2 * zn; Q.zs_lo = z + 3 * zn;
+    Q.zn = (int64_t)zn;
     Q.b = (const uint16_t*)T->b.get();
     Q.ftype = (const int8_t*)T->ftype.get();
     Q.nform = T->nform;

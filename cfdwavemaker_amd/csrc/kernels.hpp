@@ -95,6 +95,7 @@ struct QfJob {
     double x0 = 0, y0 = 0;                // wave reference point
     const PhaseRec* ph = nullptr;
     uint16_t *zc_hi = nullptr, *zc_lo = nullptr, *zs_hi = nullptr, *zs_lo = nullptr;   // ppad x npad
+    int64_t zn = 0;                       // elements between the four Z arrays (zc_lo = zc_hi + zn, ...)
     const uint16_t* b = nullptr;          // [nform][B1 hi, B1 lo, B2 hi, B2 lo][npad x npad]
     const int8_t* ftype = nullptr;        // per form: 0 cos-type, 1 sin-type
     int nform = 0;
