@@ -32,6 +32,9 @@ BATCH_API = [
     "cfdwm_grid_refill", "cfdwm_grid_force_update", "cfdwm_grid_state", "cfdwm_grid_export",
     "cfdwm_grid_query", "cfdwm_grid_query_device_f32", "cfdwm_sync", "cfdwm_last_error",
     "cfdwm_prof_enable", "cfdwm_prof_read", "cfdwm_grid_qform_info", "cfdwm_depth_basis",
+    "cfdwm_mlp_create", "cfdwm_mlp_destroy", "cfdwm_mlp_nparams", "cfdwm_mlp_get_params",
+    "cfdwm_mlp_set_params", "cfdwm_mlp_set_norm", "cfdwm_mlp_get_norm", "cfdwm_mlp_forward_device",
+    "cfdwm_mlp_loss_grad_device", "cfdwm_mlp_train_device",
 ]
 
 
@@ -123,6 +126,27 @@ def load(path: str | None = None) -> C.CDLL:
     L.cfdwm_grid_qform_info.argtypes = [C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_int), _dp, _dp]
     L.cfdwm_depth_basis.restype = C.c_int
     L.cfdwm_depth_basis.argtypes = [_d, _d, _d, C.c_int, _dp, _dp, _dp]
+    _fp = C.POINTER(C.c_float)
+    L.cfdwm_mlp_create.restype = C.c_int
+    L.cfdwm_mlp_create.argtypes = [C.POINTER(C.c_void_p), C.c_int, C.c_int, C.c_int, C.c_int, C.c_uint64]
+    L.cfdwm_mlp_destroy.argtypes = [C.c_void_p]
+    L.cfdwm_mlp_nparams.restype = _i64
+    L.cfdwm_mlp_nparams.argtypes = [C.c_void_p]
+    L.cfdwm_mlp_get_params.restype = C.c_int
+    L.cfdwm_mlp_get_params.argtypes = [C.c_void_p, _fp]
+    L.cfdwm_mlp_set_params.restype = C.c_int
+    L.cfdwm_mlp_set_params.argtypes = [C.c_void_p, _fp]
+    L.cfdwm_mlp_set_norm.restype = C.c_int
+    L.cfdwm_mlp_set_norm.argtypes = [C.c_void_p, _fp, _fp, _fp, _fp]
+    L.cfdwm_mlp_get_norm.restype = C.c_int
+    L.cfdwm_mlp_get_norm.argtypes = [C.c_void_p, _fp]
+    L.cfdwm_mlp_forward_device.restype = C.c_int
+    L.cfdwm_mlp_forward_device.argtypes = [C.c_void_p, _i64, C.c_void_p, C.c_void_p]
+    L.cfdwm_mlp_loss_grad_device.restype = C.c_int
+    L.cfdwm_mlp_loss_grad_device.argtypes = [C.c_void_p, _i64, C.c_void_p, C.c_void_p, _fp, _fp]
+    L.cfdwm_mlp_train_device.restype = C.c_int
+    L.cfdwm_mlp_train_device.argtypes = [C.c_void_p, _i64, C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_float,
+                                         C.c_uint64, C.c_int, _fp]
     if path is None:
         _LIB = L
     return L

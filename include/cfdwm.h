@@ -184,6 +184,34 @@ CFDWM_EXPORT int cfdwm_grid_qform_info(cfdwm_grid* grid, int* nform, int* rank, 
  * error.  Returns the rank, or -(rank) when tol was not reached. */
 CFDWM_EXPORT int cfdwm_depth_basis(double kmax, double tmax, double tol, int rmax, double* mu, double* tau,
                                    double* err);
+/* ---- Learned second-order surrogate: per-column MLP (no reference
+ * counterpart; the exact kernels above stay the fallback and the judge).
+ * L = nhidden + 1 linear layers (nin -> width -> ... -> nout), GELU (erf
+ * form) after all but the last; FP32 master parameters, bf16 products with
+ * FP32 accumulation on the matrix cores; inputs / outputs standardised with
+ * the per-feature mean and std set by cfdwm_mlp_set_norm.  Parameter layout:
+ * per layer W (dout x din, row-major) then b (dout). */
+typedef struct cfdwm_mlp cfdwm_mlp;
+CFDWM_EXPORT int cfdwm_mlp_create(cfdwm_mlp** out, int nin, int width, int nhidden, int nout, uint64_t seed);
+CFDWM_EXPORT void cfdwm_mlp_destroy(cfdwm_mlp* mlp);
+CFDWM_EXPORT int64_t cfdwm_mlp_nparams(cfdwm_mlp* mlp);
+CFDWM_EXPORT int cfdwm_mlp_get_params(cfdwm_mlp* mlp, float* out);
+CFDWM_EXPORT int cfdwm_mlp_set_params(cfdwm_mlp* mlp, const float* in);
+CFDWM_EXPORT int cfdwm_mlp_set_norm(cfdwm_mlp* mlp, const float* in_mean, const float* in_std,
+                                    const float* out_mean, const float* out_std);
+/* out[2 nin + 2 nout] = in_mean, in_std, out_mean, out_std */
+CFDWM_EXPORT int cfdwm_mlp_get_norm(cfdwm_mlp* mlp, float* out);
+/* Y[n][nout] = net(X[n][nin]) (device pointers, raw units) */
+CFDWM_EXPORT int cfdwm_mlp_forward_device(cfdwm_mlp* mlp, int64_t n, const float* X, float* Y);
+/* mean squared error of the standardised outputs over the first n <= 65536
+ * rows and its gradient (grad_host: nparams floats, may be NULL) */
+CFDWM_EXPORT int cfdwm_mlp_loss_grad_device(cfdwm_mlp* mlp, int64_t n, const float* X, const float* Y,
+                                            float* loss, float* grad_host);
+/* Adam on seeded minibatches of the n rows of X / Y (device pointers);
+ * cosine learning-rate decay to lr / 20; loss_curve[steps] (host, may be
+ * NULL) gets the minibatch loss every `report` steps. */
+CFDWM_EXPORT int cfdwm_mlp_train_device(cfdwm_mlp* mlp, int64_t n, const float* X, const float* Y, int steps,
+                                        int batch, float lr, uint64_t seed, int report, float* loss_curve);
 CFDWM_EXPORT int cfdwm_sync(void);
 CFDWM_EXPORT int cfdwm_last_error(char* buf, int len);
 /* Kernel timing with HIP events on the library stream. */
