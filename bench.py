@@ -103,6 +103,59 @@ def rocprof_summary(db_path: str, command: str) -> str:
     return "\n".join(lines) + "\n"
 
 
+def surrogate_leg(steps: int, warmup: int) -> dict:
+    """Learned second-order surrogate on the 512 x 512 column x 16 level
+    workload (N = 1024): features -> network -> assembly on the GPU, the
+    residual check of its elevation against the exact kernels on a sample of
+    columns (gate 1e-3 Hs), and the exact GPU path on the same columns (the
+    s-grid fill, per time level) as the fallback and the comparison."""
+    import torch
+    from cfdwavemaker_amd import engine
+    from cfdwavemaker_amd import surrogate as S
+    from cfdwavemaker_amd.seastate import config_sea
+
+    sea = config_sea(5)
+    d = sea.as_dict()
+    es = engine.Sea(d, precision=engine.FP32, with_direct=False)
+    sur = S.Surrogate.load()
+    n = 512
+    g = np.linspace(DOMAIN[0], DOMAIN[1], n)
+    xx, yy = (q.ravel() for q in np.meshgrid(g, g, indexing="ij"))
+    xd = torch.tensor(xx, dtype=torch.float64, device="cuda")
+    yd = torch.tensor(yy, dtype=torch.float64, device="cuda")
+    t = 0.25
+    for _ in range(max(warmup, 1)):
+        sur.fields(es, sea, xd, yd, t)
+    engine.sync()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        out, _, _ = sur.fields(es, sea, xd, yd, t)
+    engine.sync()
+    t_sur = (time.perf_counter() - t0) / steps
+    pts = n * n * S.NLEV
+    eta = out[:, 0].cpu().numpy().astype(float)
+    sample = np.arange(0, n * n, 256)
+    gate = 1e-3 * HS
+    t1 = time.perf_counter()
+    err, ok = sur.residual_check(es, xx, yy, t, eta, sample, gate)
+    t_chk = time.perf_counter() - t1
+    # exact GPU path on the same columns: s-grid fill (4 time levels per fill)
+    grid = es.grid(DOMAIN, n, n, S.NLEV, T0, DT, TAN_A, TAN_B, DEPTH)
+    engine.sync()
+    t2 = time.perf_counter()
+    for _ in range(steps):
+        grid.refill()
+    engine.sync()
+    t_exact = (time.perf_counter() - t2) / steps / 4.0
+    eff = pts / (t_sur + t_chk + (0.0 if ok else t_exact))
+    return {"workload": "N=1024 short-crested sea, 512x512 columns x 16 levels, one time level",
+            "points": pts, "surrogate_points_per_s": pts / t_sur, "ms_surrogate": t_sur * 1e3,
+            "residual_check": {"columns": int(sample.size), "eta_max_err_m": err, "eta_err_over_hs": err / HS,
+                               "gate": "eta error <= 1e-3 Hs vs exact", "pass": bool(ok), "ms": t_chk * 1e3},
+            "exact_points_per_s": pts / t_exact, "ms_exact_per_time_level": t_exact * 1e3,
+            "effective_points_per_s": eff, "fallback_to_exact": not ok}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -115,6 +168,7 @@ def main():
     ap.add_argument("--cpu-worker", nargs=2, default=None)
     ap.add_argument("--rocprof-summary", nargs=2, metavar=("DB", "COMMAND"), default=None,
                     help="print a markdown summary of a rocprofv3 results database and exit")
+    ap.add_argument("--no-surrogate", action="store_true", help="skip the learned-surrogate leg")
     ap.add_argument("--calib-copy", action="store_true",
                     help="run one plain 256 MiB device copy first (reference for PMC byte counters)")
     a = ap.parse_args()
@@ -222,6 +276,8 @@ def main():
     uref = g64.export("UX")
     u_err = float(np.max(np.abs(gxx.export("UX") - uref)) / np.max(np.abs(uref)))
 
+    sur = None if a.no_surrogate else surrogate_leg(a.steps, a.warmup)
+
     cpu = None
     if not a.no_cpu:
         threads = min(16, os.cpu_count() or 1)
@@ -257,6 +313,7 @@ def main():
         "accuracy": {"eta_max_err_m": eta_err, "eta_err_over_hs": eta_err / HS, "ux_scaled_err": u_err,
                      "gate": "eta error <= 1e-3 Hs vs FP64", "pass": eta_err <= 1e-3 * HS},
         "cpu_baseline": cpu,
+        "surrogate": sur,
     }
     print(json.dumps(line), flush=True)
 

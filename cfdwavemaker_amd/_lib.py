@@ -34,7 +34,7 @@ BATCH_API = [
     "cfdwm_prof_enable", "cfdwm_prof_read", "cfdwm_grid_qform_info", "cfdwm_depth_basis",
     "cfdwm_mlp_create", "cfdwm_mlp_destroy", "cfdwm_mlp_nparams", "cfdwm_mlp_get_params",
     "cfdwm_mlp_set_params", "cfdwm_mlp_set_norm", "cfdwm_mlp_get_norm", "cfdwm_mlp_forward_device",
-    "cfdwm_mlp_loss_grad_device", "cfdwm_mlp_train_device",
+    "cfdwm_mlp_loss_grad_device", "cfdwm_mlp_train_device", "cfdwm_surrogate_linear", "cfdwm_surrogate_add",
 ]
 
 
@@ -147,6 +147,11 @@ def load(path: str | None = None) -> C.CDLL:
     L.cfdwm_mlp_train_device.restype = C.c_int
     L.cfdwm_mlp_train_device.argtypes = [C.c_void_p, _i64, C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_float,
                                          C.c_uint64, C.c_int, _fp]
+    L.cfdwm_surrogate_linear.restype = C.c_int
+    L.cfdwm_surrogate_linear.argtypes = [C.c_void_p, _i64, C.c_void_p, C.c_void_p, _d, _d, C.c_int, _dp, C.c_int,
+                                         _d, C.c_void_p, C.c_void_p]
+    L.cfdwm_surrogate_add.restype = C.c_int
+    L.cfdwm_surrogate_add.argtypes = [_i64, C.c_void_p, C.c_void_p, C.c_void_p]
     if path is None:
         _LIB = L
     return L

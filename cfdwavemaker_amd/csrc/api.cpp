@@ -785,6 +785,25 @@ int cfdwm_depth_basis(double kmax, double tmax, double tol, int rmax, double* mu
     if (err) *err = B.err;
     return rc == 0 ? B.R : -B.R;
 }
+int cfdwm_surrogate_linear(cfdwm_sea* sea, int64_t n, const double* x, const double* y, double t, double h,
+                           int nlev, const double* sigma, int fstride, double kp, float* X, float* Lin) {
+    if (!sea || n < 0 || nlev < 1 || nlev > 32 || fstride < 1 || !sigma || (n && (!x || !y || !X || !Lin))) {
+        set_error("bad arguments");
+        return -1;
+    }
+    SurrJob J;
+    J.n = n; J.x = x; J.y = y; J.t = t; J.h = h; J.kp = kp;
+    J.nlev = nlev; J.fstride = fstride;
+    J.nx = 3 + 3 * ((nlev + fstride - 1) / fstride) + 2;
+    for (int l = 0; l < nlev; ++l) J.sigma[l] = sigma[l];
+    J.X = X; J.Lin = Lin;
+    return surrogate_linear(*sea->s, J);
+}
+int cfdwm_surrogate_add(int64_t n, const float* a, const float* b, float* out) {
+    if (n < 0 || (n && (!a || !b || !out))) { set_error("bad arguments"); return -1; }
+    if (int e = Engine::get().init()) return e;
+    return surrogate_add(n, a, b, out);
+}
 int cfdwm_sync(void) {
     if (Engine::get().init()) return -2;
     CFDWM_HIP(hipStreamSynchronize(Engine::get().stream()));

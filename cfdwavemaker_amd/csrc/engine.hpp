@@ -110,6 +110,17 @@ hipError_t launch_prune(const R* eta, int64_t ncell, const int32_t* cols, int64_
                         const int* slot_dst, double swl, double* eb, const UvwRec<R>* rec, int64_t nrec,
                         const double* sigma, int nl, double h, uint32_t* lvl, hipStream_t st);
 
+// Linear-field features of the learned surrogate (surrogate.hip)
+struct SurrJob {
+    int64_t n = 0;
+    const double *x = nullptr, *y = nullptr;   // device column coordinates
+    double t = 0, h = 0, kp = 0;
+    int nlev = 0, fstride = 2, nx = 0;         // nx = 3 + 3 ceil(nlev / fstride) + 2
+    double sigma[32] = {};
+    float* X = nullptr;                        // [n][nx]
+    float* Lin = nullptr;                      // [n][1 + 4 nlev]
+};
+
 // ---------------------------------------------------------------- errors
 struct Error {
     int code;
@@ -176,6 +187,10 @@ private:
 enum Precision { kFP64 = 0, kFP32 = 1 };
 
 // A sea state resident on the device.
+class Sea;
+int surrogate_linear(Sea& s, const SurrJob& J);
+int surrogate_add(int64_t n, const float* a, const float* b, float* o);
+
 class Sea {
 public:
     SeaTables tab;

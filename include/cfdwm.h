@@ -212,6 +212,16 @@ CFDWM_EXPORT int cfdwm_mlp_loss_grad_device(cfdwm_mlp* mlp, int64_t n, const flo
  * NULL) gets the minibatch loss every `report` steps. */
 CFDWM_EXPORT int cfdwm_mlp_train_device(cfdwm_mlp* mlp, int64_t n, const float* X, const float* Y, int steps,
                                         int batch, float lr, uint64_t seed, int report, float* loss_curve);
+/* Linear-field features of the surrogate for n columns (x, y: device
+ * doubles) at time t: X[n][3 + 3 ceil(nlev / fstride) + 2] = eta1, eta1_x,
+ * eta1_y, (u1, v1, w1) at every fstride-th level, h, kp; Lin[n][1 + 4 nlev] =
+ * eta1, (u1, v1, w1, p1) at the levels z_l = eta1 + sigma_l (h + eta1)
+ * (nlev <= 32).  Enqueued on the library stream. */
+CFDWM_EXPORT int cfdwm_surrogate_linear(cfdwm_sea* sea, int64_t n, const double* x, const double* y, double t,
+                                        double h, int nlev, const double* sigma, int fstride, double kp,
+                                        float* X, float* Lin);
+/* out[i] = a[i] + b[i] over n device floats (assembly of the surrogate fields). */
+CFDWM_EXPORT int cfdwm_surrogate_add(int64_t n, const float* a, const float* b, float* out);
 CFDWM_EXPORT int cfdwm_sync(void);
 CFDWM_EXPORT int cfdwm_last_error(char* buf, int len);
 /* Kernel timing with HIP events on the library stream. */
