@@ -774,6 +774,23 @@ int cfdwm_grid_qform_info(cfdwm_grid* grid, int* nform, int* rank, double* mfma_
     if (basis_err) *basis_err = g.qf_basis_err;
     return g.qf_used ? 1 : 0;
 }
+int cfdwm_grids_refill_batched(cfdwm_grid** grids, int n, double budget_gb, int release, cfdwm_batch_stats* out) {
+    if (!grids || n < 0) { set_error("bad arguments"); return -1; }
+    std::vector<Grid*> gs(n);
+    for (int k = 0; k < n; ++k) {
+        if (!grids[k]) { set_error("bad arguments"); return -1; }
+        gs[k] = grids[k]->g.get();
+    }
+    BatchStats S;
+    const int64_t budget = (int64_t)(std::max(1.0, budget_gb) * 1073741824.0);
+    if (int rc = refill_batched(gs.data(), n, budget, release != 0, &S)) return rc;
+    if (out) {
+        out->groups = S.groups; out->batched = S.batched; out->single = S.single; out->tiles = S.tiles;
+        out->blocks = S.blocks; out->order = S.order; out->tuned = S.tuned;
+        out->forms_ms = S.forms_ms; out->mfma_flop = S.mfma_flop;
+    }
+    return 0;
+}
 int cfdwm_depth_basis(double kmax, double tmax, double tol, int rmax, double* mu, double* tau, double* err) {
     if (rmax < 1 || rmax > 64 || !(kmax > 0) || !(tmax > 0)) return 0;
     DepthBasis B;

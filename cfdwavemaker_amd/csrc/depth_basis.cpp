@@ -3,6 +3,9 @@
 
 #include <algorithm>
 #include <cmath>
+#include <map>
+#include <mutex>
+#include <tuple>
 
 namespace cfdwm {
 
@@ -95,6 +98,37 @@ double DepthBasis::eval(double kappa, double t) const {
     double v = 0;
     for (int q = 0; q < R; ++q) v += c[q] * std::exp(-mu[q] * t);
     return v;
+}
+
+// e^{-kappa t} depends on kappa t only: a basis for kappa in [0, 1], t in [0, Pi]
+// serves every (kmax, tmax) with kmax tmax <= Pi after scaling mu by kmax and
+// tau by 1 / kmax.  Bases are cached for Pi on a geometric grid (ratio 1.1),
+// so the sea states of a batch share a handful of them.
+int build_depth_basis_cached(double kmax, double tmax, double tol, int rmax, DepthBasis& B) {
+    kmax = std::max(kmax, 1e-9);
+    const double prod = std::max(kmax * tmax, 1e-6);
+    const int j = (int)std::ceil(std::log(prod / 0.1) / std::log(1.1));
+    const double Pi = 0.1 * std::pow(1.1, j);
+    static std::mutex mu;
+    static std::map<std::tuple<int, double, int>, std::pair<int, DepthBasis>> cache;
+    const auto key = std::make_tuple(j, tol, rmax);
+    std::pair<int, DepthBasis> e;
+    {
+        std::lock_guard<std::mutex> lk(mu);
+        auto it = cache.find(key);
+        if (it != cache.end()) e = it->second;
+    }
+    if (e.second.R == 0) {
+        e.first = build_depth_basis(1.0, Pi, tol, rmax, e.second);
+        std::lock_guard<std::mutex> lk(mu);
+        cache[key] = e;
+    }
+    B = e.second;
+    B.kmax = kmax;
+    B.tmax = Pi / kmax;
+    for (double& m : B.mu) m *= kmax;
+    for (double& t : B.tau) t /= kmax;
+    return e.first;
 }
 
 int build_depth_basis(double kmax, double tmax, double tol, int rmax, DepthBasis& B) {

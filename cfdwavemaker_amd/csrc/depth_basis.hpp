@@ -31,5 +31,8 @@ struct DepthBasis {
 
 // 0 on success, -1 when no rank <= rmax reaches tol (B then holds rank rmax).
 int build_depth_basis(double kmax, double tmax, double tol, int rmax, DepthBasis& B);
+// The same through a process-wide cache keyed by kmax * tmax (rounded up on a
+// ratio-1.1 grid; the basis is scale invariant), thread-safe.
+int build_depth_basis_cached(double kmax, double tmax, double tol, int rmax, DepthBasis& B);
 
 }  // namespace cfdwm

@@ -6,6 +6,9 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <cstdio>
+#include <mutex>
+#include <map>
 #include <thread>
 
 namespace cfdwm {
@@ -246,19 +249,6 @@ const Segment* Sea::segments(bool direct, int nseg) {
 }
 
 namespace {
-// bf16 bits of a float, round to nearest even
-uint16_t bf16_bits(float f) {
-    uint32_t u;
-    std::memcpy(&u, &f, 4);
-    u += 0x7FFFu + ((u >> 16) & 1u);
-    return (uint16_t)(u >> 16);
-}
-float bf16_val(uint16_t b) {
-    const uint32_t u = (uint32_t)b << 16;
-    float f;
-    std::memcpy(&f, &u, 4);
-    return f;
-}
 // Irregular::freq_ramp (Irregular.cpp:200-204), host side
 double freq_ramp_host(double t, double rt, double ts, double du) {
     const double a = 1. - ((t - ts - du + rt) / (rt + 1e-12));
@@ -277,6 +267,7 @@ double freq_ramp_host(double t, double rt, double ts, double du) {
 // B2 = cwP cP - cwM cM.
 const Sea::QfTables* Sea::qform_tables() {
     if (qf_) return qf_.get();
+    if (prec != kFP32) return nullptr;
     const PairTable& P = tab.filt;
     const int N = tab.comp.n;
     const double h = tab.in.depth;
@@ -285,12 +276,12 @@ const Sea::QfTables* Sea::qform_tables() {
     double tol = 1e-6;
     if (const char* e = std::getenv("CFDWM_QF_TOL")) tol = std::atof(e);
     DepthBasis DB;
-    build_depth_basis(kmx * (1.0 + 1e-9) + 1e-12, 2.0 * h, tol, 32, DB);
+    build_depth_basis_cached(kmx * (1.0 + 1e-9) + 1e-12, 2.0 * h, tol, 32, DB);
     const int R = DB.R;
+    if (R < 1 || R > 32) return nullptr;
     const int nform = 1 + 3 * R;
-    const int npad = (N + 127) / 128 * 128, nkb = npad / 16;
+    const int npad = (N + 127) / 128 * 128;
     const size_t nn = (size_t)npad * npad;
-    std::vector<uint16_t> b((size_t)nform * 4 * nn, 0);
     std::vector<int8_t> ftype(nform, 0);
     for (int q = 0; q < R; ++q) ftype[1 + 2 * R + q] = 1;
     std::vector<int32_t> kend(npad / 64, 0);
@@ -298,62 +289,30 @@ const Sea::QfTables* Sea::qform_tables() {
         int& ke = kend[i / 64];
         ke = std::max(ke, std::min(npad, (P.mhi[i] + 15) / 16 * 16));
     }
-    auto put = [&](int f, int k, size_t o, double v) {
-        uint16_t* d = b.data() + ((size_t)f * 4 + 2 * k) * nn;
-        const float x = (float)v;
-        const uint16_t hi = bf16_bits(x);
-        d[o] = hi;
-        d[o + nn] = bf16_bits(x - bf16_val(hi));
-    };
-    auto rows = [&](int i0, int i1) {
-        std::vector<double> cp(R), cm(R);
-        for (int i = i0; i < i1; ++i) {
-            const int64_t base = P.rowoff[i] - (i + 1);
-            for (int m = i + 1; m < P.mhi[i]; ++m) {
-                const PairRec64& r = P.rec[base + m];
-                const size_t o = ((size_t)(i >> 5) * nkb + (m >> 4)) * 512 + ((m >> 3) & 1) * 256 + (i & 31) * 8 + (m & 7);
-                put(0, 0, o, r.xP + r.xM);
-                put(0, 1, o, r.xM - r.xP);
-                DB.coeffs(r.kP, cp.data());
-                DB.coeffs(r.kM, cm.data());
-                for (int q = 0; q < R; ++q) {
-                    put(1 + q, 0, o, r.cuP * cp[q] + r.cuM * cm[q]);
-                    put(1 + q, 1, o, r.cuM * cm[q] - r.cuP * cp[q]);
-                    put(1 + R + q, 0, o, r.cvP * cp[q] + r.cvM * cm[q]);
-                    put(1 + R + q, 1, o, r.cvM * cm[q] - r.cvP * cp[q]);
-                    put(1 + 2 * R + q, 0, o, r.cwP * cp[q] + r.cwM * cm[q]);
-                    put(1 + 2 * R + q, 1, o, r.cwP * cp[q] - r.cwM * cm[q]);
-                }
-            }
-        }
-    };
-    {
-        // rows are disjoint tile rows of every table: split them over threads
-        const int nt = (int)std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
-        std::vector<std::thread> th;
-        std::vector<int64_t> cum(N + 1, 0);
-        for (int i = 0; i < N; ++i) cum[i + 1] = cum[i] + std::max(0, P.mhi[i] - i - 1);
-        int i0 = 0;
-        for (int k = 0; k < nt; ++k) {
-            const int64_t goal = cum[N] * (k + 1) / nt;
-            int i1 = i0;
-            while (i1 < N && (cum[i1] < goal || k == nt - 1)) ++i1;
-            th.emplace_back(rows, i0, i1);
-            i0 = i1;
-        }
-        for (auto& t : th) t.join();
-    }
     auto T = std::make_unique<QfTables>();
     T->npad = npad; T->nform = nform; T->rank = R; T->mu = DB.mu; T->basis_err = DB.err;
     T->kend_h = kend;
     hipStream_t st = Engine::get().stream();
-    if (T->b.upload(b.data(), b.size() * sizeof(uint16_t), st)) return nullptr;
-    if (T->ftype.upload(ftype.data(), ftype.size(), st)) return nullptr;
-    if (T->kend.upload(kend.data(), kend.size() * sizeof(int32_t), st)) return nullptr;
+    DevBuf tau, einv;
+    const size_t bytes = (size_t)nform * 4 * nn * sizeof(uint16_t);
+    if (T->b.alloc(bytes) || tau.upload(DB.tau.data(), R * sizeof(double), st) ||
+        einv.upload(DB.einv.data(), (size_t)R * R * sizeof(double), st) ||
+        T->ftype.upload(ftype.data(), ftype.size(), st) ||
+        T->kend.upload(kend.data(), kend.size() * sizeof(int32_t), st))
+        return nullptr;
+    if (hipMemsetAsync(T->b.get(), 0, bytes, st) != hipSuccess) return nullptr;
+    QfBuild Bq;
+    Bq.n = N; Bq.npad = npad; Bq.R = R; Bq.nform = nform;
+    Bq.frow = d32_.frow; Bq.fmhi = d32_.fmhi; Bq.uvw = d32_.fuvw; Bq.x = d32_.fx;
+    Bq.tau = (const double*)tau.get(); Bq.einv = (const double*)einv.get();
+    Bq.b = (uint16_t*)T->b.get();
+    if (N > 0 && launch_qf_build(Bq, st) != hipSuccess) return nullptr;
     if (hipStreamSynchronize(st) != hipSuccess) return nullptr;
     qf_ = std::move(T);
     return qf_.get();
 }
+
+void Sea::release_qform() { qf_.reset(); }
 
 namespace {
 // Segments so that about 256 CUs x 4 SIMDs x 4 waves of threads are busy,
@@ -578,36 +537,40 @@ int Grid::fill_slots(int nslot, const double* slot_t, const int* slot_dst, bool 
 
 // Second-order pair sums of an FP32 fill as MFMA quadratic forms: the
 // filtered eta2 for the column pass and, through the separable depth basis,
-// the u/v/w pair sums for the level pass.  Returns 1 when used, 0 when not
-// applicable (switched off, no pairs, or frequency ramps that differ between
-// components at one of the slot times), < 0 on error.
-int Grid::qform_fill(int nslot, const double* slot_t, int64_t ncol, const int32_t* cols, ColumnJob<float>& C,
-                     LevelJob<float>& L) {
+// the u/v/w pair sums for the level pass.  qform_prepare checks
+// applicability (switched on, pairs present, frequency ramps equal for all
+// components at each slot time, levels inside the basis range), builds the
+// sea's tables once and sizes the workspace; returns 1 when the forms apply,
+// 0 when not, < 0 on error.
+int Grid::qform_prepare(int nslot, const double* slot_t, int64_t ncol, const int32_t* cols) {
     qf_used = false;
-    if (!qform_ || sea->nfilt() == 0) return 0;
+    if (!qform_ || sea->prec != kFP32 || sea->nfilt() == 0) return 0;
     const CompTable& c = sea->tab.comp;
-    double scale[4] = {1, 1, 1, 1};
+    for (int s = 0; s < 4; ++s) qf_scale_[s] = 1.0;
     for (int s = 0; s < nslot; ++s) {
         const double r0 = freq_ramp_host(slot_t[s], c.rt[0], c.ts[0], c.du[0]);
         for (int i = 1; i < c.n; ++i)
             if (freq_ramp_host(slot_t[s], c.rt[i], c.ts[i], c.du[i]) != r0) return 0;
-        scale[s] = r0;
+        qf_scale_[s] = r0;
     }
+    const double hs = sea->tab.in.depth;
+    // the depth basis covers t = -z in [0, 2 hs] and t = 2 hs + z: the grid's levels must lie above -2 hs
+    if (g.h > 2.0 * hs) return 0;
     const Sea::QfTables* T = sea->qform_tables();
     if (!T) return -3;
     const int npad = T->npad, R = T->rank;
-    const double hs = sea->tab.in.depth;
-    // the depth basis covers t = -z in [0, 2 hs] and t = 2 hs + z: the grid's levels must lie above -2 hs
-    if (R > 32 || g.h > 2.0 * hs) return 0;
+    if (R > 32) return 0;
     const int64_t P = (int64_t)nslot * ncol;
     const int64_t ppad_all = (P + 127) / 128 * 128;
     // Z operands of one chunk of points: 8 bytes per (point, component), at most 2 GiB
     const int64_t cap = std::max<int64_t>(128, ((int64_t)1 << 31) / ((int64_t)npad * 8) / 128 * 128);
-    const int64_t chunk = std::min(ppad_all, cap);
-    const size_t zn = (size_t)chunk * npad;
+    qf_chunk_ = std::min(ppad_all, cap);
+    qf_P_ = P;
+    const size_t zn = (size_t)qf_chunk_ * npad;
     if (zbuf_.alloc(zn * 4 * sizeof(uint16_t))) return -3;
     if (qout_.alloc((size_t)T->nform * ppad_all * sizeof(float))) return -3;
-    QfJob Q;
+    QfJob& Q = qj_;
+    Q = QfJob{};
     Q.n = c.n; Q.npad = npad;
     Q.grid = 1; Q.ncol = ncol; Q.cols = cols; Q.g = g;
     for (int s = 0; s < nslot; ++s) Q.slot_t[s] = slot_t[s];
@@ -622,40 +585,272 @@ int Grid::qform_fill(int nslot, const double* slot_t, int64_t ncol, const int32_
     Q.kend = (const int32_t*)T->kend.get();
     Q.out = (float*)qout_.get();
     Q.ostride = ppad_all;
-    hipStream_t st = Engine::get().stream();
+    Q.p0 = 0;
+    Q.P = std::min(qf_chunk_, P);
+    Q.ppad = (Q.P + 127) / 128 * 128;
     // executed MFMA flops per padded point and form: per 64-wide i tile,
     // 64 x (kend - i0) products, two operand pairs (c and s), three bf16 products
-    double fpp = 0;
+    qf_fpp_ = 0;
     for (size_t t = 0; t < T->kend_h.size(); ++t)
-        fpp += 64.0 * std::max(0, T->kend_h[t] - (int)(64 * t)) * 2 * 3 * 2;
-    qf_flop = 0;
-    for (int64_t p0 = 0; p0 < P; p0 += chunk) {
-        Q.p0 = p0;
-        Q.P = std::min(chunk, P - p0);
-        Q.ppad = (Q.P + 127) / 128 * 128;
-        PROF("k_qf_zgen", launch_qf_zgen(Q, st));
-        PROF("k_qf_forms", launch_qf_forms(Q, st));
-        qf_flop += fpp * (double)Q.ppad * T->nform;
-    }
-    qf_used = true;
+        qf_fpp_ += 64.0 * std::max(0, T->kend_h[t] - (int)(64 * t)) * 2 * 3 * 2;
     qf_nform = T->nform;
     qf_rank = R;
     qf_basis_err = T->basis_err;
+    qf_flop = 0;
+    return 1;
+}
+
+// Z operands and forms of every chunk of the prepared fill (one grid).
+int Grid::qform_run() {
+    hipStream_t st = Engine::get().stream();
+    QfJob Q = qj_;
+    for (int64_t p0 = 0; p0 < qf_P_; p0 += qf_chunk_) {
+        Q.p0 = p0;
+        Q.P = std::min(qf_chunk_, qf_P_ - p0);
+        Q.ppad = (Q.P + 127) / 128 * 128;
+        PROF("k_qf_zgen", launch_qf_zgen(Q, st));
+        PROF("k_qf_forms", launch_qf_forms(Q, st));
+        qf_flop += qf_fpp_ * (double)Q.ppad * qj_.nform;
+    }
+    return 0;
+}
+
+// Point the column and level passes of the fill at the form values.
+void Grid::qform_attach(ColumnJob<float>& C, LevelJob<float>& L) {
+    const Sea::QfTables* T = sea->qform_tables();
+    const int R = qf_rank;
+    const double hs = sea->tab.in.depth;
+    qf_used = true;
     C.pairs = 0;
     C.q2 = (const float*)qout_.get();
     C.nq2 = 1;
-    C.q2stride = ppad_all;
-    for (int s = 0; s < 4; ++s) C.q2scale[s] = scale[s];
-    L.qf = (const float*)qout_.get() + ppad_all;
+    C.q2stride = qj_.ostride;
+    for (int s = 0; s < 4; ++s) C.q2scale[s] = qf_scale_[s];
+    L.qf = (const float*)qout_.get() + qj_.ostride;
     L.qf_rank = R;
-    L.qf_stride = ppad_all;
+    L.qf_stride = qj_.ostride;
     constexpr double kL2e = 1.4426950408889634074;
     for (int q = 0; q < R; ++q) {
         L.qf_mu2[q] = (float)(T->mu[q] * kL2e);
         L.qf_mg[q] = (float)(-2.0 * hs * T->mu[q] * kL2e);
     }
-    for (int s = 0; s < 4; ++s) L.qf_scale[s] = scale[s];
+    for (int s = 0; s < 4; ++s) L.qf_scale[s] = qf_scale_[s];
+}
+
+int Grid::qform_fill(int nslot, const double* slot_t, int64_t ncol, const int32_t* cols, ColumnJob<float>& C,
+                     LevelJob<float>& L) {
+    if (batch_pending_) {                     // forms already evaluated by a batched launch
+        batch_pending_ = false;
+        qform_attach(C, L);
+        return 1;
+    }
+    const int rc = qform_prepare(nslot, slot_t, ncol, cols);
+    if (rc != 1) return rc;
+    if (int e = qform_run()) return e;
+    qform_attach(C, L);
     return 1;
+}
+
+// initialize_kinematics slot times (lsgrid_spline.cpp:678, :767)
+void Grid::init_slot_times(double* tt) const {
+    const double t0i = d.t0;
+    const bool ign = d.ignore_at_init != 0;
+    tt[0] = ign ? t0i : t0i - d.dt;
+    tt[1] = t0i;
+    tt[2] = t0i + d.dt;
+    tt[3] = t0i + 2 * d.dt;
+}
+
+// Batched refill, part 1: prepare the forms of a full refill and launch its Z
+// operands.  Returns 1 when this grid joins the batched forms launch (one
+// chunk), 0 when it must be refilled on its own.
+int Grid::batch_begin() {
+    batch_pending_ = false;
+    if (prec != kFP32) return 0;
+    double tt[4];
+    init_slot_times(tt);
+    const bool ign = d.ignore_at_init != 0;
+    const int64_t ncol = ign ? nact_ : (int64_t)g.nx * g.ny;
+    const int32_t* cols = ign ? (const int32_t*)cols_act_d_.get() : nullptr;
+    const int rc = qform_prepare(4, tt, ncol, cols);
+    if (rc != 1) return rc;
+    if (qf_chunk_ < qf_P_) return 0;          // needs chunking: not batched
+    PROF("k_qf_zgen", launch_qf_zgen(qj_, Engine::get().stream()));
+    qf_flop = qf_fpp_ * (double)qj_.ppad * qj_.nform;
+    batch_pending_ = true;
+    return 1;
+}
+
+// Batched refill, part 2 (after the batched forms launch): the rest of the refill.
+int Grid::batch_end() {
+    if (!batch_pending_) return refill();
+    tstep = 0;
+    t0 = d.t0;
+    double tt[4];
+    init_slot_times(tt);
+    const int dst[4] = {0, 1, 2, 3};
+    int rc = fill_slots<float>(4, tt, dst, d.ignore_at_init != 0);
+    batch_pending_ = false;
+    if (rc) return rc;
+    if ((rc = gradients<float>())) return rc;
+    return pack<float>();
+}
+
+void Grid::release_workspace() {
+    zbuf_.release();
+}
+
+namespace {
+// Tile lists of a batched launch.  order 0: all tiles by decreasing cost
+// (point-tile cost ~ the job's i-range work), so round-robin blocks get equal
+// shares; order 1: job by job (decreasing cost), inside a job groups of 8
+// forms x 32 point tiles (cache locality of the job's tables and operands).
+std::vector<QfTile> batch_tiles(const std::vector<QfJob>& jobs, const std::vector<double>& cost, int order) {
+    std::vector<int> idx(jobs.size());
+    for (size_t j = 0; j < jobs.size(); ++j) idx[j] = (int)j;
+    std::stable_sort(idx.begin(), idx.end(), [&](int a, int b) { return cost[a] > cost[b]; });
+    std::vector<QfTile> t;
+    if (order == 0) {
+        for (int j : idx) {
+            const int npt = (int)(jobs[j].ppad / 128);
+            for (int f = 0; f < jobs[j].nform; ++f)
+                for (int p = 0; p < npt; ++p) t.push_back(QfTile{j, p, f, 0});
+        }
+        std::stable_sort(t.begin(), t.end(), [&](const QfTile& a, const QfTile& b) { return cost[a.job] > cost[b.job]; });
+    } else {
+        for (int j : idx) {
+            const int npt = (int)(jobs[j].ppad / 128), nf = jobs[j].nform;
+            for (int f0 = 0; f0 < nf; f0 += 8)
+                for (int p0 = 0; p0 < npt; p0 += 32)
+                    for (int f = f0; f < std::min(nf, f0 + 8); ++f)
+                        for (int p = p0; p < std::min(npt, p0 + 32); ++p) t.push_back(QfTile{j, p, f, 0});
+        }
+    }
+    return t;
+}
+
+struct TuneKey {
+    int jobs_log2, tiles_log2;
+    bool operator<(const TuneKey& o) const {
+        return jobs_log2 != o.jobs_log2 ? jobs_log2 < o.jobs_log2 : tiles_log2 < o.tiles_log2;
+    }
+};
+std::map<TuneKey, std::pair<int, int>>& tune_cache() {
+    static std::map<TuneKey, std::pair<int, int>> m;
+    return m;
+}
+std::mutex& tune_mutex() {
+    static std::mutex m;
+    return m;
+}
+int ilog2(int64_t v) {
+    int r = 0;
+    while (v > 1) { v >>= 1; ++r; }
+    return r;
+}
+}  // namespace
+
+int refill_batched(Grid** grids, int n, int64_t budget_bytes, bool release, BatchStats* stats) {
+    BatchStats S;
+    hipStream_t st = Engine::get().stream();
+    int i = 0;
+    DevBuf djobs, dtiles;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    CFDWM_HIP(hipEventCreate(&e0));
+    CFDWM_HIP(hipEventCreate(&e1));
+    auto cleanup = [&]() { (void)hipEventDestroy(e0); (void)hipEventDestroy(e1); };
+    while (i < n) {
+        // group of grids whose tables and operands fit the budget (estimate:
+        // 61 forms of bf16 hi/lo tables + Z operands + form values)
+        std::vector<Grid*> grp;
+        int64_t bytes = 0;
+        while (i < n) {
+            Grid* g = grids[i];
+            const int64_t npad = (g->sea->tab.comp.n + 127) / 128 * 128;
+            const int64_t pts = 4LL * g->g.nx * g->g.ny;
+            const int64_t est = 61 * 8 * npad * npad + pts * npad * 8 + 61 * pts * 4;
+            if (!grp.empty() && bytes + est > budget_bytes) break;
+            grp.push_back(g);
+            bytes += est;
+            ++i;
+        }
+        ++S.groups;
+        std::vector<QfJob> jobs;
+        std::vector<double> cost;
+        std::vector<char> joined(grp.size(), 0);
+        for (size_t k = 0; k < grp.size(); ++k) {
+            const int rc = grp[k]->batch_begin();
+            if (rc < 0) { cleanup(); return rc; }
+            if (rc == 1) {
+                joined[k] = 1;
+                jobs.push_back(grp[k]->batch_job());
+                double c = 0;
+                const auto& T = *grp[k]->sea->qform_tables();
+                for (size_t t = 0; t < T.kend_h.size(); ++t) c += std::max(0, T.kend_h[t] - (int)(64 * t));
+                cost.push_back(c);
+                S.mfma_flop += grp[k]->qf_flop;
+            }
+        }
+        if (!jobs.empty()) {
+            if (djobs.upload(jobs.data(), jobs.size() * sizeof(QfJob), st)) { cleanup(); return -3; }
+            int64_t ntile = 0;
+            for (const QfJob& J : jobs) ntile += (J.ppad / 128) * J.nform;
+            const TuneKey key{ilog2((int64_t)jobs.size()), ilog2(ntile)};
+            std::pair<int, int> choice{-1, -1};
+            {
+                std::lock_guard<std::mutex> lk(tune_mutex());
+                auto it = tune_cache().find(key);
+                if (it != tune_cache().end()) choice = it->second;
+            }
+            if (const char* e = std::getenv("CFDWM_BATCH_SHAPE")) {
+                int o = 0, b = 256;
+                if (std::sscanf(e, "%d,%d", &o, &b) == 2) choice = {o, b};
+            }
+            auto run = [&](int order, int blocks, float* ms) -> int {
+                const std::vector<QfTile> tl = batch_tiles(jobs, cost, order);
+                if (dtiles.upload(tl.data(), tl.size() * sizeof(QfTile), st)) return -3;
+                CFDWM_HIP(hipEventRecord(e0, st));
+                CFDWM_HIP(launch_qf_persist((const QfJob*)djobs.get(), (const QfTile*)dtiles.get(), (int)tl.size(),
+                                            blocks, st));
+                CFDWM_HIP(hipEventRecord(e1, st));
+                CFDWM_HIP(hipEventSynchronize(e1));
+                CFDWM_HIP(hipEventElapsedTime(ms, e0, e1));
+                S.tiles += (int)tl.size();
+                return 0;
+            };
+            float ms = 0;
+            if (choice.first < 0) {
+                // autotune: every candidate computes the same values; keep the fastest
+                const int cand[4][2] = {{0, 256}, {1, 256}, {0, 1024}, {1, 1024}};
+                float best = 1e30f;
+                for (auto& c : cand) {
+                    float t = 0;
+                    if (int rc = run(c[0], c[1], &t)) { cleanup(); return rc; }
+                    if (t < best) { best = t; choice = {c[0], c[1]}; }
+                }
+                ms = best;
+                S.tuned = 1;
+                std::lock_guard<std::mutex> lk(tune_mutex());
+                tune_cache()[key] = choice;
+            } else {
+                if (int rc = run(choice.first, choice.second, &ms)) { cleanup(); return rc; }
+            }
+            S.forms_ms += ms;
+            S.order = choice.first;
+            S.blocks = choice.second;
+        }
+        for (size_t k = 0; k < grp.size(); ++k) {
+            const int rc = joined[k] ? grp[k]->batch_end() : grp[k]->refill();
+            if (rc) { cleanup(); return rc; }
+            if (joined[k]) ++S.batched; else ++S.single;
+        }
+        if (hipStreamSynchronize(st) != hipSuccess) { cleanup(); return -3; }
+        if (release)
+            for (Grid* g : grp) { g->sea->release_qform(); g->release_workspace(); }
+    }
+    cleanup();
+    if (stats) *stats = S;
+    return 0;
 }
 
 template <typename R>

@@ -103,6 +103,8 @@ hipError_t launch_pack(const PackJob<R>& J, hipStream_t st);
 // Z operands, then the MFMA quadratic form of the filtered eta2 pair sum
 hipError_t launch_qf_zgen(const QfJob& J, hipStream_t st);
 hipError_t launch_qf_forms(const QfJob& J, hipStream_t st);
+hipError_t launch_qf_build(const QfBuild& B, hipStream_t st);
+hipError_t launch_qf_persist(const QfJob* jobs, const QfTile* tiles, int ntiles, int blocks, hipStream_t st);
 // eta range of the columns of a fill (minus swl) -> eb[2]; then per filtered
 // pair record the levels where its profile exponentials exceed e^{-T}
 template <typename R>
@@ -216,7 +218,8 @@ public:
         DevBuf kend;                         // per 64-wide tile of i
         std::vector<int32_t> kend_h;
     };
-    const QfTables* qform_tables();
+    const QfTables* qform_tables();          // FP32 seas only
+    void release_qform();
     DevBuf partial;                          // split-K partial sums of point batches
     std::mutex mu;                           // serialises the per-point scratch use
     // scratch for point batches (inputs 4n doubles, outputs 6n doubles)
@@ -279,8 +282,23 @@ private:
     bool qform_ = true;                       // FP32: eta2 pair sum as an MFMA quadratic form
     DevBuf zbuf_;                             // its Z operands (one chunk of points)
     DevBuf qout_;                             // form values [nform][points of the fill]
+    QfJob qj_{};                              // prepared forms launch (first chunk)
+    int64_t qf_chunk_ = 0, qf_P_ = 0;
+    double qf_fpp_ = 0, qf_scale_[4] = {1, 1, 1, 1};
+    bool batch_pending_ = false;
+    int qform_prepare(int nslot, const double* slot_t, int64_t ncol, const int32_t* cols);
+    int qform_run();
+    void qform_attach(ColumnJob<float>& C, LevelJob<float>& L);
     int qform_fill(int nslot, const double* slot_t, int64_t ncol, const int32_t* cols, ColumnJob<float>& C,
                    LevelJob<float>& L);
+    void init_slot_times(double* tt) const;
+public:
+    // batched refill over many grids (refill_batched)
+    int batch_begin();
+    int batch_end();
+    const QfJob& batch_job() const { return qj_; }
+    void release_workspace();
+private:
     // fields: 0 ETA(4) 1 ETAdx(2) 2 ETAdy(2) 3 ETAdt(4) 4..6 U/V/W(4) 7..9 dt(4) 10..18 dx/dy/ds(2)
     DevBuf f_[19];
     DevBuf q_in_, q_out_;
@@ -296,5 +314,15 @@ private:
                   TQ* eta, TQ* u, TQ* v, TQ* w, const double* ramp);
     int64_t field_count(int which) const;
 };
+
+// Refill many s-grids (sea states) with their second-order forms evaluated
+// in persistent launches over all of them (groups bounded by budget_bytes of
+// device memory); launch shape and tile order picked by an autotuner and
+// cached per workload size.
+struct BatchStats {
+    int groups = 0, batched = 0, single = 0, tiles = 0, blocks = 0, order = 0, tuned = 0;
+    double forms_ms = 0, mfma_flop = 0;
+};
+int refill_batched(Grid** grids, int n, int64_t budget_bytes, bool release, BatchStats* stats);
 
 }  // namespace cfdwm

@@ -104,4 +104,21 @@ struct QfJob {
     int64_t ostride = 0;
 };
 
+// One tile of a batched forms launch: job (sea state), point tile, form.
+struct QfTile {
+    int job, ptile, form, pad;
+};
+
+// Device build of the form tables from the FP32 pair records (qform.hip).
+struct QfBuild {
+    int n = 0, npad = 0, R = 0, nform = 0;
+    const int64_t* frow = nullptr;        // filtered table rows
+    const int32_t* fmhi = nullptr;
+    const UvwRec<float>* uvw = nullptr;   // pk = kappa log2 e
+    const XRec<float>* x = nullptr;
+    const double* tau = nullptr;          // [R] skeleton depths
+    const double* einv = nullptr;         // [R][R]
+    uint16_t* b = nullptr;                // [nform][4][npad x npad], zeroed by the caller
+};
+
 }  // namespace cfdwm

@@ -239,19 +239,10 @@ __device__ __forceinline__ bf16x8 lds8(const char* base, int off) {
     return __builtin_bit_cast(bf16x8, *reinterpret_cast<const uint4*>(base + off));
 }
 
-__global__ __launch_bounds__(256) void k_qf_forms_lds(QfJob J, int npt) {
-    __shared__ __attribute__((aligned(16))) char lds[2][32768];
+// One (point tile, form) of a job: the whole i range for 128 points.
+__device__ __forceinline__ void qf_tile(const QfJob& J, int ptile, int form, char (*lds)[32768]) {
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const int wp = w & 1, wi = w >> 1;
-    // block swizzle: groups of kQfGF forms x kQfGP point tiles, point tile fastest
-    const int id = blockIdx.x;
-    const int per = kQfGF * kQfGP;
-    const int grp = id / per, in = id - grp * per;
-    const int nfg = (J.nform + kQfGF - 1) / kQfGF;
-    const int pg = grp / nfg, fg = grp - pg * nfg;
-    const int ptile = pg * kQfGP + in % kQfGP;
-    const int form = fg * kQfGF + in / kQfGP;
-    if (ptile >= npt || form >= J.nform) return;      // whole block, before any barrier
     const int nkb = J.npad >> 4;
     const size_t nn = (size_t)J.npad * J.npad;
     const uint16_t* __restrict__ Bf = J.b + (size_t)form * 4 * nn;
@@ -388,7 +379,91 @@ __global__ __launch_bounds__(256) void k_qf_forms_lds(QfJob J, int npt) {
     }
 }
 
+__global__ __launch_bounds__(256) void k_qf_forms_lds(QfJob J, int npt) {
+    __shared__ __attribute__((aligned(16))) char lds[2][32768];
+    // block swizzle: groups of kQfGF forms x kQfGP point tiles, point tile fastest
+    const int id = blockIdx.x;
+    const int per = kQfGF * kQfGP;
+    const int grp = id / per, in = id - grp * per;
+    const int nfg = (J.nform + kQfGF - 1) / kQfGF;
+    const int pg = grp / nfg, fg = grp - pg * nfg;
+    const int ptile = pg * kQfGP + in % kQfGP;
+    const int form = fg * kQfGF + in / kQfGP;
+    if (ptile >= npt || form >= J.nform) return;      // whole block, before any barrier
+    qf_tile(J, ptile, form, lds);
+}
+
+// Persistent launch over the tiles of many jobs (sea states): block b takes
+// tiles b, b + gridDim.x, ... of the ordered tile list; every block exits
+// after its last tile.
+__global__ __launch_bounds__(256) void k_qf_forms_persist(const QfJob* __restrict__ jobs,
+                                                          const QfTile* __restrict__ tiles, int ntiles) {
+    __shared__ __attribute__((aligned(16))) char lds[2][32768];
+    for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
+        const QfTile tl = tiles[t];
+        __syncthreads();                              // the previous tile's LDS readers are done
+        qf_tile(jobs[tl.job], tl.ptile, tl.form, lds);
+    }
+}
+
+// Form tables: one thread per (i, m) of the filtered pair set.  Depth-basis
+// coefficients c_q(kappa) = sum_s einv[q][s] e^{-kappa tau_s} in FP64 (einv is
+// ill-conditioned: its entries reach 1e8 while |c| <= 1.4), entries written
+// as bf16 hi + lo into the tiled layout.
+__global__ __launch_bounds__(256) void k_qf_build(QfBuild B) {
+    const int i = blockIdx.y;
+    const int m = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= B.n || m <= i || m >= B.fmhi[i]) return;
+    const int64_t e = B.frow[i] - (i + 1) + m;
+    const UvwRec<float> r = B.uvw[e];
+    const XRec<float> xr = B.x[e];
+    constexpr double kL2eInv = 0.69314718055994530942;   // ln 2: kappa = pk / log2 e
+    const double kp = (double)r.pkP * kL2eInv, km = (double)r.pkM * kL2eInv;
+    double eP[32], eM[32];
+#pragma unroll
+    for (int q = 0; q < 32; ++q) {
+        eP[q] = q < B.R ? exp(-kp * B.tau[q]) : 0.0;
+        eM[q] = q < B.R ? exp(-km * B.tau[q]) : 0.0;
+    }
+    const int nkb = B.npad >> 4;
+    const size_t nn = (size_t)B.npad * B.npad;
+    const size_t o = ((size_t)(i >> 5) * nkb + (m >> 4)) * 512 + ((m >> 3) & 1) * 256 + (i & 31) * 8 + (m & 7);
+    auto put = [&](int f, int k, double v) {
+        uint16_t* d = B.b + ((size_t)f * 4 + 2 * k) * nn;
+        const float x = (float)v;
+        const uint32_t hi = bf16_bits(x);
+        d[o] = (uint16_t)hi;
+        d[o + nn] = (uint16_t)bf16_bits(x - bf16_val(hi));
+    };
+    put(0, 0, (double)xr.xP + (double)xr.xM);
+    put(0, 1, (double)xr.xM - (double)xr.xP);
+    const int R = B.R;
+    for (int q = 0; q < R; ++q) {
+        double cp = 0, cm = 0;
+#pragma unroll
+        for (int s2 = 0; s2 < 32; ++s2) {
+            if (s2 < R) {
+                const double w = B.einv[q * R + s2];
+                cp += w * eP[s2];
+                cm += w * eM[s2];
+            }
+        }
+        put(1 + q, 0, r.cuP * cp + r.cuM * cm);
+        put(1 + q, 1, r.cuM * cm - r.cuP * cp);
+        put(1 + R + q, 0, r.cvP * cp + r.cvM * cm);
+        put(1 + R + q, 1, r.cvM * cm - r.cvP * cp);
+        put(1 + 2 * R + q, 0, r.cwP * cp + r.cwM * cm);
+        put(1 + 2 * R + q, 1, r.cwP * cp - r.cwM * cm);
+    }
+}
+
 }  // namespace
+
+hipError_t launch_qf_build(const QfBuild& B, hipStream_t st) {
+    if (B.n <= 0 || B.R < 0 || B.R > 32 || (B.npad & 127) || B.npad < B.n) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_qf_build, dim3((unsigned)((B.npad + 255) / 256), (unsigned)B.n), dim3(256), 0, st, B);
+    return hipGetLastError();
+}
 
 static bool qf_shape_ok(const QfJob& J) {
     return !((J.ppad & 127) || (J.npad & 127) || J.ppad < J.P || J.npad < J.n);
@@ -399,6 +474,13 @@ hipError_t launch_qf_zgen(const QfJob& J, hipStream_t st) {
     if (!qf_shape_ok(J)) return hipErrorInvalidValue;
     const int64_t nthr = (J.ppad >> 5) * (J.npad >> 4) * 64;
     hipLaunchKernelGGL(k_qf_zgen, dim3((unsigned)((nthr + 255) / 256)), dim3(256), 0, st, J);
+    return hipGetLastError();
+}
+
+hipError_t launch_qf_persist(const QfJob* jobs, const QfTile* tiles, int ntiles, int blocks, hipStream_t st) {
+    if (ntiles <= 0) return hipSuccess;
+    if (blocks <= 0) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_qf_forms_persist, dim3((unsigned)blocks), dim3(256), 0, st, jobs, tiles, ntiles);
     return hipGetLastError();
 }
 

@@ -192,3 +192,15 @@ def depth_basis(kmax, tmax, tol=1e-6, rmax=32):
         raise ValueError("cfdwm_depth_basis: invalid arguments")
     k = abs(r)
     return dict(rank=k, mu=mu[:k].copy(), tau=tau[:k].copy(), err=err.value, ok=r > 0)
+
+
+def refill_batched(grids, budget_gb=96.0, release=True):
+    """Refill many grids with their second-order forms in persistent launches
+    over all of them (autotuned launch shape); returns the batch statistics."""
+    from ._lib import BatchStatsC
+    L = load()
+    arr = (C.c_void_p * len(grids))(*[g.h for g in grids])
+    st = BatchStatsC()
+    check(L.cfdwm_grids_refill_batched(arr, len(grids), float(budget_gb), int(bool(release)), C.byref(st)),
+          "cfdwm_grids_refill_batched")
+    return {f: getattr(st, f) for f, _ in BatchStatsC._fields_}

@@ -177,6 +177,20 @@ CFDWM_EXPORT int cfdwm_grid_query_device_f32(cfdwm_grid* grid, int64_t n, double
  * the pair loop. */
 CFDWM_EXPORT int cfdwm_grid_qform_info(cfdwm_grid* grid, int* nform, int* rank, double* mfma_flop,
                                        double* basis_err);
+/* Batched refill of many s-grids (no reference counterpart: the batched
+ * workload of many sea states).  Equivalent to cfdwm_grid_refill on each grid;
+ * FP32 grids have their second-order forms evaluated in persistent launches
+ * over all grids of a group (groups bounded by budget_gb of device memory),
+ * with the launch shape and tile order picked by an autotuner on the first
+ * call per workload size (stats->tuned = 1; forms_ms then times the last
+ * candidate run).  release != 0 frees each sea's form tables and each grid's
+ * operand workspace after its group (they are rebuilt on the next fill). */
+typedef struct {
+    int groups, batched, single, tiles, blocks, order, tuned;
+    double forms_ms, mfma_flop;
+} cfdwm_batch_stats;
+CFDWM_EXPORT int cfdwm_grids_refill_batched(cfdwm_grid** grids, int n, double budget_gb, int release,
+                                            cfdwm_batch_stats* stats);
 /* Skeleton-exponential depth basis e^{-kappa t} ~ sum_q c_q(kappa) e^{-mu_q t}
  * over kappa in [0, kmax], t in [0, tmax] (host only, no GPU needed): the
  * smallest rank <= rmax whose max error is <= tol.  mu and tau (capacity
