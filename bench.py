@@ -156,6 +156,56 @@ def surrogate_leg(steps: int, warmup: int) -> dict:
             "effective_points_per_s": eff, "fallback_to_exact": not ok}
 
 
+def batched_leg(nstates: int, nsample_err: int) -> dict:
+    """Batched workload: the first nstates of the 512 random sea states (N
+    128-2048, Hs 1-8 m, Tp 6-16 s, depth 20-200 m, half long-crested, half
+    16-direction), each on its own 64 x 64 x 8 s-grid over [0, 20 lambda_p]^2
+    with 4 time levels, refilled together (persistent forms launch, autotuned);
+    per-state error of a sample against FP64 fills."""
+    from cfdwavemaker_amd import engine
+    from cfdwavemaker_amd import surrogate as S
+    from cfdwavemaker_amd.seastate import config_sea
+
+    seas, grids = [], []
+    t0 = time.perf_counter()
+    for st in range(nstates):
+        sea = config_sea(4, st)
+        lam = 2 * np.pi / S.peak_wavenumber(sea)
+        es = engine.Sea(sea.as_dict(), precision=engine.FP32, with_direct=False)
+        g = es.grid((0.0, 20 * lam, 0.0, 20 * lam), 64, 64, 8, 0.0, 0.5, 0.7, 1.5, sea.depth)
+        g.release_workspace()
+        seas.append((st, sea, es))
+        grids.append(g)
+    t_setup = time.perf_counter() - t0
+    engine.refill_batched(grids, release=True)             # first call: autotune
+    engine.sync()
+    t1 = time.perf_counter()
+    stats = engine.refill_batched(grids, release=True)
+    engine.sync()
+    el = time.perf_counter() - t1
+    pts = nstates * 64 * 64 * 8 * 4
+    errs = []
+    for k in range(min(nsample_err, nstates)):
+        st, sea, _ = seas[k]
+        lam = 2 * np.pi / S.peak_wavenumber(sea)
+        g64 = engine.Sea(sea.as_dict(), precision=engine.FP64, with_direct=False).grid(
+            (0.0, 20 * lam, 0.0, 20 * lam), 64, 64, 8, 0.0, 0.5, 0.7, 1.5, sea.depth)
+        e = {"state": st, "components": len(sea.omega)}
+        for f in ("ETA", "UX", "UZ"):
+            ref = g64.export(f)
+            e[f] = float(np.max(np.abs(grids[k].export(f) - ref)) / np.max(np.abs(ref)))
+        e["eta_err_over_hs"] = float(np.max(np.abs(grids[k].export("ETA") - g64.export("ETA"))) / sea.hs)
+        errs.append(e)
+    return {"workload": f"{nstates} of the 512 random sea states, 64x64x8 s-grid x 4 time levels each",
+            "states": nstates, "points": pts, "points_per_s": pts / el, "ms_refill_all": el * 1e3,
+            "setup_s": t_setup, "forms_ms": stats["forms_ms"],
+            "forms_tflops": stats["mfma_flop"] / (stats["forms_ms"] * 1e-3) / 1e12 if stats["forms_ms"] else None,
+            "forms_frac_bf16_peak": stats["mfma_flop"] / (stats["forms_ms"] * 1e-3) / PEAK_MFMA_BF16
+            if stats["forms_ms"] else None,
+            "groups": stats["groups"], "launch": {"blocks": stats["blocks"], "order": stats["order"]},
+            "per_state_err_vs_fp64": errs}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -169,6 +219,8 @@ def main():
     ap.add_argument("--rocprof-summary", nargs=2, metavar=("DB", "COMMAND"), default=None,
                     help="print a markdown summary of a rocprofv3 results database and exit")
     ap.add_argument("--no-surrogate", action="store_true", help="skip the learned-surrogate leg")
+    ap.add_argument("--batched-states", type=int, default=48,
+                    help="sea states of the batched leg (512 = the full batched workload; 0 skips it)")
     ap.add_argument("--calib-copy", action="store_true",
                     help="run one plain 256 MiB device copy first (reference for PMC byte counters)")
     a = ap.parse_args()
@@ -277,6 +329,7 @@ def main():
     u_err = float(np.max(np.abs(gxx.export("UX") - uref)) / np.max(np.abs(uref)))
 
     sur = None if a.no_surrogate else surrogate_leg(a.steps, a.warmup)
+    bat = batched_leg(a.batched_states, 3) if a.batched_states > 0 else None
 
     cpu = None
     if not a.no_cpu:
@@ -314,6 +367,7 @@ def main():
                      "gate": "eta error <= 1e-3 Hs vs FP64", "pass": eta_err <= 1e-3 * HS},
         "cpu_baseline": cpu,
         "surrogate": sur,
+        "batched": bat,
     }
     print(json.dumps(line), flush=True)
 

@@ -774,6 +774,12 @@ int cfdwm_grid_qform_info(cfdwm_grid* grid, int* nform, int* rank, double* mfma_
     if (basis_err) *basis_err = g.qf_basis_err;
     return g.qf_used ? 1 : 0;
 }
+int cfdwm_grid_release_workspace(cfdwm_grid* grid) {
+    if (!grid) return -1;
+    grid->g->sea->release_qform();
+    grid->g->release_workspace();
+    return 0;
+}
 int cfdwm_grids_refill_batched(cfdwm_grid** grids, int n, double budget_gb, int release, cfdwm_batch_stats* out) {
     if (!grids || n < 0) { set_error("bad arguments"); return -1; }
     std::vector<Grid*> gs(n);

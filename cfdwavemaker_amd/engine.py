@@ -144,6 +144,10 @@ class Grid:
         check(self.L.cfdwm_grid_query(self.h, t.size, _p(t), _p(x), _p(y), _p(z), _p(out)), "cfdwm_grid_query")
         return out
 
+    def release_workspace(self):
+        """Free the sea's form tables and this grid's operand workspace."""
+        check(self.L.cfdwm_grid_release_workspace(self.h), "cfdwm_grid_release_workspace")
+
     def qform_info(self):
         """Quadratic forms of the last fill: None when the pair loop ran."""
         nf, rk = C.c_int(), C.c_int()

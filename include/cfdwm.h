@@ -189,6 +189,9 @@ typedef struct {
     int groups, batched, single, tiles, blocks, order, tuned;
     double forms_ms, mfma_flop;
 } cfdwm_batch_stats;
+/* Free the sea's form tables and the grid's operand workspace (rebuilt on the
+ * next FP32 fill); the grid's fields are kept. */
+CFDWM_EXPORT int cfdwm_grid_release_workspace(cfdwm_grid* grid);
 CFDWM_EXPORT int cfdwm_grids_refill_batched(cfdwm_grid** grids, int n, double budget_gb, int release,
                                             cfdwm_batch_stats* stats);
 /* Skeleton-exponential depth basis e^{-kappa t} ~ sum_q c_q(kappa) e^{-mu_q t}
