@@ -90,7 +90,7 @@ def rocprof_summary(db_path: str, command: str) -> str:
         for nm, gx, n, tot, avg, mn, mx in rows:
             if tot < 1e4:
                 continue
-            short = nm.split("(")[0].replace("void ", "")
+            short = nm.replace("(anonymous namespace)::", "").split("(")[0].replace("void ", "")
             lines.append(f"| `{short}` | {gx} | {n} | {tot / 1e6:.3f} | {avg / 1e6:.3f} | {mn / 1e6:.3f} | {mx / 1e6:.3f} |")
     pmc = con.execute("select kernel_name, grid_size, counter_name, count(*), avg(value), avg(duration) "
                       "from counters_collection group by kernel_name, grid_size, counter_name "
@@ -98,7 +98,7 @@ def rocprof_summary(db_path: str, command: str) -> str:
     if pmc:
         lines += ["", "| kernel | grid_size | counter | dispatches | mean KiB/dispatch | mean ms |", "|---|---|---|---|---|---|"]
         for nm, gs, cn, n, v, d in pmc:
-            short = nm.split("(")[0].replace("void ", "")
+            short = nm.replace("(anonymous namespace)::", "").split("(")[0].replace("void ", "")
             lines.append(f"| `{short}` | {gs} | {cn} | {n} | {v:.1f} | {d / 1e6:.3f} |")
     return "\n".join(lines) + "\n"
 
