@@ -35,7 +35,8 @@ BATCH_API = [
     "cfdwm_mlp_create", "cfdwm_mlp_destroy", "cfdwm_mlp_nparams", "cfdwm_mlp_get_params",
     "cfdwm_mlp_set_params", "cfdwm_mlp_set_norm", "cfdwm_mlp_get_norm", "cfdwm_mlp_forward_device",
     "cfdwm_mlp_loss_grad_device", "cfdwm_mlp_train_device", "cfdwm_surrogate_linear", "cfdwm_surrogate_add",
-    "cfdwm_grids_refill_batched", "cfdwm_grid_release_workspace",
+    "cfdwm_grids_refill_batched", "cfdwm_grid_release_workspace", "wave_PrefetchPoints", "wave_PrefetchClear",
+    "wave_PrefetchStats",
 ]
 
 

@@ -5,6 +5,9 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <atomic>
+#include <unordered_map>
+#include <shared_mutex>
 #include <fstream>
 #include <iostream>
 #include <memory>
@@ -388,8 +391,67 @@ double grid_point(double x, double y, double z, double t, int comp) {
     return o4[comp];
 }
 
+// ------------------------------------------------- prefetched point values
+// A CFD host knows the points it will ask for in a time step: evaluating
+// them in one batch (wave_PrefetchPoints) and serving the per-point calls
+// from the results turns one GPU round trip per call into one per batch.
+// Keys are the bit patterns of the call arguments; values are the final API
+// values (z clamp and ramps applied), so a hit returns exactly what the
+// uncached call returns.
+struct PKey {
+    uint64_t a, b, c, d;
+    bool operator==(const PKey& o) const { return a == o.a && b == o.b && c == o.c && d == o.d; }
+};
+struct PKeyHash {
+    size_t operator()(const PKey& k) const noexcept {
+        uint64_t h = k.a * 0x9E3779B97F4A7C15ull;
+        h ^= (k.b + 0x632BE59BD9B4E019ull + (h << 6) + (h >> 2));
+        h ^= (k.c * 0xBF58476D1CE4E5B9ull + (h << 6) + (h >> 2));
+        h ^= (k.d * 0x94D049BB133111EBull + (h << 6) + (h >> 2));
+        return (size_t)h;
+    }
+};
+struct PVals {
+    double u, v, w, p;
+};
+struct PointCache {
+    std::shared_mutex mu;
+    std::unordered_map<PKey, PVals, PKeyHash> vel;     // (x, y, z, t)
+    std::unordered_map<PKey, double, PKeyHash> eta;    // (x, y, 0, t)
+    bool has_vel = false, has_p = false, has_eta = false;
+    std::atomic<int64_t> hits{0}, misses{0};
+};
+PointCache PC;
+inline uint64_t bits(double v) {
+    uint64_t u;
+    std::memcpy(&u, &v, 8);
+    return u;
+}
+bool cached_vel(double x, double y, double z, double t, PVals& out) {
+    std::shared_lock<std::shared_mutex> lk(PC.mu);
+    if (!PC.has_vel) return false;
+    auto it = PC.vel.find(PKey{bits(x), bits(y), bits(z), bits(t)});
+    if (it == PC.vel.end()) { ++PC.misses; return false; }
+    ++PC.hits;
+    out = it->second;
+    return true;
+}
+bool cached_eta(double x, double y, double t, double& out) {
+    std::shared_lock<std::shared_mutex> lk(PC.mu);
+    if (!PC.has_eta) return false;
+    auto it = PC.eta.find(PKey{bits(x), bits(y), 0, bits(t)});
+    if (it == PC.eta.end()) { ++PC.misses; return false; }
+    ++PC.hits;
+    out = it->second;
+    return true;
+}
+
 double velo(double x, double y, double z, double t, int comp) {
     if (!G.initialized) return 0.0;
+    {
+        PVals c;
+        if (cached_vel(x, y, z, t, c)) return comp == 1 ? c.u : (comp == 2 ? c.v : c.w);
+    }
     z = std::max(-G.depth, z);                       // CFDwavemaker.cpp:1547
     if (G.wavetype == 1) {
         double r = 0;
@@ -450,6 +512,7 @@ int wave_Initialize(void) {
 }
 
 int wave_Cleanup(void) {
+    wave_PrefetchClear();
     if (G.cgrid) { cfdwm_grid_destroy(G.cgrid); G.cgrid = nullptr; }
     if (G.csea) { cfdwm_sea_destroy(G.csea); G.csea = nullptr; }
     return 0;
@@ -461,6 +524,10 @@ double wave_VeloZ(double x, double y, double z, double t) { return velo(x, y, z,
 
 double wave_DynPres(double x, double y, double z, double t) {
     if (!G.initialized || G.wavetype != 1) return 0.0;
+    {
+        PVals c;
+        if (PC.has_p && cached_vel(x, y, z, t, c)) return c.p;
+    }
     z = std::max(-G.depth, z);
     double p = 0;
     direct_point(x, y, z, t, nullptr, nullptr, nullptr, nullptr, &p);
@@ -477,6 +544,10 @@ double wave_nhPres(double x, double y, double z, double t) {
 
 double wave_SurfElev(double x, double y, double t) {
     if (!G.initialized) return 0.0;
+    {
+        double e;
+        if (cached_eta(x, y, t, e)) return e;
+    }
     if (G.wavetype == 1) {
         double eta = 0;
         direct_point(x, y, 0.0, t, &eta, nullptr, nullptr, nullptr, nullptr);
@@ -499,6 +570,14 @@ double* wave_Kinematics(double x, double y, double z, double t) {
     thread_local double res[4];
     res[0] = res[1] = res[2] = res[3] = 0.0;
     if (!G.initialized) return res;
+    {
+        PVals c;
+        double e;
+        if (cached_eta(x, y, t, e) && cached_vel(x, y, z, t, c)) {
+            res[0] = e; res[1] = c.u; res[2] = c.v; res[3] = c.w;
+            return res;
+        }
+    }
     if (G.wavetype == 5) {
         double o4[4];
         if (G.cgrid->g->query(1, &t, &x, &y, &z, o4, nullptr, nullptr)) gpu_fatal("s-grid query");
@@ -519,6 +598,7 @@ void wave_force_update(double t) {
     if (!G.initialized || G.wavetype != 5) return;
     int n = 0;
     if (cfdwm_grid_force_update(G.cgrid, t, &n)) gpu_fatal("s-grid update");
+    if (n > 0) wave_PrefetchClear();                  // kept values belong to the previous grid state
 }
 
 void wave_set_mpi_comm(int) {}
@@ -629,6 +709,47 @@ static int eval_points_impl(int64_t n, const double* x, const double* y, const d
     return -1;
 }
 
+int wave_PrefetchPoints(int64_t n, const double* x, const double* y, const double* z, const double* t,
+                        uint32_t flags) {
+    if (!G.initialized) { set_error("wave_Initialize has not been called"); return -1; }
+    if (n < 0 || (n && (!x || !y || !t))) { set_error("bad arguments"); return -1; }
+    const bool we = (flags & CFDWM_OUT_ETA) != 0;
+    const bool wv = (flags & (CFDWM_OUT_U | CFDWM_OUT_V | CFDWM_OUT_W | CFDWM_OUT_P)) != 0;
+    const bool wp = (flags & CFDWM_OUT_P) != 0 && G.wavetype == 1;
+    if (wv && !z) { set_error("z required for velocities"); return -1; }
+    std::vector<double> eta(we ? n : 0), u(wv ? n : 0), v(wv ? n : 0), w(wv ? n : 0), p(wp ? n : 0);
+    if (n) {
+        if (int rc = eval_points_impl(n, x, y, z, t, we ? eta.data() : nullptr, wv ? u.data() : nullptr,
+                                      wv ? v.data() : nullptr, wv ? w.data() : nullptr, wp ? p.data() : nullptr, flags,
+                                      false))
+            return rc;
+    }
+    std::unique_lock<std::shared_mutex> lk(PC.mu);
+    PC.vel.clear();
+    PC.eta.clear();
+    PC.has_vel = wv && n > 0;
+    PC.has_p = wp && n > 0;
+    PC.has_eta = we && n > 0;
+    if (PC.has_vel) PC.vel.reserve((size_t)n);
+    if (PC.has_eta) PC.eta.reserve((size_t)n);
+    for (int64_t q = 0; q < n; ++q) {
+        if (wv) PC.vel[PKey{bits(x[q]), bits(y[q]), bits(z[q]), bits(t[q])}] = PVals{u[q], v[q], w[q], wp ? p[q] : 0.0};
+        if (we) PC.eta[PKey{bits(x[q]), bits(y[q]), 0, bits(t[q])}] = eta[q];
+    }
+    return 0;
+}
+int wave_PrefetchClear(void) {
+    std::unique_lock<std::shared_mutex> lk(PC.mu);
+    PC.vel.clear();
+    PC.eta.clear();
+    PC.has_vel = PC.has_p = PC.has_eta = false;
+    return 0;
+}
+int wave_PrefetchStats(int64_t* hits, int64_t* misses) {
+    if (hits) *hits = PC.hits.load();
+    if (misses) *misses = PC.misses.load();
+    return 0;
+}
 int wave_EvalPoints(int64_t n, const double* x, const double* y, const double* z, const double* t,
                     double* eta, double* u, double* v, double* w, double* p, uint32_t flags) {
     return eval_points_impl(n, x, y, z, t, eta, u, v, w, p, flags, false);

@@ -101,6 +101,17 @@ CFDWM_EXPORT int Cleanup(void);
 CFDWM_EXPORT int wave_EvalPoints(int64_t n, const double* x, const double* y, const double* z,
                                  const double* t, double* eta, double* u, double* v, double* w,
                                  double* p, uint32_t flags);
+/* Prefetch (no reference counterpart; a micro-batcher for per-point hosts):
+ * evaluate the n points (outputs selected by CFDWM_OUT_* flags) in one batch
+ * and keep the results; later wave_SurfElev / wave_VeloX/Y/Z / wave_DynPres /
+ * wave_Kinematics calls whose arguments are bitwise equal to a prefetched
+ * point return the kept value (identical to the uncached result) without a
+ * GPU round trip.  A new prefetch replaces the previous one.  Call from one
+ * thread, outside concurrent per-point calls (like wave_force_update). */
+CFDWM_EXPORT int wave_PrefetchPoints(int64_t n, const double* x, const double* y, const double* z,
+                                     const double* t, uint32_t flags);
+CFDWM_EXPORT int wave_PrefetchClear(void);
+CFDWM_EXPORT int wave_PrefetchStats(int64_t* hits, int64_t* misses);
 /* As wave_EvalPoints, all pointers in device memory (the call synchronises
  * the library's stream before returning). */
 CFDWM_EXPORT int wave_EvalPointsDevice(int64_t n, const double* x, const double* y, const double* z,
