@@ -206,6 +206,59 @@ def batched_leg(nstates: int, nsample_err: int) -> dict:
             "per_state_err_vs_fp64": errs}
 
 
+def direct_leg(steps: int) -> dict:
+    """Direct (per-point semantics) workload: N=512 long-crested sea, 256 x on
+    [0, 500] m x 16 t on [0, 15] s columns, 64 z per column on [-30, eta]:
+    eta1 + eta2 per column (FP32: eta2 as the bf16 MFMA quadratic form) and
+    u, v, w, dp (unfiltered second-order sums) at the 262,144 points, FP32 and
+    FP64, errors of FP32 vs FP64."""
+    import torch
+    from cfdwavemaker_amd import engine
+    from cfdwavemaker_amd.seastate import config_sea
+
+    d = config_sea(2).as_dict()
+    nx, nt, nz = 256, 16, 64
+    xs = np.linspace(0.0, 500.0, nx)
+    ts = np.arange(nt, dtype=float)
+    X, T = (q.ravel() for q in np.meshgrid(xs, ts, indexing="ij"))
+    dev = torch.device("cuda:0")
+    xc = torch.tensor(X, dtype=torch.float64, device=dev)
+    yc = torch.zeros_like(xc)
+    tc = torch.tensor(T, dtype=torch.float64, device=dev)
+    res, fields = {}, {}
+    for prec, name in ((engine.FP32, "fp32"), (engine.FP64, "fp64")):
+        es = engine.Sea(d, precision=prec, with_direct=True)
+
+        def run():
+            c = es.eval_fields_device(tc, xc, yc, want=("eta1", "eta2"))
+            eta = c["eta1"] + c["eta2"]
+            frac = torch.linspace(0.0, 1.0, nz, dtype=torch.float64, device=dev)
+            z = (-30.0 + (eta[:, None] + 30.0) * frac[None, :]).reshape(-1)
+            rep = lambda a: a[:, None].expand(-1, nz).reshape(-1).contiguous()
+            f = es.eval_fields_device(rep(tc), rep(xc), rep(yc), z, want=("u", "v", "w", "p"))
+            return eta, f
+
+        run()
+        engine.sync()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            eta, f = run()
+        engine.sync()
+        el = (time.perf_counter() - t0) / steps
+        res[name] = {"points_per_s": nx * nt * nz / el, "ms": el * 1e3}
+        fields[name] = (eta.cpu().numpy(), {k: v.cpu().numpy() for k, v in f.items()})
+        es.close()
+    e32, f32 = fields["fp32"]
+    e64, f64 = fields["fp64"]
+    err = {"eta_max_err_over_hs": float(np.max(np.abs(e32 - e64)) / 6.0)}
+    for k in ("u", "w", "p"):
+        err[k + "_scaled_err"] = float(np.max(np.abs(f32[k] - f64[k])) / np.max(np.abs(f64[k])))
+    return {"workload": "N=512 long-crested sea, 256 x 16 (x, t) columns x 64 z: eta1+eta2 per column, "
+                        "u, v, w, dp (unfiltered second order) per point", "points": nx * nt * nz,
+            "fp32": res["fp32"], "fp64": res["fp64"], "fp32_vs_fp64": err,
+            "eta2": "fp32: bf16 MFMA quadratic form (hi/lo split, FP32 accumulation)"}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -219,6 +272,7 @@ def main():
     ap.add_argument("--rocprof-summary", nargs=2, metavar=("DB", "COMMAND"), default=None,
                     help="print a markdown summary of a rocprofv3 results database and exit")
     ap.add_argument("--no-surrogate", action="store_true", help="skip the learned-surrogate leg")
+    ap.add_argument("--no-direct", action="store_true", help="skip the direct per-point workload leg")
     ap.add_argument("--batched-states", type=int, default=48,
                     help="sea states of the batched leg (512 = the full batched workload; 0 skips it)")
     ap.add_argument("--calib-copy", action="store_true",
@@ -330,6 +384,7 @@ def main():
 
     sur = None if a.no_surrogate else surrogate_leg(a.steps, a.warmup)
     bat = batched_leg(a.batched_states, 3) if a.batched_states > 0 else None
+    dirl = None if a.no_direct else direct_leg(a.steps)
 
     cpu = None
     if not a.no_cpu:
@@ -368,6 +423,7 @@ def main():
         "cpu_baseline": cpu,
         "surrogate": sur,
         "batched": bat,
+        "direct": dirl,
     }
     print(json.dumps(line), flush=True)
 

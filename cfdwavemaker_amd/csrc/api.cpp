@@ -849,6 +849,18 @@ int cfdwm_eval(cfdwm_sea* cs, int kind, int64_t n, const double* t, const double
     return s.eval_points(n, x, y, z, t, fl, false, nullptr, nullptr, nullptr, o[0], o[1], o[2], o[3], false);
 }
 
+int cfdwm_eval_fields(cfdwm_sea* cs, int64_t n, const double* t, const double* x, const double* y, const double* z,
+                      double* eta1, double* eta2, double* u, double* v, double* w, double* p, int device) {
+    if (!cs || n < 0 || (n && (!t || !x || !y))) { set_error("bad arguments"); return -1; }
+    Sea& s = *cs->s;
+    const int order = s.tab.in.order;
+    const bool lev = u || v || w || p;
+    if (lev && !z) { set_error("z required"); return -1; }
+    if (lev && order >= 1 && !cs->with_direct) { set_error("sea created without the unfiltered table"); return -1; }
+    const uint32_t fl = order_flags(order) | (p ? EV_P : 0u);
+    return s.eval_points(n, x, y, z, t, fl, order == 2 && eta2 != nullptr, eta1, order == 2 ? eta2 : nullptr, nullptr,
+                         u, v, w, p, device != 0);
+}
 int cfdwm_grid_create(cfdwm_grid** out, cfdwm_sea* sea, const cfdwm_grid_desc* d) {
     if (!out || !sea || !d) { set_error("bad arguments"); return -1; }
     GridDesc gd{};

@@ -333,6 +333,55 @@ template <>
 const SeaDev<float>& Sea::dev<float>() const { return d32_; }
 
 namespace {
+// FP32 point batches: the filtered eta2 pair sum as the MFMA quadratic form
+// (form 0 of the sea's tables) when the batch is large and every component
+// has the same frequency-ramp triple (the ramp factor is then one number per
+// point, applied by the column kernel).  Returns 1 when used.
+int points_eta2_form(Sea& s, int64_t n, const double* dx, const double* dy, const double* dt, bool want_eta2,
+                     ColumnJob<float>& J) {
+    const char* e = std::getenv("CFDWM_QFORM");
+    if (!want_eta2 || n < 2048 || (e && std::atoi(e) == 0) || s.nfilt() == 0) return 0;
+    const CompTable& c = s.tab.comp;
+    for (int i = 1; i < c.n; ++i)
+        if (c.rt[i] != c.rt[0] || c.ts[i] != c.ts[0] || c.du[i] != c.du[0]) return 0;
+    const Sea::QfTables* T = s.qform_tables();
+    if (!T) return -3;
+    const int npad = T->npad;
+    const int64_t ppad_all = (n + 127) / 128 * 128;
+    const int64_t cap = std::max<int64_t>(128, ((int64_t)1 << 31) / ((int64_t)npad * 8) / 128 * 128);
+    const int64_t chunk = std::min(ppad_all, cap);
+    const size_t zn = (size_t)chunk * npad;
+    if (s.qz.alloc(zn * 4 * sizeof(uint16_t)) || s.qo.alloc((size_t)ppad_all * sizeof(float))) return -3;
+    QfJob Q;
+    Q.n = c.n; Q.npad = npad;
+    Q.grid = 0; Q.x = dx; Q.y = dy; Q.t = dt;
+    Q.x0 = s.tab.in.x0; Q.y0 = s.tab.in.y0;
+    Q.ph = s.dev<float>().ph;
+    uint16_t* z = (uint16_t*)s.qz.get();
+    Q.zc_hi = z; Q.zc_lo = z + zn; Q.zs_hi = z + 2 * zn; Q.zs_lo = z + 3 * zn;
+    Q.zn = (int64_t)zn;
+    Q.b = (const uint16_t*)T->b.get();
+    Q.ftype = (const int8_t*)T->ftype.get();
+    Q.nform = 1;                                       // eta2 only
+    Q.kend = (const int32_t*)T->kend.get();
+    Q.out = (float*)s.qo.get();
+    Q.ostride = ppad_all;
+    hipStream_t st = Engine::get().stream();
+    for (int64_t p0 = 0; p0 < n; p0 += chunk) {
+        Q.p0 = p0;
+        Q.P = std::min(chunk, n - p0);
+        Q.ppad = (Q.P + 127) / 128 * 128;
+        PROF("k_qf_zgen", launch_qf_zgen(Q, st));
+        PROF("k_qf_forms", launch_qf_forms(Q, st));
+    }
+    J.pairs = 0;
+    J.q2 = (const float*)s.qo.get();
+    J.nq2 = 1;
+    J.q2stride = ppad_all;
+    J.q2_rt = c.rt[0]; J.q2_ts = c.ts[0]; J.q2_du = c.du[0];
+    return 1;
+}
+
 template <typename R>
 int run_points(Sea& s, int64_t n, const double* dx, const double* dy, const double* dz, const double* dt,
                uint32_t flags, bool want_cols, bool want_eta2, bool want_lev, double* o) {
@@ -345,6 +394,11 @@ int run_points(Sea& s, int64_t n, const double* dx, const double* dy, const doub
         J.want_eta2 = want_eta2 ? 1 : 0;
         J.o_eta1 = o; J.o_eta2 = o + n; J.o_gx = o + 2 * n; J.o_gy = o + 3 * n; J.o_gz = o + 4 * n; J.o_gp = o + 5 * n;
         J.nseg = want_eta2 ? pick_nseg(n, s.table_records(false)) : 1;
+        if constexpr (sizeof(R) == 4) {
+            const int rc = points_eta2_form(s, n, dx, dy, dt, want_eta2, J);
+            if (rc < 0) return rc;
+            if (rc == 1) J.nseg = 1;
+        }
         J.seg = s.segments(false, J.nseg);
         if (!J.seg) return -3;
         if (J.nseg > 1) {

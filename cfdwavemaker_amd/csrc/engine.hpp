@@ -29,7 +29,8 @@ struct ColumnJob {
     const float* q2 = nullptr;            // [nq2][q2stride] per-point partials (GRID)
     int nq2 = 0;
     int64_t q2stride = 0;
-    double q2scale[4] = {1, 1, 1, 1};     // per slot ramp factor of the partials
+    double q2scale[4] = {1, 1, 1, 1};     // per slot ramp factor of the partials (GRID)
+    double q2_rt = 0, q2_ts = 0, q2_du = 0;  // POINTS: the ramp triple shared by all components
     double *o_eta1 = nullptr, *o_eta2 = nullptr, *o_gx = nullptr, *o_gy = nullptr, *o_gz = nullptr, *o_gp = nullptr;
     const Segment* seg = nullptr;         // nseg segments of the filtered table (blockIdx.y)
     int nseg = 1;
@@ -221,6 +222,7 @@ public:
     const QfTables* qform_tables();          // FP32 seas only
     void release_qform();
     DevBuf partial;                          // split-K partial sums of point batches
+    DevBuf qz, qo;                           // quadratic-form operands / values of point batches
     std::mutex mu;                           // serialises the per-point scratch use
     // scratch for point batches (inputs 4n doubles, outputs 6n doubles)
     DevBuf pts_in, pts_out;

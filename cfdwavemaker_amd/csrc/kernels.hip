@@ -107,7 +107,9 @@ __device__ __forceinline__ void col_store(const SeaDev<R>& S, const ColumnJob<R>
         }
         J.eta_grid[(int64_t)J.slot_dst[P.slot] * J.ncell + P.col] = (R)(v[0] + e2 + S.swl);
     } else {
-        J.o_eta1[P.q] = v[0]; J.o_eta2[P.q] = v[1];
+        double e2 = v[1];
+        if (J.q2) e2 += freq_ramp(P.t, J.q2_rt, J.q2_ts, J.q2_du) * (double)J.q2[P.q];
+        J.o_eta1[P.q] = v[0]; J.o_eta2[P.q] = e2;
         J.o_gx[P.q] = v[2]; J.o_gy[P.q] = v[3]; J.o_gz[P.q] = v[4]; J.o_gp[P.q] = v[5];
     }
 }

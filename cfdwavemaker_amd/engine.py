@@ -82,6 +82,18 @@ class Sea:
         check(self.L.cfdwm_eval(self.h, k, t.size, _p(t), _p(x), _p(y), _p(zz), _p(out)), f"cfdwm_eval({kind})")
         return out.reshape(shape)
 
+    def eval_fields_device(self, t, x, y, z=None, want=("eta1", "eta2", "u", "v", "w", "p")):
+        """Direct-mode fields in one pass on float64 cuda tensors (no ramps,
+        no z clamp); returns a dict of float64 cuda tensors."""
+        import torch
+        n = t.shape[0]
+        out = {k: torch.empty(n, dtype=torch.float64, device=t.device) for k in want}
+        ptr = lambda k: out[k].data_ptr() if k in out else None
+        check(self.L.cfdwm_eval_fields(self.h, n, t.data_ptr(), x.data_ptr(), y.data_ptr(),
+                                       None if z is None else z.data_ptr(), ptr("eta1"), ptr("eta2"), ptr("u"),
+                                       ptr("v"), ptr("w"), ptr("p"), 1), "cfdwm_eval_fields")
+        return out
+
     def grid(self, domain, nx, ny, nl, t0, dt, tan_a, tan_b, water_depth, swl=0.0,
              domain_ignore=(1e12, -1e12, 1e12, -1e12), ignore_at_init=0) -> "Grid":
         return Grid(self, domain, nx, ny, nl, t0, dt, tan_a, tan_b, water_depth, swl, domain_ignore,

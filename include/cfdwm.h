@@ -160,6 +160,13 @@ typedef struct {
 } cfdwm_grid_desc;
 
 /* lsGridSpline: allocate and fill (initialize_kinematics). */
+/* Direct-mode fields of Irregular::eta/u/v/w/dp (src/Irregular.cpp:9-157)
+ * for n points in one pass (no ramps, no z clamp): eta1, eta2 (elevation =
+ * swl + eta1 + eta2), u, v, w, p; NULL outputs are skipped.  device != 0: all
+ * pointers are device memory. */
+CFDWM_EXPORT int cfdwm_eval_fields(cfdwm_sea* sea, int64_t n, const double* t, const double* x, const double* y,
+                                   const double* z, double* eta1, double* eta2, double* u, double* v, double* w,
+                                   double* p, int device);
 CFDWM_EXPORT int cfdwm_grid_create(cfdwm_grid** out, cfdwm_sea* sea, const cfdwm_grid_desc* desc);
 CFDWM_EXPORT int cfdwm_grid_destroy(cfdwm_grid* grid);
 /* Re-run initialize_kinematics (time ring back to t0). */

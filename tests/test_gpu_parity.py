@@ -162,3 +162,25 @@ def test_eta2_quadratic_form_matches_pair_loop(eng, oracle, monkeypatch):
         print(f"qform {qf} ETA err vs oracle {err:.2e}", flush=True)
         assert err <= 1e-5, (qf, err)
     assert scaled_err(out["1"], out["0"]) <= 1e-5
+
+
+def test_points_eta2_quadratic_form(eng, oracle, monkeypatch):
+    """FP32 point batches (>= 2048 points) take the filtered eta2 pair sum from
+    the MFMA quadratic form; against the FP64 oracle and the FP32 pair loop."""
+    from cfdwavemaker_amd.seastate import config_sea
+    s = config_sea(2).as_dict()
+    rng = np.random.default_rng(21)
+    n = 4096
+    t, x, y = rng.uniform(0, 15, n), rng.uniform(0, 500, n), np.zeros(n)
+    osea = oracle.sea(s)
+    sub = np.arange(0, n, 16)
+    ref = osea.eval("eta2", t[sub], x[sub], y[sub])
+    out = {}
+    for qf in ("1", "0"):
+        monkeypatch.setenv("CFDWM_QFORM", qf)
+        sea = eng.Sea(s, precision=eng.FP32, with_direct=False)
+        out[qf] = sea.eval("eta2", t, x, y)
+        err = scaled_err(out[qf][sub], ref)
+        print(f"points eta2 qform {qf} err vs oracle {err:.2e}", flush=True)
+        assert err <= 1e-5, (qf, err)
+    assert scaled_err(out["1"], out["0"]) <= 1e-5
