@@ -36,7 +36,7 @@ BATCH_API = [
     "cfdwm_mlp_set_params", "cfdwm_mlp_set_norm", "cfdwm_mlp_get_norm", "cfdwm_mlp_forward_device",
     "cfdwm_mlp_loss_grad_device", "cfdwm_mlp_train_device", "cfdwm_surrogate_linear", "cfdwm_surrogate_add",
     "cfdwm_grids_refill_batched", "cfdwm_grid_release_workspace", "wave_PrefetchPoints", "wave_PrefetchClear",
-    "wave_PrefetchStats", "cfdwm_eval_fields",
+    "wave_PrefetchStats", "cfdwm_eval_fields", "cfdwm_grid_write_vtu",
 ]
 
 
@@ -159,6 +159,8 @@ def load(path: str | None = None) -> C.CDLL:
                                          _d, C.c_void_p, C.c_void_p]
     L.cfdwm_surrogate_add.restype = C.c_int
     L.cfdwm_surrogate_add.argtypes = [_i64, C.c_void_p, C.c_void_p, C.c_void_p]
+    L.cfdwm_grid_write_vtu.restype = C.c_int
+    L.cfdwm_grid_write_vtu.argtypes = [C.c_void_p, C.c_char_p, C.c_int, C.c_char_p]
     L.cfdwm_eval_fields.restype = C.c_int
     L.cfdwm_eval_fields.argtypes = [C.c_void_p, _i64] + [C.c_void_p] * 10 + [C.c_int]
     L.cfdwm_grid_release_workspace.restype = C.c_int

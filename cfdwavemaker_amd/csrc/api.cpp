@@ -907,6 +907,10 @@ int cfdwm_grid_qform_info(cfdwm_grid* grid, int* nform, int* rank, double* mfma_
     if (basis_err) *basis_err = g.qf_basis_err;
     return g.qf_used ? 1 : 0;
 }
+int cfdwm_grid_write_vtu(cfdwm_grid* grid, const char* path, int last, const char* time_label) {
+    if (!grid || !path) { set_error("bad arguments"); return -1; }
+    return grid->g->write_vtu(path, last != 0, time_label);
+}
 int cfdwm_grid_release_workspace(cfdwm_grid* grid) {
     if (!grid) return -1;
     grid->g->sea->release_qform();

@@ -1002,6 +1002,80 @@ int64_t Grid::export_field(int which, double* out) {
     return n;
 }
 
+// ASCII VTK XML UnstructuredGrid of the slot that the query uses as its base
+// (last: the following slot), at time t0 (last: t0 + dt): velocity vectors at
+// the nodes, node positions from the sigma map of the slot's elevation,
+// hexahedra between neighbouring nodes (quadrilaterals or lines when the grid
+// is flat in x or y), and the time as field data.
+int Grid::write_vtu(const char* path, bool last, const char* time_label) {
+    const int nx = g.nx, ny = g.ny, nl = g.nl;
+    const int64_t nc = (int64_t)nx * ny, nv = nc * nl;
+    const int tt = last ? (tstep + 2) % 4 : (tstep + 1) % 4;
+    const double tv = last ? t0 + d.dt : t0;
+    std::vector<double> eta(4 * nc), U(4 * nv), V(4 * nv), W(4 * nv);
+    if (export_field(0, eta.data()) < 0 || export_field(4, U.data()) < 0 || export_field(5, V.data()) < 0 ||
+        export_field(6, W.data()) < 0) return -3;
+    FILE* fp = std::fopen(path, "w");
+    if (!fp) { set_error(std::string("cannot open ") + path); return -1; }
+    const char* lab = time_label && time_label[0] ? time_label : "TimeValue";
+    // cells: hexahedra (VTK type 12), or quadrilaterals (9) / lines (3) for flat grids
+    const int ex = nx > 1 ? nx - 1 : 0, ey = ny > 1 ? ny - 1 : 0, el = nl > 1 ? nl - 1 : 0;
+    int ctype = 0, cn = 0;
+    int64_t ncell = 0;
+    if (ex && ey && el) { ctype = 12; cn = 8; ncell = (int64_t)ex * ey * el; }
+    else if ((ex || ey) && el) { ctype = 9; cn = 4; ncell = (int64_t)(ex ? ex : ey) * el; }
+    else if (el) { ctype = 3; cn = 2; ncell = el; }
+    std::fprintf(fp, "<?xml version=\"1.0\"?>\n<VTKFile type=\"UnstructuredGrid\" version=\"1.0\" "
+                     "byte_order=\"LittleEndian\" header_type=\"UInt64\">\n <UnstructuredGrid>\n");
+    std::fprintf(fp, "  <FieldData>\n   <DataArray type=\"Float64\" Name=\"%s\" NumberOfTuples=\"1\" "
+                     "format=\"ascii\">%.6f</DataArray>\n  </FieldData>\n", lab, tv);
+    std::fprintf(fp, "  <Piece NumberOfPoints=\"%lld\" NumberOfCells=\"%lld\">\n", (long long)nv, (long long)ncell);
+    std::fprintf(fp, "   <PointData Vectors=\"Velocity\">\n    <DataArray type=\"Float64\" NumberOfComponents=\"3\" "
+                     "Name=\"Velocity\" format=\"ascii\">\n");
+    const int64_t so = (int64_t)tt * nv;
+    for (int64_t q = 0; q < nv; ++q)
+        std::fprintf(fp, "%.9g %.9g %.9g\n", U[so + q], V[so + q], W[so + q]);
+    std::fprintf(fp, "    </DataArray>\n   </PointData>\n   <Points>\n    <DataArray type=\"Float64\" "
+                     "NumberOfComponents=\"3\" format=\"ascii\">\n");
+    for (int i = 0; i < nx; ++i)
+        for (int j = 0; j < ny; ++j) {
+            const double e = eta[(int64_t)tt * nc + (int64_t)i * ny + j] - g.swl;
+            for (int m = 0; m < nl; ++m) {
+                const double z = e + sigma_[m] * (g.h + e);          // s2z (lsgrid_spline.cpp:85-87)
+                std::fprintf(fp, "%.9g %.9g %.9g\n", g.x0 + g.dx * i, g.y0 + g.dy * j, z);
+            }
+        }
+    std::fprintf(fp, "    </DataArray>\n   </Points>\n   <Cells>\n    <DataArray type=\"Int64\" "
+                     "Name=\"connectivity\" format=\"ascii\">\n");
+    auto node = [&](int i, int j, int m) -> long long { return ((long long)i * ny + j) * nl + m; };
+    if (ctype == 12) {
+        for (int i = 0; i < ex; ++i)
+            for (int j = 0; j < ey; ++j)
+                for (int m = 0; m < el; ++m)
+                    std::fprintf(fp, "%lld %lld %lld %lld %lld %lld %lld %lld\n", node(i, j, m), node(i + 1, j, m),
+                                 node(i + 1, j + 1, m), node(i, j + 1, m), node(i, j, m + 1), node(i + 1, j, m + 1),
+                                 node(i + 1, j + 1, m + 1), node(i, j + 1, m + 1));
+    } else if (ctype == 9) {
+        const int nh = ex ? ex : ey;
+        for (int a = 0; a < nh; ++a)
+            for (int m = 0; m < el; ++m) {
+                const long long p0 = ex ? node(a, 0, m) : node(0, a, m), p1 = ex ? node(a + 1, 0, m) : node(0, a + 1, m);
+                std::fprintf(fp, "%lld %lld %lld %lld\n", p0, p1, p1 + 1, p0 + 1);
+            }
+    } else if (ctype == 3) {
+        for (int m = 0; m < el; ++m) std::fprintf(fp, "%lld %lld\n", node(0, 0, m), node(0, 0, m + 1));
+    }
+    std::fprintf(fp, "    </DataArray>\n    <DataArray type=\"Int64\" Name=\"offsets\" format=\"ascii\">\n");
+    for (int64_t c = 1; c <= ncell; ++c) std::fprintf(fp, "%lld\n", (long long)(c * cn));
+    std::fprintf(fp, "    </DataArray>\n    <DataArray type=\"UInt8\" Name=\"types\" format=\"ascii\">\n");
+    for (int64_t c = 0; c < ncell; ++c) std::fprintf(fp, "%d\n", ctype);
+    std::fprintf(fp, "    </DataArray>\n   </Cells>\n  </Piece>\n </UnstructuredGrid>\n</VTKFile>\n");
+    const bool ok = std::ferror(fp) == 0;
+    std::fclose(fp);
+    if (!ok) { set_error(std::string("write error on ") + path); return -1; }
+    return 0;
+}
+
 template <typename R, typename TQ>
 int Grid::run_query(int64_t n, const TQ* t, double tconst, const TQ* x, const TQ* y, const TQ* z,
                     TQ* eta, TQ* u, TQ* v, TQ* w, const double* ramp) {

@@ -256,6 +256,8 @@ public:
     int refill();                            // initialize_kinematics again (tstep back to 0)
     int force_update(double t, int* n);      // wave_force_update
     int64_t export_field(int which, double* out);
+    // VTK XML unstructured grid of one time slot (lsgrid_spline.cpp:1200-1403 role)
+    int write_vtu(const char* path, bool last, const char* time_label);
     int query(int64_t n, const double* t, const double* x, const double* y, const double* z,
               double* out4, const int* ramp_flags, const double* ramp_se);
     int query_device_f32(int64_t n, double t, const float* x, const float* y, const float* z,

@@ -156,6 +156,11 @@ class Grid:
         check(self.L.cfdwm_grid_query(self.h, t.size, _p(t), _p(x), _p(y), _p(z), _p(out)), "cfdwm_grid_query")
         return out
 
+    def write_vtu(self, path, last=False, time_label="TimeValue"):
+        """ASCII .vtu of the query's base slot (last: the following slot)."""
+        check(self.L.cfdwm_grid_write_vtu(self.h, str(path).encode(), int(bool(last)), time_label.encode()),
+              "cfdwm_grid_write_vtu")
+
     def release_workspace(self):
         """Free the sea's form tables and this grid's operand workspace."""
         check(self.L.cfdwm_grid_release_workspace(self.h), "cfdwm_grid_release_workspace")

@@ -207,6 +207,12 @@ typedef struct {
     int groups, batched, single, tiles, blocks, order, tuned;
     double forms_ms, mfma_flop;
 } cfdwm_batch_stats;
+/* Write the s-grid as an ASCII VTK XML UnstructuredGrid (.vtu) for QA in
+ * ParaView (lsGridSpline::export_vtu, src/lsgrid_spline.cpp:1222-1403): the
+ * slot the spline query uses as its base (last != 0: the following slot), its
+ * time as field data named time_label (NULL: "TimeValue"), velocity vectors at
+ * the nodes, node positions from the sigma map, hexahedral cells. */
+CFDWM_EXPORT int cfdwm_grid_write_vtu(cfdwm_grid* grid, const char* path, int last, const char* time_label);
 /* Free the sea's form tables and the grid's operand workspace (rebuilt on the
  * next FP32 fill); the grid's fields are kept. */
 CFDWM_EXPORT int cfdwm_grid_release_workspace(cfdwm_grid* grid);
