@@ -69,6 +69,15 @@ def load(path: str | None = None) -> C.CDLL:
     if not os.path.exists(p):
         raise RuntimeError(f"{p} is missing: build it with `make -C cfdwavemaker_amd/csrc` "
                            "(there is no CPU fallback)")
+    # torch bundles a HIP runtime whose soname is libamdhip64.so.7 but which
+    # torch's own libraries request as "libamdhip64.so".  Loaded after
+    # libcfdwm (which requests libamdhip64.so.7 from /opt/rocm), torch would
+    # bring in a second runtime that sees no device.  Loading torch first makes
+    # libcfdwm bind to the runtime already in the process.
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     L = C.CDLL(p)
     xyzt = [_d, _d, _d, _d]
     L.CFDwavemaker_is_initialized.restype = C.c_int

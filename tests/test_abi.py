@@ -29,9 +29,9 @@ def test_header_lists_reference_api():
 
 
 def test_library_exports_every_declared_symbol():
-    import ctypes
+    from cfdwavemaker_amd._lib import load
     assert os.path.exists(LIB), "build libcfdwm.so first (make -C cfdwavemaker_amd/csrc)"
-    L = ctypes.CDLL(LIB)
+    L = load(LIB)
     missing = [s for s in declared_symbols() if not hasattr(L, s)]
     assert not missing, missing
 
