@@ -24,8 +24,6 @@
 // checks this and passes one factor per time slot.
 #include <hip/hip_runtime.h>
 
-#include <cstdlib>
-
 #include "engine.hpp"
 #include "kernels.hpp"
 
@@ -45,11 +43,6 @@ __device__ __forceinline__ uint32_t bf16_bits(float f) {
     return u >> 16;
 }
 __device__ __forceinline__ float bf16_val(uint32_t b) { return __uint_as_float(b << 16); }
-
-// tiled operand offset of element (row r, k): 32 x 16 tiles, two 32 x 8 halves
-__device__ __forceinline__ int64_t qf_off(int64_t r, int k, int nkb) {
-    return ((r >> 5) * nkb + (k >> 4)) * 512 + ((k >> 3) & 1) * 256 + (r & 31) * 8 + (k & 7);
-}
 
 // ------------------------------------------------------------------ Z operands
 // One thread per (point, 8 consecutive components): FP64 phase and range
@@ -108,10 +101,6 @@ __global__ __launch_bounds__(256) void k_qf_zgen(QfJob J) {
     *reinterpret_cast<uint4*>(J.zs_lo + o) = pack(sl);
 }
 
-__device__ __forceinline__ bf16x8 ld8(const uint16_t* p) {
-    return __builtin_bit_cast(bf16x8, *reinterpret_cast<const uint4*>(p));
-}
-
 // butterfly reduce-scatter over the 32 lanes of each half-wave: on return
 // v[0] of lane l holds the half-wave sum of the lane values v[l & 31]
 template <int O>
@@ -128,110 +117,17 @@ __device__ __forceinline__ void bfly(float* v, int lane) {
 // ----------------------------------------------------------- quadratic forms
 // Form f of a launch:  Q_f(p) = sum_i e1_pi sum_m B1_f[i][m] c_pm + e2_pi sum_m B2_f[i][m] s_pm
 // with (e1, e2) = (c, s) for cos-type forms and (s, c) for sin-type forms.
-// Block (blockIdx.x, blockIdx.y = f): 4 waves, 128 points; the block walks
-// the i range in tiles of 128, wave (w & 1, w >> 1) taking 64 points x 64 i as
-// 2 x 2 MFMA 32 x 32 blocks for each of the two products.  The k (= m) range
-// of a 64-wide i tile is [i0, kend): pairs have m > i and the filtered rows
-// stop at mhi[i].  Per point the i sums stay in lane registers across tiles;
-// one cross-lane reduction and one LDS exchange between the two waves of a
-// point range at the end write Q_f(p).
-__global__ __launch_bounds__(256) void k_qf_forms(QfJob J) {
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    const int nkb = J.npad >> 4;
-    const int form = blockIdx.y;
-    const size_t nn = (size_t)J.npad * J.npad;
-    const uint16_t* __restrict__ B1h = J.b + (size_t)form * 4 * nn;
-    const uint16_t* __restrict__ B1l = B1h + nn;
-    const uint16_t* __restrict__ B2h = B1h + 2 * nn;
-    const uint16_t* __restrict__ B2l = B1h + 3 * nn;
-    const bool sint = J.ftype[form] != 0;
-    const int64_t P0 = (int64_t)blockIdx.x * 128 + (w & 1) * 64;
-    const int64_t rb0 = P0 >> 5;
-    const int loff = (lane >> 5) * 256 + (lane & 31) * 8;
-    const int hl = lane >> 5, cl32 = lane & 31;
-    float v[32];
-#pragma unroll
-    for (int k = 0; k < 32; ++k) v[k] = 0.f;
-    for (int it = 0; it < J.npad; it += 128) {
-        const int I0 = it + (w >> 1) * 64;
-        const int kb0 = I0 >> 4, kb1 = J.kend[I0 >> 6] >> 4;
-        if (kb1 <= kb0) continue;
-        const int ib0 = I0 >> 5;
-        f32x16 accC[2][2], accS[2][2];
-#pragma unroll
-        for (int a = 0; a < 2; ++a)
-#pragma unroll
-            for (int b = 0; b < 2; ++b)
-#pragma unroll
-                for (int r = 0; r < 16; ++r) { accC[a][b][r] = 0.f; accS[a][b][r] = 0.f; }
-        for (int kb = kb0; kb < kb1; ++kb) {
-            bf16x8 zch[2], zcl[2], zsh[2], zsl[2], bch[2], bcl[2], bsh[2], bsl[2];
-#pragma unroll
-            for (int a = 0; a < 2; ++a) {
-                const int64_t o = ((rb0 + a) * nkb + kb) * 512 + loff;
-                zch[a] = ld8(J.zc_hi + o); zcl[a] = ld8(J.zc_lo + o);
-                zsh[a] = ld8(J.zs_hi + o); zsl[a] = ld8(J.zs_lo + o);
-            }
-#pragma unroll
-            for (int b = 0; b < 2; ++b) {
-                const int64_t o = ((int64_t)(ib0 + b) * nkb + kb) * 512 + loff;
-                bch[b] = ld8(B1h + o); bcl[b] = ld8(B1l + o);
-                bsh[b] = ld8(B2h + o); bsl[b] = ld8(B2l + o);
-            }
-#pragma unroll
-            for (int a = 0; a < 2; ++a)
-#pragma unroll
-                for (int b = 0; b < 2; ++b) {
-                    accC[a][b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(zch[a], bch[b], accC[a][b], 0, 0, 0);
-                    accC[a][b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(zch[a], bcl[b], accC[a][b], 0, 0, 0);
-                    accC[a][b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(zcl[a], bch[b], accC[a][b], 0, 0, 0);
-                    accS[a][b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(zsh[a], bsh[b], accS[a][b], 0, 0, 0);
-                    accS[a][b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(zsh[a], bsl[b], accS[a][b], 0, 0, 0);
-                    accS[a][b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(zsl[a], bsh[b], accS[a][b], 0, 0, 0);
-                }
-        }
-        // epilogue: accumulator element r of lane l is (row 8(r/4) + 4(l/32) + r%4,
-        // column l%32) of its 32 x 32 block; times e1_pi (e2_pi), summed over i
-#pragma unroll
-        for (int a = 0; a < 2; ++a)
-#pragma unroll
-            for (int r = 0; r < 16; ++r) {
-                const int64_t p = P0 + a * 32 + 8 * (r >> 2) + 4 * hl + (r & 3);
-                float sum = 0.f;
-#pragma unroll
-                for (int b = 0; b < 2; ++b) {
-                    const int i = I0 + b * 32 + cl32;
-                    const int64_t o = qf_off(p, i, nkb);
-                    const float c = bf16_val(J.zc_hi[o]) + bf16_val(J.zc_lo[o]);
-                    const float s = bf16_val(J.zs_hi[o]) + bf16_val(J.zs_lo[o]);
-                    sum += accC[a][b][r] * (sint ? s : c) + accS[a][b][r] * (sint ? c : s);
-                }
-                v[a * 16 + r] += sum;
-            }
-    }
-    bfly<16>(v, lane);
-    bfly<8>(v, lane);
-    bfly<4>(v, lane);
-    bfly<2>(v, lane);
-    bfly<1>(v, lane);
-    // waves (w & 1, 0) and (w & 1, 1) hold the two i halves of the same points
-    __shared__ float red[2][64];
-    if (w >> 1) red[w & 1][lane] = v[0];
-    __syncthreads();
-    if (!(w >> 1)) {
-        const int k = cl32;
-        const int64_t p = P0 + (k >> 4) * 32 + 8 * ((k & 15) >> 2) + 4 * hl + (k & 3);
-        if (p < J.P) J.out[(int64_t)form * J.ostride + J.p0 + p] = v[0] + red[w & 1][lane];
-    }
-}
-
-// LDS-staged variant.  Block: 4 waves (one per SIMD), 128 points x 128 i
-// per i tile; wave (w & 1, w >> 1) takes 64 points x 64 i.  Per k step (16
-// values of m) the block stages the A operands of its 128 points and the B
-// operands of its 128 i (4 arrays x 4 row blocks each, 32 KiB) in one of two
-// LDS buffers while the other feeds the matrix cores; each operand tile is
-// read from L2 once per block instead of once per wave.  Blocks are swizzled
-// in groups of 8 forms x 32 point tiles so that the Z and B tables a group
+// One tile of work is (128 points, form): 4 waves, one per SIMD, walk the i
+// range in tiles of 128; wave (w & 1, w >> 1) takes 64 points x 64 i as
+// 2 x 2 MFMA 32 x 32 blocks for each of the two products.  The k (= m)
+// range of an i tile runs from its first row (pairs have m > i) to the end
+// of its filtered rows.  Per k step (16 values of m) the block stages the A
+// operands of its 128 points and the B operands of its 128 i (4 arrays x 4
+// row blocks each, 32 KiB) in LDS, each operand tile read from L2 once per
+// block.  Per point the i sums stay in lane registers across tiles; one
+// cross-lane reduction and one LDS exchange between the two waves of a point
+// range at the end write Q_f(p).  Single-job launches swizzle blocks in
+// groups of 8 forms x 32 point tiles so that the Z and B tables a group
 // reads stay in the last-level cache.
 constexpr int kQfGF = 8, kQfGP = 32;
 
@@ -239,97 +135,137 @@ __device__ __forceinline__ bf16x8 lds8(const char* base, int off) {
     return __builtin_bit_cast(bf16x8, *reinterpret_cast<const uint4*>(base + off));
 }
 
-// One (point tile, form) of a job: the whole i range for 128 points.
-__device__ __forceinline__ void qf_tile(const QfJob J, int ptile, int form) {
-    __shared__ __attribute__((aligned(16))) char lds[2][32768];
+// A wave-uniform global pointer as scalar registers: the address arithmetic
+// before it stays scalar and is not re-associated with per-lane offsets
+// added after it, so loads take the scalar-base + vector-offset form.
+typedef const uint16_t __attribute__((address_space(1)))* gu16p;
+__device__ __forceinline__ gu16p uni_ptr(const uint16_t* p) {
+    const uint64_t a = reinterpret_cast<uint64_t>(p);
+    const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)a);
+    const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(a >> 32));
+    return reinterpret_cast<gu16p>(((uint64_t)hi << 32) | lo);
+}
+
+// Operands move global -> LDS directly (global_load_lds_dwordx4, no
+// registers) through four LDS stages, three steps ahead of the products.
+// Step j of an i tile:
+//   1. s_waitcnt vmcnt(8) lgkmcnt(0) + s_barrier: the requests of step j + 1
+//      (issued during step j - 2) have landed for all waves, and all waves'
+//      LDS reads of step j - 1 are done;
+//   2. the 24 products of step j (operand set X) are issued interleaved with
+//      the 16 LDS reads of step j + 1's operands (set Y, stage (j + 1) % 4)
+//      and the 8 requests of step j + 3 (stage (j + 3) % 4 = (j - 1) % 4,
+//      last read during step j - 2, before step j - 1's barrier), so the
+//      matrix pipe stays busy while they issue.
+// The barrier is an asm statement with a memory clobber after an explicit
+// s_waitcnt: __syncthreads() would also wait for the requests of steps
+// j + 2 and j + 3.  Every wave runs every step of the block's k range: table
+// entries below the pair triangle (m <= i) and past a row tile's filtered end
+// are zero, so the extra products add exact zeros, and the step body has no
+// branch.
+__device__ __forceinline__ void qf_tile_il(const QfJob& J, int ptile, int form) {
+    constexpr int NB = 4;
+    __shared__ __attribute__((aligned(16))) char lds[NB][32768];
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const int wp = w & 1, wi = w >> 1;
     const int nkb = J.npad >> 4;
     const size_t nn = (size_t)J.npad * J.npad;
     const uint16_t* __restrict__ Bf = J.b + (size_t)form * 4 * nn;
     const bool sint = J.ftype[form] != 0;
-    const int64_t PB = (int64_t)ptile * 128;          // first point of the block
-    const int64_t rbB = PB >> 5;                      // first point row block
+    const int64_t PB = (int64_t)ptile * 128;
+    const int64_t rbB = PB >> 5;
     const int64_t P0 = PB + wp * 64;
     const int hl = lane >> 5, cl32 = lane & 31;
-    const int loff = ((lane >> 5) * 256 + (lane & 31) * 8) * 2;   // lane's bytes in a 1 KiB operand tile
-    float tot = 0.f;                                  // this lane's point after the reductions
-    // stage piece q = r * 256 + thread (r < 8): chunk q >> 6 (16 A chunks:
-    // array x 4 row blocks, then 16 B chunks), 16 bytes at (q & 63) * 16
-#define QF_GL1(R, KB, IB0)                                                                         \
-    {                                                                                              \
-        const int q = (R) * 256 + threadIdx.x;                                                     \
-        const int ch = q >> 6, o = (q & 63) * 8;                                                   \
-        const uint16_t* src = ch < 16                                                              \
-            ? J.zc_hi + (int64_t)(ch >> 2) * J.zn + ((rbB + (ch & 3)) * nkb + (KB)) * 512 + o       \
-            : Bf + (size_t)((ch - 16) >> 2) * nn + ((int64_t)((IB0) + (ch & 3)) * nkb + (KB)) * 512 + o; \
-        st##R = *reinterpret_cast<const uint4*>(src);                                              \
-    }
-#define QF_GLOAD(KB, IB0) \
-    QF_GL1(0, KB, IB0) QF_GL1(1, KB, IB0) QF_GL1(2, KB, IB0) QF_GL1(3, KB, IB0) \
-    QF_GL1(4, KB, IB0) QF_GL1(5, KB, IB0) QF_GL1(6, KB, IB0) QF_GL1(7, KB, IB0)
-#define QF_SW1(R, BUF)                                                                             \
-    {                                                                                              \
-        const int q = (R) * 256 + threadIdx.x;                                                     \
-        *reinterpret_cast<uint4*>((BUF) + (q >> 6) * 1024 + (q & 63) * 16) = st##R;                 \
-    }
-#define QF_SWRITE(BUF) \
-    QF_SW1(0, BUF) QF_SW1(1, BUF) QF_SW1(2, BUF) QF_SW1(3, BUF) \
-    QF_SW1(4, BUF) QF_SW1(5, BUF) QF_SW1(6, BUF) QF_SW1(7, BUF)
-    uint4 st0, st1, st2, st3, st4, st5, st6, st7;
+    const int loff = ((lane >> 5) * 256 + (lane & 31) * 8) * 2;
+    float tot = 0.f;
+    const int wv = __builtin_amdgcn_readfirstlane(w);
+    const int lo8 = lane * 8;
+    typedef __attribute__((address_space(3))) void* lds_vp;
+    typedef const __attribute__((address_space(1))) void* glb_vp;
+#define QF4_SYNC(VM)                                                                 \
+    __builtin_amdgcn_s_waitcnt(((VM) & 15) | (7 << 4) | (((VM) >> 4) << 14));        \
+    __asm__ volatile("s_barrier" ::: "memory")
+    // chunk r (< 8) of a stage: A (array r, row block rbB + wv) for r < 4,
+    // B (array r - 4, i row block ib0 + wv) for r >= 4; gp[r] is its address
+    // at k step 0 of the current i tile (wave-uniform)
+    const uint16_t* gp[8];
+    auto dma = [&](int r, int kb, int st) {
+        __builtin_amdgcn_global_load_lds((glb_vp)(uni_ptr(gp[r] + kb * 512) + lo8),
+                                         (lds_vp)(lds[0] + st * 32768 + (r * 4 + wv) * 1024), 16, 0, 0);
+    };
+    // operand fragment f of a stage: f < 8 A (a2 = f / 4, array f % 4), else
+    // B (b = (f - 8) / 4, array f % 4)
+    auto frag = [&](int st, int f) {
+        const char* L = lds[0] + st * 32768;
+        return f < 8 ? lds8(L, ((f & 3) * 4 + wp * 2 + (f >> 2)) * 1024 + loff)
+                     : lds8(L, 16384 + ((f & 3) * 4 + wi * 2 + ((f - 8) >> 2)) * 1024 + loff);
+    };
+    bf16x8 opA[16], opB[16];
     for (int it = 0; it < J.npad; it += 128) {
         const int t0 = it >> 6;
         const int kb_lo = it >> 4;
-        const int kb_hi = max(J.kend[t0], J.kend[t0 + 1]) >> 4;
+        const int kb_hi = __builtin_amdgcn_readfirstlane(max(J.kend[t0], J.kend[t0 + 1]) >> 4);
         if (kb_hi <= kb_lo) continue;                 // block-uniform
         const int I0 = it + wi * 64;
         const int wk0 = I0 >> 4, wk1 = J.kend[t0 + wi] >> 4;
         const int ib0 = it >> 5;
-        f32x16 accC[2][2], accS[2][2];
+        f32x16 acc[8];                                // [(a2 * 2 + b) * 2 + (0 cos part, 1 sin part)]
 #pragma unroll
-        for (int a2 = 0; a2 < 2; ++a2)
+        for (int q = 0; q < 8; ++q)
 #pragma unroll
-            for (int b = 0; b < 2; ++b)
+            for (int r = 0; r < 16; ++r) acc[q][r] = 0.f;
+        // product s (< 24) of a step: level s / 8 (hi x hi, hi x lo, lo x hi),
+        // accumulator s % 8; an accumulator's products are 8 issues apart
+        auto mma = [&](const bf16x8 (&op)[16], int s) {
+            const int lv = s >> 3, q = s & 7, cs = q & 1, a2 = q >> 2, b = (q >> 1) & 1;
+            const int fa = a2 * 4 + 2 * cs + (lv == 2 ? 1 : 0);
+            const int fb = 8 + b * 4 + 2 * cs + (lv == 1 ? 1 : 0);
+            acc[q] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(op[fa], op[fb], acc[q], 0, 0, 0);
+        };
+        // one step: products of operand set x, reads of step j + 1 into set y
+        // from stage s1, requests of step kbd into stage s3; 8 chunks of 3
+        // products, 2 reads and 1 request in a fixed issue order
+        auto step = [&](const bf16x8 (&x)[16], bf16x8 (&y)[16], int s1, int s3, int kbd) {
 #pragma unroll
-                for (int r = 0; r < 16; ++r) { accC[a2][b][r] = 0.f; accS[a2][b][r] = 0.f; }
-        QF_GLOAD(kb_lo, ib0)
-        QF_SWRITE(lds[0])
-        __syncthreads();
-        for (int kb = kb_lo; kb < kb_hi; ++kb) {
-            const int cur = (kb - kb_lo) & 1;
-            const bool more = kb + 1 < kb_hi;
-            if (more) { QF_GLOAD(kb + 1, ib0) }
-            if (kb >= wk0 && kb < wk1) {
-                const char* L = lds[cur];
-#pragma unroll
-                for (int a2 = 0; a2 < 2; ++a2) {
-                    const int rb = wp * 2 + a2;
-                    const bf16x8 zch = lds8(L, (0 * 4 + rb) * 1024 + loff);
-                    const bf16x8 zcl = lds8(L, (1 * 4 + rb) * 1024 + loff);
-                    const bf16x8 zsh = lds8(L, (2 * 4 + rb) * 1024 + loff);
-                    const bf16x8 zsl = lds8(L, (3 * 4 + rb) * 1024 + loff);
-#pragma unroll
-                    for (int b = 0; b < 2; ++b) {
-                        const int ib = wi * 2 + b;
-                        const bf16x8 bch = lds8(L, 16384 + (0 * 4 + ib) * 1024 + loff);
-                        const bf16x8 bcl = lds8(L, 16384 + (1 * 4 + ib) * 1024 + loff);
-                        const bf16x8 bsh = lds8(L, 16384 + (2 * 4 + ib) * 1024 + loff);
-                        const bf16x8 bsl = lds8(L, 16384 + (3 * 4 + ib) * 1024 + loff);
-                        accC[a2][b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(zch, bch, accC[a2][b], 0, 0, 0);
-                        accC[a2][b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(zch, bcl, accC[a2][b], 0, 0, 0);
-                        accC[a2][b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(zcl, bch, accC[a2][b], 0, 0, 0);
-                        accS[a2][b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(zsh, bsh, accS[a2][b], 0, 0, 0);
-                        accS[a2][b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(zsh, bsl, accS[a2][b], 0, 0, 0);
-                        accS[a2][b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(zsl, bsh, accS[a2][b], 0, 0, 0);
-                    }
-                }
+            for (int g = 0; g < 8; ++g) {
+                mma(x, 3 * g);
+                y[2 * g] = frag(s1, 2 * g);
+                mma(x, 3 * g + 1);
+                y[2 * g + 1] = frag(s1, 2 * g + 1);
+                dma(g, kbd, s3);
+                mma(x, 3 * g + 2);
+                __builtin_amdgcn_sched_barrier(0);
             }
-            if (more) { QF_SWRITE(lds[cur ^ 1]) }
-            __syncthreads();
+        };
+#pragma unroll
+        for (int r = 0; r < 8; ++r)
+            gp[r] = r < 4 ? J.zc_hi + (int64_t)r * J.zn + (rbB + wv) * nkb * 512
+                          : Bf + (size_t)(r - 4) * nn + (int64_t)(ib0 + wv) * nkb * 512;
+        // requests past the last step re-read step kb_hi - 1 (in bounds, unused)
+        const int kl = kb_hi - 1;
+#pragma unroll
+        for (int d = 0; d < 3; ++d)
+#pragma unroll
+            for (int r = 0; r < 8; ++r) dma(r, min(kb_lo + d, kl), d);
+        QF4_SYNC(16);
+#pragma unroll
+        for (int f = 0; f < 16; ++f) opA[f] = frag(0, f);
+        int s3 = 3, s1 = 1;                           // stages of steps j + 3 and j + 1
+        for (int kb = kb_lo; kb < kb_hi; kb += 2) {
+            QF4_SYNC(8);
+            step(opA, opB, s1, s3, min(kb + 3, kl));
+            if (kb + 1 >= kb_hi) break;               // block-uniform
+            s3 = (s3 + 1) & 3;
+            s1 = (s1 + 1) & 3;
+            QF4_SYNC(8);
+            step(opB, opA, s1, s3, min(kb + 4, kl));
+            s3 = (s3 + 1) & 3;
+            s1 = (s1 + 1) & 3;
         }
+        // every request has landed and every wave is past its reads before
+        // the next i tile's requests reuse stages 0 .. 2
+        QF4_SYNC(0);
         if (wk1 > wk0) {                              // wave-uniform
-            // element (p, i) of a Z array: ((p >> 5) nkb + (i >> 4)) 512 + ((i >> 3) & 1) 256
-            // + (p & 31) 8 + (i & 7); the lane's rows differ by compile-time offsets
             float v[32];
 #pragma unroll
             for (int a2 = 0; a2 < 2; ++a2) {
@@ -349,7 +285,7 @@ __device__ __forceinline__ void qf_tile(const QfJob J, int ptile, int form) {
                         const int ro = (8 * (r >> 2) + (r & 3)) * 8;
                         const float c = bf16_val(zc[ro]) + bf16_val(zcl[ro]);
                         const float sv = bf16_val(zs[ro]) + bf16_val(zsl[ro]);
-                        sum[r] += accC[a2][b][r] * (sint ? sv : c) + accS[a2][b][r] * (sint ? c : sv);
+                        sum[r] += acc[(a2 * 2 + b) * 2][r] * (sint ? sv : c) + acc[(a2 * 2 + b) * 2 + 1][r] * (sint ? c : sv);
                         if ((r & 3) == 3) __asm__ volatile("" ::: "memory");
                     }
                 }
@@ -364,11 +300,7 @@ __device__ __forceinline__ void qf_tile(const QfJob J, int ptile, int form) {
             tot += v[0];
         }
     }
-#undef QF_GL1
-#undef QF_GLOAD
-#undef QF_SW1
-#undef QF_SWRITE
-    // waves (wp, 0) and (wp, 1) hold the two i halves of the same points
+#undef QF4_SYNC
     float* red = reinterpret_cast<float*>(lds[0]);
     __syncthreads();
     if (wi) red[wp * 64 + lane] = tot;
@@ -380,14 +312,8 @@ __device__ __forceinline__ void qf_tile(const QfJob J, int ptile, int form) {
     }
 }
 
-// Single-job launch.  The body repeats qf_tile's: compiled as this one
-// kernel the schedule runs the N = 1024 fill's 55 forms in 123 ms, through the
-// shared inline function in 145 ms (same instructions, different scheduling).
-__global__ __launch_bounds__(256) void k_qf_forms_lds(QfJob J, int npt) {
-    __shared__ __attribute__((aligned(16))) char lds[2][32768];
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    const int wp = w & 1, wi = w >> 1;
-    // block swizzle: groups of kQfGF forms x kQfGP point tiles, point tile fastest
+// Single-job launch: block -> (point tile, form) through the swizzle.
+__global__ __launch_bounds__(256) void k_qf_forms_il(QfJob J, int npt) {
     const int id = blockIdx.x;
     const int per = kQfGF * kQfGP;
     const int grp = id / per, in = id - grp * per;
@@ -396,140 +322,7 @@ __global__ __launch_bounds__(256) void k_qf_forms_lds(QfJob J, int npt) {
     const int ptile = pg * kQfGP + in % kQfGP;
     const int form = fg * kQfGF + in / kQfGP;
     if (ptile >= npt || form >= J.nform) return;      // whole block, before any barrier
-    const int nkb = J.npad >> 4;
-    const size_t nn = (size_t)J.npad * J.npad;
-    const uint16_t* __restrict__ Bf = J.b + (size_t)form * 4 * nn;
-    const bool sint = J.ftype[form] != 0;
-    const int64_t PB = (int64_t)ptile * 128;          // first point of the block
-    const int64_t rbB = PB >> 5;                      // first point row block
-    const int64_t P0 = PB + wp * 64;
-    const int hl = lane >> 5, cl32 = lane & 31;
-    const int loff = ((lane >> 5) * 256 + (lane & 31) * 8) * 2;   // lane's bytes in a 1 KiB operand tile
-    float tot = 0.f;                                  // this lane's point after the reductions
-    // stage piece q = r * 256 + thread (r < 8): chunk q >> 6 (16 A chunks:
-    // array x 4 row blocks, then 16 B chunks), 16 bytes at (q & 63) * 16
-#define QF_GL1(R, KB, IB0)                                                                         \
-    {                                                                                              \
-        const int q = (R) * 256 + threadIdx.x;                                                     \
-        const int ch = q >> 6, o = (q & 63) * 8;                                                   \
-        const uint16_t* src = ch < 16                                                              \
-            ? J.zc_hi + (int64_t)(ch >> 2) * J.zn + ((rbB + (ch & 3)) * nkb + (KB)) * 512 + o       \
-            : Bf + (size_t)((ch - 16) >> 2) * nn + ((int64_t)((IB0) + (ch & 3)) * nkb + (KB)) * 512 + o; \
-        st##R = *reinterpret_cast<const uint4*>(src);                                              \
-    }
-#define QF_GLOAD(KB, IB0) \
-    QF_GL1(0, KB, IB0) QF_GL1(1, KB, IB0) QF_GL1(2, KB, IB0) QF_GL1(3, KB, IB0) \
-    QF_GL1(4, KB, IB0) QF_GL1(5, KB, IB0) QF_GL1(6, KB, IB0) QF_GL1(7, KB, IB0)
-#define QF_SW1(R, BUF)                                                                             \
-    {                                                                                              \
-        const int q = (R) * 256 + threadIdx.x;                                                     \
-        *reinterpret_cast<uint4*>((BUF) + (q >> 6) * 1024 + (q & 63) * 16) = st##R;                 \
-    }
-#define QF_SWRITE(BUF) \
-    QF_SW1(0, BUF) QF_SW1(1, BUF) QF_SW1(2, BUF) QF_SW1(3, BUF) \
-    QF_SW1(4, BUF) QF_SW1(5, BUF) QF_SW1(6, BUF) QF_SW1(7, BUF)
-    uint4 st0, st1, st2, st3, st4, st5, st6, st7;
-    for (int it = 0; it < J.npad; it += 128) {
-        const int t0 = it >> 6;
-        const int kb_lo = it >> 4;
-        const int kb_hi = max(J.kend[t0], J.kend[t0 + 1]) >> 4;
-        if (kb_hi <= kb_lo) continue;                 // block-uniform
-        const int I0 = it + wi * 64;
-        const int wk0 = I0 >> 4, wk1 = J.kend[t0 + wi] >> 4;
-        const int ib0 = it >> 5;
-        f32x16 accC[2][2], accS[2][2];
-#pragma unroll
-        for (int a2 = 0; a2 < 2; ++a2)
-#pragma unroll
-            for (int b = 0; b < 2; ++b)
-#pragma unroll
-                for (int r = 0; r < 16; ++r) { accC[a2][b][r] = 0.f; accS[a2][b][r] = 0.f; }
-        QF_GLOAD(kb_lo, ib0)
-        QF_SWRITE(lds[0])
-        __syncthreads();
-        for (int kb = kb_lo; kb < kb_hi; ++kb) {
-            const int cur = (kb - kb_lo) & 1;
-            const bool more = kb + 1 < kb_hi;
-            if (more) { QF_GLOAD(kb + 1, ib0) }
-            if (kb >= wk0 && kb < wk1) {
-                const char* L = lds[cur];
-#pragma unroll
-                for (int a2 = 0; a2 < 2; ++a2) {
-                    const int rb = wp * 2 + a2;
-                    const bf16x8 zch = lds8(L, (0 * 4 + rb) * 1024 + loff);
-                    const bf16x8 zcl = lds8(L, (1 * 4 + rb) * 1024 + loff);
-                    const bf16x8 zsh = lds8(L, (2 * 4 + rb) * 1024 + loff);
-                    const bf16x8 zsl = lds8(L, (3 * 4 + rb) * 1024 + loff);
-#pragma unroll
-                    for (int b = 0; b < 2; ++b) {
-                        const int ib = wi * 2 + b;
-                        const bf16x8 bch = lds8(L, 16384 + (0 * 4 + ib) * 1024 + loff);
-                        const bf16x8 bcl = lds8(L, 16384 + (1 * 4 + ib) * 1024 + loff);
-                        const bf16x8 bsh = lds8(L, 16384 + (2 * 4 + ib) * 1024 + loff);
-                        const bf16x8 bsl = lds8(L, 16384 + (3 * 4 + ib) * 1024 + loff);
-                        accC[a2][b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(zch, bch, accC[a2][b], 0, 0, 0);
-                        accC[a2][b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(zch, bcl, accC[a2][b], 0, 0, 0);
-                        accC[a2][b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(zcl, bch, accC[a2][b], 0, 0, 0);
-                        accS[a2][b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(zsh, bsh, accS[a2][b], 0, 0, 0);
-                        accS[a2][b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(zsh, bsl, accS[a2][b], 0, 0, 0);
-                        accS[a2][b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(zsl, bsh, accS[a2][b], 0, 0, 0);
-                    }
-                }
-            }
-            if (more) { QF_SWRITE(lds[cur ^ 1]) }
-            __syncthreads();
-        }
-        if (wk1 > wk0) {                              // wave-uniform
-            // element (p, i) of a Z array: ((p >> 5) nkb + (i >> 4)) 512 + ((i >> 3) & 1) 256
-            // + (p & 31) 8 + (i & 7); the lane's rows differ by compile-time offsets
-            float v[32];
-#pragma unroll
-            for (int a2 = 0; a2 < 2; ++a2) {
-                float sum[16];
-#pragma unroll
-                for (int r = 0; r < 16; ++r) sum[r] = 0.f;
-#pragma unroll
-                for (int b = 0; b < 2; ++b) {
-                    const int i = I0 + b * 32 + cl32;
-                    const uint16_t* zc = J.zc_hi + (((P0 >> 5) + a2) * nkb + (i >> 4)) * 512 + ((i >> 3) & 1) * 256
-                                       + 4 * hl * 8 + (i & 7);
-                    const uint16_t* zcl = zc + J.zn;
-                    const uint16_t* zs = zc + 2 * J.zn;
-                    const uint16_t* zsl = zc + 3 * J.zn;
-#pragma unroll
-                    for (int r = 0; r < 16; ++r) {
-                        const int ro = (8 * (r >> 2) + (r & 3)) * 8;
-                        const float c = bf16_val(zc[ro]) + bf16_val(zcl[ro]);
-                        const float sv = bf16_val(zs[ro]) + bf16_val(zsl[ro]);
-                        sum[r] += accC[a2][b][r] * (sint ? sv : c) + accS[a2][b][r] * (sint ? c : sv);
-                        if ((r & 3) == 3) __asm__ volatile("" ::: "memory");
-                    }
-                }
-#pragma unroll
-                for (int r = 0; r < 16; ++r) v[a2 * 16 + r] = sum[r];
-            }
-            bfly<16>(v, lane);
-            bfly<8>(v, lane);
-            bfly<4>(v, lane);
-            bfly<2>(v, lane);
-            bfly<1>(v, lane);
-            tot += v[0];
-        }
-    }
-#undef QF_GL1
-#undef QF_GLOAD
-#undef QF_SW1
-#undef QF_SWRITE
-    // waves (wp, 0) and (wp, 1) hold the two i halves of the same points
-    float* red = reinterpret_cast<float*>(lds[0]);
-    __syncthreads();
-    if (wi) red[wp * 64 + lane] = tot;
-    __syncthreads();
-    if (!wi) {
-        const int k = cl32;
-        const int64_t p = P0 + (k >> 4) * 32 + 8 * ((k & 15) >> 2) + 4 * hl + (k & 3);
-        if (p < J.P) J.out[(int64_t)form * J.ostride + J.p0 + p] = tot + red[wp * 64 + lane];
-    }
+    qf_tile_il(J, ptile, form);
 }
 
 // Persistent launch over the tiles of many jobs (sea states): block b takes
@@ -539,8 +332,9 @@ __global__ __launch_bounds__(256) void k_qf_forms_persist(const QfJob* __restric
                                                           const QfTile* __restrict__ tiles, int ntiles) {
     for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
         const QfTile tl = tiles[t];
+        const QfJob J = jobs[tl.job];
         __syncthreads();                              // the previous tile's LDS readers are done
-        qf_tile(jobs[tl.job], tl.ptile, tl.form);
+        qf_tile_il(J, tl.ptile, tl.form);
     }
 }
 
@@ -625,15 +419,10 @@ hipError_t launch_qf_persist(const QfJob* jobs, const QfTile* tiles, int ntiles,
 hipError_t launch_qf_forms(const QfJob& J, hipStream_t st) {
     if (J.P <= 0 || J.nform <= 0) return hipSuccess;
     if (!qf_shape_ok(J)) return hipErrorInvalidValue;
-    const char* v = std::getenv("CFDWM_QF_KERNEL");
-    if (v && std::atoi(v) == 0) {
-        hipLaunchKernelGGL(k_qf_forms, dim3((unsigned)(J.ppad / 128), (unsigned)J.nform), dim3(256), 0, st, J);
-    } else {
-        const int npt = (int)(J.ppad / 128);
-        const int nfg = (J.nform + kQfGF - 1) / kQfGF, npg = (npt + kQfGP - 1) / kQfGP;
-        const unsigned nblk = (unsigned)((int64_t)nfg * npg * kQfGF * kQfGP);
-        hipLaunchKernelGGL(k_qf_forms_lds, dim3(nblk), dim3(256), 0, st, J, npt);
-    }
+    const int npt = (int)(J.ppad / 128);
+    const int nfg = (J.nform + kQfGF - 1) / kQfGF, npg = (npt + kQfGP - 1) / kQfGP;
+    const unsigned nblk = (unsigned)((int64_t)nfg * npg * kQfGF * kQfGP);
+    hipLaunchKernelGGL(k_qf_forms_il, dim3(nblk), dim3(256), 0, st, J, npt);
     return hipGetLastError();
 }
 
