@@ -76,6 +76,35 @@ def cpu_baseline(ncols: int, threads: int):
             return json.load(f)
 
 
+# PMC calibration of the committed profiles: a 256 MiB device copy reads
+# 131,088 KiB by FETCH_SIZE and a 256 MiB fill writes 262,144 KiB by WRITE_SIZE
+FETCH_SCALE, WRITE_SCALE = 2.0, 1.0
+PROFILE_BASE = os.path.join(ROOT, "profiles", "sgrid_fill_query_N1024_256x256x16x4_fp32_")
+
+
+def profiled_traffic(kernel: str):
+    """HBM-side bytes per launch of the largest launch of `kernel` from the
+    committed FETCH_SIZE / WRITE_SIZE summaries (rocprofv3 --pmc, KiB per
+    dispatch, calibrated as above); None if they do not list it."""
+    tot = 0.0
+    for counter, scale in (("fetch_size", FETCH_SCALE), ("write_size", WRITE_SCALE)):
+        best = None
+        try:
+            with open(PROFILE_BASE + counter + ".md") as f:
+                for line in f:
+                    c = [x.strip() for x in line.strip().strip("|").split("|")]
+                    if len(c) == 6 and kernel in c[0] and c[2] == counter.upper():
+                        g, kib = int(c[1]), float(c[4])
+                        if best is None or g > best[0]:
+                            best = (g, kib)
+        except OSError:
+            return None
+        if best is None:
+            return None
+        tot += best[1] * 1024.0 * scale
+    return tot
+
+
 def rocprof_summary(db_path: str, command: str) -> str:
     """Markdown summary of a rocprofv3 results database: per kernel and launch
     size the dispatch count and durations, and the mean of any PMC counter
@@ -362,7 +391,9 @@ def main():
         flop_launch = qfi["mfma_flop"] / per_fill
         roof = {"bound": "mfma", "kernel": "k_qf_forms", "achieved": flop_launch / avg / 1e12,
                 "peak": PEAK_MFMA_BF16 / 1e12, "unit": "TFLOP/s", "frac": flop_launch / avg / PEAK_MFMA_BF16,
-                "traffic": None, "avg_launch_ms": avg * 1e3, "flop_per_launch": flop_launch,
+                "traffic": profiled_traffic("k_qf_forms"),
+                "traffic_source": "profiles/*_fetch_size.md + *_write_size.md (rocprofv3 --pmc, calibrated)",
+                "avg_launch_ms": avg * 1e3, "flop_per_launch": flop_launch,
                 "forms": qfi["nform"], "depth_basis_rank": qfi["rank"], "depth_basis_err": qfi["basis_err"]}
     else:
         lev_ms, lev_n = prof.get("k_levels_grid", (float("nan"), 1))
