@@ -25,11 +25,13 @@ _dp = C.POINTER(C.c_double)
 _ip = C.POINTER(C.c_int)
 _bp = C.POINTER(C.c_ubyte)
 
-# evaluation kinds (cfdwm_oracle.h / ref_harness.cpp share the numbering)
+# evaluation kinds (cfdwm_oracle.h / ref_harness.cpp share the numbering):
+# the Sharma-Dean members, then the Smit et al. members
 KINDS = ["eta1", "eta2", "eta", "u1", "v1", "w1", "dp1", "phi1_dxdz", "phi1_dydz",
          "phi1_dzdz", "dp1_dz", "u2", "v2", "w2", "dp2", "u", "v", "w", "dp",
          "uvw2_u", "uvw2_v", "uvw2_w", "uvw2_p"]
-KIND = {k: i for i, k in enumerate(KINDS)}
+SMIT_KINDS = ["u_smit", "v_smit", "w_smit", "dp_smit"]
+KIND = {k: i for i, k in enumerate(KINDS + SMIT_KINDS)}
 GRID_FIELDS = ["ETA", "ETAdx", "ETAdy", "ETAdt", "UX", "UY", "UZ", "Udt", "Vdt", "Wdt",
                "Udx", "Udy", "Uds", "Vdx", "Vdy", "Vds", "Wdx", "Wdy", "Wds", "IGNORE"]
 
@@ -90,6 +92,8 @@ class Oracle:
         L.or_pair_count.restype = C.c_long
         L.or_pair_count.argtypes = [C.c_void_p]
         L.or_pair_export.argtypes = [C.c_void_p, _ip, _bp, _dp]
+        L.or_sea_set_vertical_theory.argtypes = [C.c_void_p, C.c_int]
+        L.or_pair_export_h.argtypes = [C.c_void_p, _dp]
         L.or_eval.argtypes = [C.c_void_p, C.c_int, C.c_long, _dp, _dp, _dp, _dp, _dp]
         L.or_freq_ramp.restype = C.c_double
         L.or_freq_ramp.argtypes = [C.c_double] * 4
@@ -166,6 +170,17 @@ class OracleSea:
         v = np.zeros((n, 7))
         self.o.lib.or_pair_export(self.h, im.ctypes.data_as(_ip), fl.ctypes.data_as(_bp), _ptr(v))
         return im, fl, v
+
+    def set_vertical_theory(self, vt):
+        """1: u/v/w/dp dispatch to the Smit members (Irregular.cpp:25, :53, :77, :143)."""
+        self.o.lib.or_sea_set_vertical_theory(self.h, int(vt))
+
+    def pairs_h(self):
+        """Smit H_nm_plus, H_nm_minus of each pair-cache entry."""
+        n = self.o.lib.or_pair_count(self.h)
+        h = np.zeros((n, 2))
+        self.o.lib.or_pair_export_h(self.h, _ptr(h))
+        return h
 
     def eval(self, kind, t, x, y, z=None):
         t, x, y = np.broadcast_arrays(np.asarray(t, float), np.asarray(x, float), np.asarray(y, float))
@@ -258,6 +273,8 @@ class RefHarness:
         L.refh_pair_count.restype = C.c_long
         L.refh_pair_export.argtypes = [_ip, _bp, _dp]
         L.refh_kmax.restype = C.c_double
+        L.refh_set_vertical_theory.argtypes = [C.c_int]
+        L.refh_pair_export_h.argtypes = [_dp]
         L.refh_grid_init.restype = C.c_double
         L.refh_grid_init.argtypes = [_dp, C.c_int, C.c_int, C.c_int, C.c_double, C.c_double,
                                      C.c_double, C.c_double, C.c_double, C.c_double, _dp,
@@ -310,6 +327,15 @@ class RefHarness:
 
     def kmax(self):
         return self.lib.refh_kmax()
+
+    def set_vertical_theory(self, vt):
+        self.lib.refh_set_vertical_theory(int(vt))
+
+    def pairs_h(self):
+        n = self.lib.refh_pair_count()
+        h = np.zeros((n, 2))
+        self.lib.refh_pair_export_h(_ptr(h))
+        return h
 
     def grid_init(self, domain, nx, ny, nl, t0, dt, tan_a, tan_b, water_depth, swl=0.0,
                   domain_ignore=(1e12, -1e12, 1e12, -1e12), ignore_domain=0, ignore_at_init=0):

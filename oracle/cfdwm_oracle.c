@@ -36,6 +36,9 @@ struct or_sea {
     int *pi, *pm;
     unsigned char *pflag;
     double *gam, *kp, *km, *Dp, *Dm, *ap, *am;
+    double *Hp, *Hm;              /* Smit et al. H coefficients (Irregular.cpp:919-960) */
+    int pairs_built;
+    int vt;                       /* vertical_theory: 0 Sharma & Dean, 1 Smit (Irregular.h:83) */
 };
 
 static double* dup_or(const double* p, int n, double dflt) {
@@ -145,7 +148,8 @@ static void build_component_cache(or_sea* s) {
 }
 
 /* Sharma-Dean sum/difference transfer coefficients, in the order of
- * Irregular.cpp:903-960 (the Smit H coefficients are not on this path). */
+ * Irregular.cpp:903-960, with the Smit H coefficients of the same entries
+ * (Irregular.cpp:919-927, :950-958: Eq. 10 inverted with C = alpha / 2). */
 static void build_pair_cache(or_sea* s) {
     const int N = s->n;
     long cap = (long)N * (N - 1) / 2 + 1;
@@ -154,6 +158,7 @@ static void build_pair_cache(or_sea* s) {
     s->gam = dup_or(NULL, cap, 0.); s->kp = dup_or(NULL, cap, 0.); s->km = dup_or(NULL, cap, 0.);
     s->Dp = dup_or(NULL, cap, 0.); s->Dm = dup_or(NULL, cap, 0.);
     s->ap = dup_or(NULL, cap, 0.); s->am = dup_or(NULL, cap, 0.);
+    s->Hp = dup_or(NULL, cap, 0.); s->Hm = dup_or(NULL, cap, 0.);
     long e = 0;
     for (int i = 0; i < N; ++i) {
         if (s->omega[i] > s->cutoff) continue;
@@ -168,14 +173,18 @@ static void build_pair_cache(or_sea* s) {
             const double kminus = sqrt(ki2 + km2 - 2.0 * kikm * gamma);
             const double sp = Rsni + Rsm, sm = Rsni - Rsm;
             const double R2p = sp * sp, R2m = sm * sm;
+            const double om = s->omega[m];
             int hp = 0, hm = 0;
-            double Dp = 0.0, Dm = 0.0, ap = 0.0, am = 0.0;
+            double Dp = 0.0, Dm = 0.0, ap = 0.0, am = 0.0, Hp = 0.0, Hm = 0.0;
             if ((ki + km) <= s->kmax) {
                 const double den = R2p - kplus * tanh(kplus * s->depth);
                 Dp = (2.0 * R2p * (kikm * gamma - Rni * Rnm)) / den +
                      (sp * (Rsni * (km2 - Rnm * Rnm) + Rsm * (ki2 - Rni * Rni))) / den;
                 ap = (Dp - (kikm * gamma - Rni * Rnm)) / sqrt(Rni * Rnm) + (Rni + Rnm);
                 hp = 1;
+                const double om12 = oi + om, k1dk2 = kikm * gamma;
+                Hp = -s->g / om12 * (ap / 2.0 - (oi * oi + om * om + oi * om) / (2.0 * s->g)
+                                     + s->g * k1dk2 / (2.0 * oi * om));
             }
             if (ki != km) {
                 const double den = R2m - kminus * tanh(kminus * s->depth);
@@ -183,16 +192,22 @@ static void build_pair_cache(or_sea* s) {
                      (sm * (-Rsni * (km2 - Rnm * Rnm) + Rsm * (ki2 - Rni * Rni))) / den;
                 am = (Dm - (kikm * gamma + Rni * Rnm)) / sqrt(Rni * Rnm) + (Rni + Rnm);
                 hm = 1;
+                const double om12 = oi - om, k1dk2 = kikm * gamma;
+                if (fabs(om12) > OR_SMALL)
+                    Hm = -s->g / om12 * (am / 2.0 - (oi * oi + om * om - oi * om) / (2.0 * s->g)
+                                         + s->g * k1dk2 / (2.0 * oi * om));
             }
             if (hp || hm) {
                 s->pi[e] = i; s->pm[e] = m; s->pflag[e] = (unsigned char)(hp | (hm << 1));
                 s->gam[e] = gamma; s->kp[e] = kplus; s->km[e] = kminus;
                 s->Dp[e] = Dp; s->Dm[e] = Dm; s->ap[e] = ap; s->am[e] = am;
+                s->Hp[e] = Hp; s->Hm[e] = Hm;
                 ++e;
             }
         }
     }
     s->np = e;
+    s->pairs_built = 1;
 }
 
 or_sea* or_sea_create(int n, const double* omega, const double* A, const double* k,
@@ -232,7 +247,7 @@ void or_sea_destroy(or_sea* s) {
     if (!s) return;
     double* arrs[] = {s->omega, s->A, s->k, s->phase, s->theta, s->rt, s->ts, s->du, s->cth, s->sth,
                       s->R, s->Rs, s->ag, s->cosh2kd, s->ph0, s->gam, s->kp, s->km, s->Dp, s->Dm,
-                      s->ap, s->am};
+                      s->ap, s->am, s->Hp, s->Hm};
     for (size_t q = 0; q < sizeof(arrs) / sizeof(arrs[0]); ++q) free(arrs[q]);
     free(s->pi); free(s->pm); free(s->pflag);
     free(s);
@@ -243,6 +258,15 @@ void or_sea_components(const or_sea* s, double* A, double* phase, double* bandwi
     *bandwidth = s->bw;
 }
 double or_sea_kmax(const or_sea* s) { return s->kmax; }
+void or_sea_set_vertical_theory(or_sea* s, int vt) {
+    s->vt = vt;
+    /* the Smit members build the second-order caches when absent
+     * (Irregular.cpp:1536, :1634) */
+    if (vt == 1 && !s->pairs_built) build_pair_cache(s);
+}
+void or_pair_export_h(const or_sea* s, double* h2) {
+    for (long e = 0; e < s->np; ++e) { h2[2 * e] = s->Hp[e]; h2[2 * e + 1] = s->Hm[e]; }
+}
 long or_pair_count(const or_sea* s) { return s->np; }
 void or_pair_export(const or_sea* s, int* im, unsigned char* flags, double* v) {
     for (long e = 0; e < s->np; ++e) {
@@ -531,6 +555,142 @@ static double direct2(const or_sea* s, int comp, double t, double x, double y, d
 }
 
 /* ------------------------------------------------------------ dispatch (a12) */
+/* ------------------------------------- Smit et al. (2017) s-coordinate theory */
+/* Stretched profiles, Irregular.cpp:122-141: the vertical argument k (z + h)
+ * becomes k h (h + z) / (h + eta) (Wheeler stretching). */
+static double sm_pX(const or_sea* s, double kk, double z, double eta) {
+    const double sf = s->depth * (s->depth + z) / (s->depth + eta);
+    return cosh(kk * sf) / cosh(kk * s->depth);
+}
+static double sm_pZ(const or_sea* s, double kk, double z, double eta) {
+    const double sf = s->depth * (s->depth + z) / (s->depth + eta);
+    return sinh(kk * sf) / cosh(kk * s->depth);
+}
+/* Diagonal H from the Sharma-Dean alpha_nn through Eq. 10, as each of
+ * u_smit/v_smit/w_smit/dp_smit writes it (e.g. Irregular.cpp:1672-1678). */
+static double sm_Hdiag(const or_sea* s, int i) {
+    const double ki = s->k[i], oi = s->omega[i], om2 = 2.0 * oi;
+    const double Rni = s->R[i], Rsni = s->Rs[i];
+    const double D_nn = (6.0 * Rsni * Rsni * (ki * ki - Rni * Rni)) / (2.0 * Rsni * Rsni - ki * tanh(2.0 * ki * s->depth));
+    const double alpha_nn = (D_nn - (ki * ki - Rni * Rni)) / Rni + 2.0 * Rni;
+    return -s->g / om2 * (alpha_nn / 2.0 - (3.0 * oi * oi) / (2.0 * s->g) + s->g * ki * ki / (2.0 * oi * oi));
+}
+static double sm_phase(const or_sea* s, int i, double t, double x, double y) {
+    return s->k[i] * (s->cth[i] * (x - s->x0) + s->sth[i] * (y - s->y0)) - s->omega[i] * t + s->ph0[i];
+}
+
+/* u_smit / v_smit / w_smit / dp_smit, Irregular.cpp:1623-2027.  comp: 0 u,
+ * 1 v, 2 w, 3 dp.  The stretch height is the Sharma-Dean eta1 + eta2
+ * (swl excluded); second-order terms for order >= 1. */
+static double smit(const or_sea* s, int comp, double t, double x, double y, double zz) {
+    const int N = s->n;
+    double eta_val = eta1(s, t, x, y);
+    if (s->order >= 1) eta_val += eta2(s, t, x, y);
+    if (zz > eta_val + s->swl) return 0.0;
+    zz = dmax(-s->depth, zz - s->swl);
+    const double g = s->g, rho = s->rho, depth = s->depth;
+    double v1 = 0.0;
+    for (int i = 0; i < N; ++i) {
+        const double phi = sm_phase(s, i, t, x, y);
+        const double f = fr(s, i, t);
+        if (comp == 0) v1 += s->A[i] * g * s->k[i] / s->omega[i] * sm_pX(s, s->k[i], zz, eta_val) * s->cth[i] * cos(phi) * f;
+        else if (comp == 1) v1 += s->A[i] * g * s->k[i] / s->omega[i] * sm_pX(s, s->k[i], zz, eta_val) * s->sth[i] * cos(phi) * f;
+        else if (comp == 2) v1 += s->A[i] * g * s->k[i] / s->omega[i] * sm_pZ(s, s->k[i], zz, eta_val) * sin(phi) * f;
+        else v1 += s->A[i] * rho * g * sm_pX(s, s->k[i], zz, eta_val) * cos(phi) * f;
+    }
+    if (s->order < 1) return v1;
+    double v2 = 0.0, dphi2ds = 0.0;
+    const double sp1 = (depth + zz) / (depth + eta_val), g1s = g * sp1;
+    for (int i = 0; i < N; ++i) {                                       /* diagonal */
+        const double phi_i = sm_phase(s, i, t, x, y);
+        const double f = fr(s, i, t);
+        const double ki = s->k[i], oi = s->omega[i];
+        const double AA = s->A[i] * s->A[i] * f * f;
+        const double k2 = 2.0 * ki, om2 = 2.0 * oi;
+        const double H_diag = sm_Hdiag(s, i);
+        if (comp == 0) {                                                /* :1657-1689 */
+            const double kx2 = 2.0 * ki * s->cth[i];
+            const double Ch2 = sm_pX(s, k2, zz, eta_val);
+            v2 += 0.5 * AA * H_diag * kx2 * (-cos(2.0 * phi_i)) * Ch2;
+            const double Sh_i = sm_pZ(s, ki, zz, eta_val);
+            v2 += 0.5 * AA * kx2 * g1s * (ki * Sh_i / oi) * cos(2.0 * phi_i);
+            const double Ch_i = sm_pX(s, ki, zz, eta_val);
+            const double Sh2 = sm_pZ(s, k2, zz, eta_val);
+            dphi2ds += -0.5 * AA * H_diag * (k2 * depth * Sh2) * sin(2.0 * phi_i);
+            dphi2ds += 0.5 * AA * g * (ki * Sh_i / oi + sp1 * ki * (ki * depth * Ch_i) / oi) * sin(2.0 * phi_i);
+        } else if (comp == 1) {                                         /* :1797-1799 */
+            const double ky2 = 2.0 * ki * s->sth[i];
+            const double Ch2 = sm_pX(s, k2, zz, eta_val);
+            v2 += 0.5 * AA * H_diag * ky2 * (-cos(2.0 * phi_i)) * Ch2;
+        } else if (comp == 2) {                                         /* :1888-1890 */
+            const double Sh2 = sm_pZ(s, k2, zz, eta_val);
+            v2 += 0.5 * AA * H_diag * k2 * (-sin(2.0 * phi_i)) * Sh2;
+        } else {                                                        /* :1961-1967 */
+            const double Ch2 = sm_pX(s, k2, zz, eta_val);
+            v2 += 0.5 * AA * rho * om2 * H_diag * (-cos(2.0 * phi_i)) * Ch2;
+            const double Ch_i = sm_pX(s, ki, zz, eta_val);
+            const double Sh_i = sm_pZ(s, ki, zz, eta_val);
+            v2 -= 0.5 * AA * rho * g * g * ki * ki * (Ch_i * Ch_i - Sh_i * Sh_i) / (oi * oi) * cos(2.0 * phi_i);
+        }
+    }
+    for (long e = 0; e < s->np; ++e) {                                  /* pairs */
+        const int i = s->pi[e], m = s->pm[e];
+        const double Ai = s->A[i], Am = s->A[m], ki = s->k[i], km = s->k[m];
+        const double oi = s->omega[i], om = s->omega[m];
+        const double phi_i = sm_phase(s, i, t, x, y), phi_m = sm_phase(s, m, t, x, y);
+        const double AA = Ai * Am * fr(s, i, t) * fr(s, m, t);
+        const double Sh_i = sm_pZ(s, ki, zz, eta_val), Sh_m = sm_pZ(s, km, zz, eta_val);
+        const double Ch_i = sm_pX(s, ki, zz, eta_val), Ch_m = sm_pX(s, km, zz, eta_val);
+        for (int br = 0; br < 2; ++br) {
+            if (!(s->pflag[e] & (1 << br))) continue;
+            const double sg = br == 0 ? 1.0 : -1.0;                     /* k2 -> -k2, omega2 -> -omega2 */
+            const double ph = br == 0 ? phi_i + phi_m : phi_i - phi_m;
+            const double H = br == 0 ? s->Hp[e] : s->Hm[e];
+            const double knm = br == 0 ? s->kp[e] : s->km[e];
+            if (comp == 0) {                                            /* :1718-1746 */
+                const double kx12 = br == 0 ? ki * s->cth[i] + km * s->cth[m] : ki * s->cth[i] - km * s->cth[m];
+                const double Ch12 = sm_pX(s, knm, zz, eta_val);
+                v2 += AA * H * kx12 * (-cos(ph)) * Ch12;
+                const double a1 = br == 0 ? ki * Sh_i / oi + km * Sh_m / om : ki * Sh_i / oi - km * Sh_m / om;
+                v2 += AA * kx12 * g1s * a1 * cos(ph);
+                const double Sh12 = sm_pZ(s, knm, zz, eta_val);
+                dphi2ds += -AA * H * (knm * depth * Sh12) * sin(ph);
+                const double a2 = br == 0 ? ki * (ki * depth * Ch_i) / oi + km * (km * depth * Ch_m) / om
+                                          : ki * (ki * depth * Ch_i) / oi - km * (km * depth * Ch_m) / om;
+                dphi2ds += AA * g * (a1 + sp1 * a2) * sin(ph);
+            } else if (comp == 1) {                                     /* :1832-1844 */
+                const double ky12 = br == 0 ? ki * s->sth[i] + km * s->sth[m] : ki * s->sth[i] - km * s->sth[m];
+                const double Ch12 = sm_pX(s, knm, zz, eta_val);
+                v2 += AA * H * ky12 * (-cos(ph)) * Ch12;
+            } else if (comp == 2) {                                     /* :1911-1921 */
+                const double Sh12 = sm_pZ(s, knm, zz, eta_val);
+                v2 += AA * H * knm * (-sin(ph)) * Sh12;
+            } else {                                                    /* :1986-2021 */
+                const double om12 = oi + sg * om;
+                const double Ch12 = sm_pX(s, knm, zz, eta_val);
+                v2 += AA * rho * om12 * H * (-cos(ph)) * Ch12;
+                const double k1dk2 = ki * km * s->gam[e];
+                const double br2 = br == 0 ? Ch_i * Ch_m - Sh_i * Sh_m : Ch_i * Ch_m + Sh_i * Sh_m;
+                v2 -= AA * rho * g * g * k1dk2 * br2 / (oi * om) * cos(ph);
+            }
+        }
+    }
+    if (comp == 0) {                                                    /* chain term, :1752-1769 */
+        double eta_x = 0.0, phi_s = 0.0;
+        for (int n = 0; n < N; ++n) {
+            const double phi_n = sm_phase(s, n, t, x, y);
+            const double f = fr(s, n, t);
+            const double sphi = sin(phi_n);
+            eta_x += -s->A[n] * s->k[n] * s->cth[n] * sphi * f;
+            phi_s += (g * s->A[n] / s->omega[n]) * s->k[n] * depth * sm_pZ(s, s->k[n], zz, eta_val) * sphi * f;
+        }
+        phi_s += dphi2ds;
+        const double oneplus_s_over_D = (depth + zz) / ((depth + eta_val) * (depth + eta_val));
+        v2 += -oneplus_s_over_D * eta_x * phi_s;
+    }
+    return v1 + v2;
+}
+
 static double eval1(const or_sea* s, int kind, double t, double x, double y, double z) {
     double o4[4];
     switch (kind) {
@@ -551,7 +711,12 @@ static double eval1(const or_sea* s, int kind, double t, double x, double y, dou
     case OR_V2: return direct2(s, 1, t, x, y, z);
     case OR_W2: return direct2(s, 2, t, x, y, z);
     case OR_DP2: return direct2(s, 3, t, x, y, z);
+    case OR_U_SMIT: return smit(s, 0, t, x, y, z);
+    case OR_V_SMIT: return smit(s, 1, t, x, y, z);
+    case OR_W_SMIT: return smit(s, 2, t, x, y, z);
+    case OR_DP_SMIT: return smit(s, 3, t, x, y, z);
     case OR_U: case OR_V: case OR_W: case OR_DP: {                   /* :24-157 */
+        if (s->vt == 1) return smit(s, kind - OR_U, t, x, y, z);    /* :25, :53, :77, :143 */
         if (s->order == 2) {
             const double z0 = dmin(0., z - s->swl), za = dmax(0., z - s->swl);
             switch (kind) {

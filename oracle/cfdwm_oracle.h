@@ -24,7 +24,8 @@ typedef struct or_grid or_grid;
 enum {
     OR_ETA1 = 0, OR_ETA2, OR_ETA, OR_U1, OR_V1, OR_W1, OR_DP1, OR_PHI1_DXDZ, OR_PHI1_DYDZ,
     OR_PHI1_DZDZ, OR_DP1_DZ, OR_U2, OR_V2, OR_W2, OR_DP2, OR_U, OR_V, OR_W, OR_DP,
-    OR_UVW2_U, OR_UVW2_V, OR_UVW2_W, OR_UVW2_P
+    OR_UVW2_U, OR_UVW2_V, OR_UVW2_W, OR_UVW2_P,
+    OR_U_SMIT, OR_V_SMIT, OR_W_SMIT, OR_DP_SMIT
 };
 
 /* Create a sea state from component arrays (already sorted by omega, as the
@@ -42,6 +43,11 @@ void or_sea_destroy(or_sea* s);
 void or_sea_components(const or_sea* s, double* A, double* phase, double* bandwidth);
 double or_sea_kmax(const or_sea* s);
 long or_pair_count(const or_sea* s);
+/* vertical_theory (Irregular.h:83): 1 routes OR_U/OR_V/OR_W/OR_DP to the Smit
+ * et al. s-coordinate members (Irregular.cpp:25, :53, :77, :143). */
+void or_sea_set_vertical_theory(or_sea* s, int vt);
+/* Smit H_nm_plus, H_nm_minus of each pair-cache entry (2 per entry). */
+void or_pair_export_h(const or_sea* s, double* h2);
 void or_pair_export(const or_sea* s, int* im, unsigned char* flags, double* vals7);
 
 void or_eval(const or_sea* s, int kind, long n, const double* t, const double* x,

@@ -94,7 +94,7 @@ double refh_freq_ramp(double t, double rt, double ts, double du) { return g_irr.
 // Point evaluation of one Irregular member over n points.
 // kind: 0 eta1, 1 eta2, 2 eta, 3 u1, 4 v1, 5 w1, 6 dp1, 7 phi1_dxdz, 8 phi1_dydz,
 //       9 phi1_dzdz, 10 dp1_dz, 11 u2, 12 v2, 13 w2, 14 dp2, 15 u, 16 v, 17 w, 18 dp,
-//       19..22 uvw2 (u, v, w, p)
+//       19..22 uvw2 (u, v, w, p), 23..26 u_smit, v_smit, w_smit, dp_smit
 void refh_eval(int kind, long n, const double* t, const double* x, const double* y,
                const double* z, double* out)
 {
@@ -127,6 +127,10 @@ void refh_eval(int kind, long n, const double* t, const double* x, const double*
             r = kind == 19 ? q.u : kind == 20 ? q.v : kind == 21 ? q.w : q.p;
             break;
         }
+        case 23: r = g_irr.u_smit(tt, xx, yy, zz); break;
+        case 24: r = g_irr.v_smit(tt, xx, yy, zz); break;
+        case 25: r = g_irr.w_smit(tt, xx, yy, zz); break;
+        case 26: r = g_irr.dp_smit(tt, xx, yy, zz); break;
         default: r = 0.0;
         }
         out[p] = r;
@@ -146,6 +150,19 @@ void refh_pair_export(int* im, unsigned char* flags, double* vals /* 7 per entry
     }
 }
 double refh_kmax() { return g_irr.kmax_cache; }
+// vertical_theory as the parser sets it (src/CFDwavemaker.cpp:571-585).  The
+// Smit members build the second-order caches themselves when absent; build
+// them here, before any parallel evaluation, so no two threads do it at once.
+void refh_set_vertical_theory(int vt) {
+    g_irr.vertical_theory = vt;
+    if ((int)g_irr.cth_cache.size() != g_irr.nfreq * g_irr.ndir) g_irr.rebuild_second_order_cache();
+}
+void refh_pair_export_h(double* h2) {
+    for (size_t e = 0; e < g_irr.pair_cache.size(); ++e) {
+        h2[2 * e] = g_irr.pair_cache[e].H_nm_plus;
+        h2[2 * e + 1] = g_irr.pair_cache[e].H_nm_minus;
+    }
+}
 
 // lsGridSpline set-up as in process_inputdata (src/CFDwavemaker.cpp:1180-1196,
 // 1426-1437).  Returns the wall time of initialize_kinematics[_with_ignore].

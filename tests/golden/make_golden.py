@@ -9,6 +9,8 @@ Outputs (small npz fixtures, committed):
               duplicates (k_i == k_m), bandwidth/cutoff filters, individual
               ramps, shallow and deep water;
   pairs.npz   the reference pair cache (Irregular::rebuild_pair_cache) of those seas;
+  smit.npz    the Smit et al. members u_smit/v_smit/w_smit/dp_smit of those seas
+              (orders 2 and 1) and their pair-cache H coefficients;
   grid.npz    lsGridSpline fields after initialize_kinematics and three
               wave_force_update steps (plain and ignore_at_init), plus spline
               queries at each step;
@@ -58,6 +60,24 @@ def make_pairs(R):
         out[f"{name}__im"], out[f"{name}__flags"], out[f"{name}__vals"] = im, fl, v
         out[f"{name}__kmax"] = np.float64(R.kmax())
     np.savez_compressed(os.path.join(HERE, "pairs.npz"), **out)
+
+
+def make_smit(R):
+    """Smit et al. s-coordinate members (vertical_theory smit) of the same seas,
+    orders 2 and 1, at the same points, with the pair cache's H coefficients."""
+    out = {}
+    for name, sea in sea_cases().items():
+        for order in (2, 1):
+            sd = dict(sea, order=order)
+            R.setup(sd)
+            R.set_vertical_theory(1)
+            t, x, y, z = points_for(sd, seed=ord(name))
+            key = f"{name}{order}"
+            out[f"{key}__pts"] = np.stack([t, x, y, z])
+            out[f"{key}__H"] = R.pairs_h()
+            for kind in ("u_smit", "v_smit", "w_smit", "dp_smit"):
+                out[f"{key}__{kind}"] = R.eval(kind, t, x, y, z)
+    np.savez_compressed(os.path.join(HERE, "smit.npz"), **out)
 
 
 def make_grid(R):
@@ -172,6 +192,7 @@ def main():
     R = RefHarness()
     make_points(R)
     make_pairs(R)
+    make_smit(R)
     make_grid(R)
     make_abi()
     print("golden fixtures written to", HERE)
