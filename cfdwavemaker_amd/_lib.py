@@ -36,7 +36,7 @@ BATCH_API = [
     "cfdwm_mlp_set_params", "cfdwm_mlp_set_norm", "cfdwm_mlp_get_norm", "cfdwm_mlp_forward_device",
     "cfdwm_mlp_loss_grad_device", "cfdwm_mlp_train_device", "cfdwm_surrogate_linear", "cfdwm_surrogate_add",
     "cfdwm_grids_refill_batched", "cfdwm_grid_release_workspace", "wave_PrefetchPoints", "wave_PrefetchClear",
-    "wave_PrefetchStats", "cfdwm_eval_fields", "cfdwm_grid_write_vtu",
+    "wave_PrefetchStats", "cfdwm_eval_fields", "cfdwm_grid_write_vtu", "cfdwm_sea_set_vertical_theory",
 ]
 
 
@@ -113,6 +113,8 @@ def load(path: str | None = None) -> C.CDLL:
     L.cfdwm_sea_create.restype = C.c_int
     L.cfdwm_sea_create.argtypes = [C.POINTER(C.c_void_p), C.c_int] + [_dp] * 8 + [C.POINTER(SeaParams)]
     L.cfdwm_sea_destroy.argtypes = [C.c_void_p]
+    L.cfdwm_sea_set_vertical_theory.restype = C.c_int
+    L.cfdwm_sea_set_vertical_theory.argtypes = [C.c_void_p, C.c_int]
     L.cfdwm_sea_terms.argtypes = [C.c_void_p] + [C.POINTER(_i64)] * 3
     L.cfdwm_eval.restype = C.c_int
     L.cfdwm_eval.argtypes = [C.c_void_p, C.c_int, _i64] + [_dp] * 5

@@ -194,8 +194,8 @@ void parse(const std::string& res) {
                     read_after_key(lineA, v);
                     G.vertical_theory = !v.compare(0, 4, "smit") ? 1 : 0;
                     if (G.vertical_theory == 1)
-                        std::cerr << "cfdwm: vertical_theory 'smit' (experimental in the reference) is not "
-                                     "supported; Sharma & Dean kinematics are used" << std::endl;
+                        std::cout << "Smit et al. (2017) s-coordinate vertical theory selected (experimental in "
+                                     "the reference: off-diagonal second-order terms not yet validated)." << std::endl;
                 }
                 if (starts(lineA, "nfreq")) break;
             }
@@ -351,6 +351,7 @@ int setup_after_parse() {
     int rc = cfdwm_sea_create(&G.csea, n, S.omega.data(), S.A.data(), S.k.data(), S.phase.data(), S.theta.data(),
                               S.ramptime.data(), S.starttime.data(), S.duration.data(), &p);
     if (rc) gpu_fatal("sea set-up");
+    G.csea->s->vertical_theory = G.vertical_theory;        // Irregular::u/v/w/dp dispatch
     if (G.wavetype == 5) {
         if (G.lsg_ny < 1 || G.lsg_nx < 1 || G.lsg_nl < 1)
             input_error("lsgrid needs nx, ny and nl of at least 1");
@@ -796,6 +797,12 @@ int cfdwm_sea_create(cfdwm_sea** out, int n, const double* omega, const double* 
 
 int cfdwm_sea_destroy(cfdwm_sea* s) { delete s; return 0; }
 
+int cfdwm_sea_set_vertical_theory(cfdwm_sea* s, int vt) {
+    if (!s || (vt != 0 && vt != 1)) { set_error("bad arguments"); return -1; }
+    s->s->vertical_theory = vt;
+    return 0;
+}
+
 int cfdwm_sea_terms(cfdwm_sea* s, int64_t* pair_terms, int64_t* diag_terms, int64_t* direct_terms) {
     if (!s) return -1;
     if (pair_terms) *pair_terms = s->s->tab.filt.nterms;
@@ -839,11 +846,20 @@ int cfdwm_eval(cfdwm_sea* cs, int kind, int64_t n, const double* t, const double
     case 15: case 16: case 17: fl = order_flags(order); comp = kind - 15; break;
     case 18: fl = order_flags(order) | EV_P; comp = 3; break;
     case 19: case 20: case 21: fl = EV_SEC; comp = kind - 19; break;
+    case 23: case 24: case 25: case 26: {                  // u_smit, v_smit, w_smit, dp_smit
+        double* so[4] = {nullptr, nullptr, nullptr, nullptr};
+        so[kind - 23] = out;
+        return s.eval_points(n, x, y, z, t, 0, false, nullptr, nullptr, nullptr, so[0], so[1], so[2], so[3],
+                             false, true);
+    }
     default:
         set_error("evaluation kind not supported (uvw2.p is read by no API path of the reference)");
         return -1;
     }
-    if ((fl & EV_DIRECT) && !cs->with_direct) { set_error("sea created without the unfiltered table"); return -1; }
+    if ((fl & EV_DIRECT) && !cs->with_direct && s.vertical_theory != 1) {
+        set_error("sea created without the unfiltered table");
+        return -1;
+    }
     double* o[4] = {nullptr, nullptr, nullptr, nullptr};
     o[comp] = out;
     return s.eval_points(n, x, y, z, t, fl, false, nullptr, nullptr, nullptr, o[0], o[1], o[2], o[3], false);
@@ -856,7 +872,10 @@ int cfdwm_eval_fields(cfdwm_sea* cs, int64_t n, const double* t, const double* x
     const int order = s.tab.in.order;
     const bool lev = u || v || w || p;
     if (lev && !z) { set_error("z required"); return -1; }
-    if (lev && order >= 1 && !cs->with_direct) { set_error("sea created without the unfiltered table"); return -1; }
+    if (lev && order >= 1 && !cs->with_direct && s.vertical_theory != 1) {
+        set_error("sea created without the unfiltered table");
+        return -1;
+    }
     const uint32_t fl = order_flags(order) | (p ? EV_P : 0u);
     return s.eval_points(n, x, y, z, t, fl, order == 2 && eta2 != nullptr, eta1, order == 2 ? eta2 : nullptr, nullptr,
                          u, v, w, p, device != 0);

@@ -314,6 +314,20 @@ const Sea::QfTables* Sea::qform_tables() {
 
 void Sea::release_qform() { qf_.reset(); }
 
+const Sea::SmitDev* Sea::smit_tables() {
+    if (smit_) return smit_.get();
+    std::vector<SmitComp> comp;
+    std::vector<SmitPair> pair;
+    build_smit(tab, comp, pair);
+    auto T = std::make_unique<SmitDev>();
+    hipStream_t st = Engine::get().stream();
+    if (T->comp.upload(comp.data(), std::max<size_t>(1, comp.size()) * sizeof(SmitComp), st)) return nullptr;
+    if (T->pair.upload(pair.data(), std::max<size_t>(1, pair.size()) * sizeof(SmitPair), st)) return nullptr;
+    if (hipStreamSynchronize(st) != hipSuccess) return nullptr;
+    smit_ = std::move(T);
+    return smit_.get();
+}
+
 namespace {
 // Segments so that about 256 CUs x 4 SIMDs x 4 waves of threads are busy,
 // with at least 64 pair records per segment.
@@ -425,17 +439,45 @@ int run_points(Sea& s, int64_t n, const double* dx, const double* dy, const doub
     }
     return 0;
 }
+
+// Smit et al. u, v, w, p at points (smit.hip); o[0..n) and o[n..2n) hold the
+// points' eta1 and eta2, the outputs go to o[6n..10n).
+template <typename R>
+int run_smit(Sea& s, int64_t n, const double* dx, const double* dy, const double* dz, const double* dt, double* o) {
+    hipStream_t st = Engine::get().stream();
+    const Sea::SmitDev* T = s.smit_tables();
+    if (!T) { set_error("Smit coefficient tables: device allocation failed"); return -3; }
+    const SeaDev<R>& S = s.dev<R>();
+    const SeaInput& in = s.tab.in;
+    SmitJob J;
+    J.n = n; J.x = dx; J.y = dy; J.z = dz; J.t = dt;
+    J.eta1 = o; J.eta2 = o + n;
+    J.order = in.order; J.N = S.n;
+    J.h = in.depth; J.g = in.g; J.rho = in.rho; J.swl = in.swl; J.x0 = in.x0; J.y0 = in.y0;
+    J.ph = S.ph; J.frow = S.frow; J.fmhi = S.fmhi;
+    J.sc = (const SmitComp*)T->comp.get();
+    J.sp = (const SmitPair*)T->pair.get();
+    J.nseg = in.order >= 1 ? pick_nseg(n, s.table_records(false)) : 1;
+    J.seg = s.segments(false, J.nseg);
+    if (!J.seg) return -3;
+    if (s.partial.alloc((size_t)J.nseg * 5 * n * sizeof(double))) return -3;
+    J.part = (double*)s.partial.get();
+    J.o_u = o + 6 * n; J.o_v = o + 7 * n; J.o_w = o + 8 * n; J.o_p = o + 9 * n;
+    PROF("k_smit", launch_smit<R>(J, st));
+    return 0;
+}
 }  // namespace
 
 int Sea::eval_points(int64_t n, const double* x, const double* y, const double* z, const double* t,
                      uint32_t flags, bool want_eta2, double* eta1, double* eta2, double* grads4,
-                     double* u, double* v, double* w, double* p, bool device_ptrs) {
+                     double* u, double* v, double* w, double* p, bool device_ptrs, bool smit_always) {
     if (n <= 0) return 0;
     std::lock_guard<std::mutex> lk(mu);
     hipStream_t st = Engine::get().stream();
     const bool want_cols = eta1 || eta2 || grads4;
     const bool want_lev = u || v || w || p;
-    if (want_lev && (flags & EV_DIRECT) && tab.direct.rowoff.empty()) {
+    const bool smit = (vertical_theory == 1 || smit_always) && want_lev;
+    if (want_lev && !smit && (flags & EV_DIRECT) && tab.direct.rowoff.empty()) {
         set_error("the unfiltered (direct) pair table was not built for this sea");
         return -1;
     }
@@ -452,8 +494,19 @@ int Sea::eval_points(int64_t n, const double* x, const double* y, const double* 
     }
     if (pts_out.alloc(10 * n * sizeof(double))) return -3;
     double* o = (double*)pts_out.get();
-    int rc = prec == kFP32 ? run_points<float>(*this, n, dx, dy, dz, dt, flags, want_cols, want_eta2, want_lev, o)
+    int rc;
+    if (smit) {
+        // the stretch height eta1 + eta2 of every point, then the Smit kernels
+        const bool e2 = tab.in.order >= 1;
+        rc = prec == kFP32 ? run_points<float>(*this, n, dx, dy, dz, dt, 0, true, e2, false, o)
+                           : run_points<double>(*this, n, dx, dy, dz, dt, 0, true, e2, false, o);
+        if (!rc)
+            rc = prec == kFP32 ? run_smit<float>(*this, n, dx, dy, dz, dt, o)
+                               : run_smit<double>(*this, n, dx, dy, dz, dt, o);
+    } else {
+        rc = prec == kFP32 ? run_points<float>(*this, n, dx, dy, dz, dt, flags, want_cols, want_eta2, want_lev, o)
                            : run_points<double>(*this, n, dx, dy, dz, dt, flags, want_cols, want_eta2, want_lev, o);
+    }
     if (rc) return rc;
     const hipMemcpyKind kind = device_ptrs ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost;
     auto get = [&](double* dst, int slot) -> int {

@@ -89,6 +89,8 @@ template <typename R>
 hipError_t launch_columns(const SeaDev<R>& S, const ColumnJob<R>& J, bool grid, hipStream_t st);
 template <typename R>
 hipError_t launch_levels(const SeaDev<R>& S, const LevelJob<R>& J, bool grid, int lc, hipStream_t st);
+template <typename R>
+hipError_t launch_smit(const SmitJob& J, hipStream_t st);
 // thread counts of one launch (partial buffers hold nseg x threads records)
 int64_t column_threads(int64_t nslot, int64_t ncol);
 int64_t level_threads(int64_t nslot, int64_t ncol, int nl, int lc, bool grid);
@@ -221,6 +223,13 @@ public:
     };
     const QfTables* qform_tables();          // FP32 seas only
     void release_qform();
+    // vertical_theory (Irregular.h:83): 1 = Smit et al. s-coordinate
+    // kinematics for u, v, w, p at points (elevation stays Sharma-Dean)
+    int vertical_theory = 0;
+    struct SmitDev {
+        DevBuf comp, pair;                   // SmitComp[n], SmitPair[filtered records]
+    };
+    const SmitDev* smit_tables();            // built on first use
     DevBuf partial;                          // split-K partial sums of point batches
     DevBuf qz, qo;                           // quadratic-form operands / values of point batches
     std::mutex mu;                           // serialises the per-point scratch use
@@ -228,11 +237,12 @@ public:
     DevBuf pts_in, pts_out;
     int eval_points(int64_t n, const double* x, const double* y, const double* z, const double* t,
                     uint32_t flags, bool want_eta2, double* eta1, double* eta2, double* grads4,
-                    double* u, double* v, double* w, double* p, bool device_ptrs);
+                    double* u, double* v, double* w, double* p, bool device_ptrs, bool smit_always = false);
 private:
     DevBuf ph_, cr_, act_, frow_, fmhi_, fuvw_, fx_, drow_, duvw_, dx_;
     std::map<std::pair<bool, int>, std::unique_ptr<DevBuf>> seg_cache_;
     std::unique_ptr<QfTables> qf_;
+    std::unique_ptr<SmitDev> smit_;
     SeaDev<double> d64_{};
     SeaDev<float> d32_{};
     template <typename R>

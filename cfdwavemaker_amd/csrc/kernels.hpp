@@ -2,6 +2,8 @@
 #pragma once
 #include <cstdint>
 
+#include "sea_tables.hpp"
+
 namespace cfdwm {
 
 // Per-component phase data, FP64 in both precisions: the phase
@@ -119,6 +121,27 @@ struct QfBuild {
     const double* tau = nullptr;          // [R] skeleton depths
     const double* einv = nullptr;         // [R][R]
     uint16_t* b = nullptr;                // [nform][4][npad x npad], zeroed by the caller
+};
+
+// Smit et al. s-coordinate point evaluation (smit.hip).  eta1/eta2 are the
+// points' Sharma-Dean elevations (eta2 unused for order < 1); sc and sp are
+// the Smit coefficients of the components and of the filtered pair table
+// (frow/fmhi layout), split into nseg balanced segments; part holds
+// [nseg][5][n] partial sums.
+struct SmitJob {
+    int64_t n = 0;
+    const double *x = nullptr, *y = nullptr, *z = nullptr, *t = nullptr;
+    const double *eta1 = nullptr, *eta2 = nullptr;
+    int order = 2, N = 0, nseg = 1;
+    double h = 0, g = 9.81, rho = 1025, swl = 0, x0 = 0, y0 = 0;
+    const PhaseRec* ph = nullptr;
+    const SmitComp* sc = nullptr;
+    const int64_t* frow = nullptr;
+    const int32_t* fmhi = nullptr;
+    const SmitPair* sp = nullptr;
+    const Segment* seg = nullptr;
+    double* part = nullptr;
+    double *o_u = nullptr, *o_v = nullptr, *o_w = nullptr, *o_p = nullptr;
 };
 
 }  // namespace cfdwm

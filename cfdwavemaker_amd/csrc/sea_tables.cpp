@@ -310,4 +310,60 @@ double mean_length(const SeaInput& in, int opt) {               // :1397-1415
     return 0.;
 }
 
+void build_smit(const SeaTables& t, std::vector<SmitComp>& comp, std::vector<SmitPair>& pair) {
+    const SeaInput& in = t.in;
+    const int N = (int)in.omega.size();
+    const double h = in.depth, g = in.g;
+    std::vector<double> Rn(N), Rs(N);
+    comp.assign(N, SmitComp{});
+    for (int i = 0; i < N; ++i) {
+        const double k = in.k[i], w = in.omega[i];
+        Rn[i] = k * std::tanh(k * h);                              // Irregular.cpp:826-828
+        Rs[i] = std::sqrt(Rn[i]);
+        const double D_nn = (6.0 * Rs[i] * Rs[i] * (k * k - Rn[i] * Rn[i])) /
+                            (2.0 * Rs[i] * Rs[i] - k * std::tanh(2.0 * k * h));
+        const double alpha_nn = (D_nn - (k * k - Rn[i] * Rn[i])) / Rn[i] + 2.0 * Rn[i];
+        const double om2 = 2.0 * w;
+        comp[i].A = in.A[i];
+        comp[i].Ck = cfac(k, h);
+        comp[i].C2k = cfac(2.0 * k, h);
+        comp[i].Hd = -g / om2 * (alpha_nn / 2.0 - (3.0 * w * w) / (2.0 * g) + g * k * k / (2.0 * w * w));
+    }
+    const PairTable& P = t.filt;
+    pair.assign(P.rec.size(), SmitPair{});
+    const CompTable& c = t.comp;
+    parallel_rows(N, [&](int i) {
+        if (in.omega[i] > in.cutoff) return;
+        const double ki = in.k[i], oi = in.omega[i];
+        for (int m = i + 1; m < P.mhi[i]; ++m) {
+            if (in.omega[m] > in.cutoff) continue;
+            if (std::abs(oi - in.omega[m]) > in.bandwidth) continue;
+            const double km = in.k[m], om = in.omega[m];
+            const double gamma = c.cth[i] * c.cth[m] + c.sth[i] * c.sth[m];
+            double kp, kn, Dp, Dm;
+            cached_D(ki, km, Rn[i], Rn[m], Rs[i], Rs[m], gamma, h, kp, kn, Dp, Dm);
+            SmitPair& r = pair[P.rowoff[i] + (m - i - 1)];
+            const double k1dk2 = ki * km * gamma;
+            r.k1dk2 = k1dk2;
+            r.kp = kp;
+            r.km = kn;
+            r.CkP = cfac(kp, h);
+            r.CkM = cfac(kn, h);
+            if ((ki + km) <= t.kmax) {
+                const double ap = (Dp - (ki * km * gamma - Rn[i] * Rn[m])) / std::sqrt(Rn[i] * Rn[m]) + (Rn[i] + Rn[m]);
+                const double om12 = oi + om;
+                r.Hp = -g / om12 * (ap / 2.0 - (oi * oi + om * om + oi * om) / (2.0 * g) + g * k1dk2 / (2.0 * oi * om));
+                r.fP = 1.0;
+            }
+            if (ki != km) {
+                const double am = (Dm - (ki * km * gamma + Rn[i] * Rn[m])) / std::sqrt(Rn[i] * Rn[m]) + (Rn[i] + Rn[m]);
+                const double om12 = oi - om;
+                if (std::abs(om12) > kSmall)
+                    r.Hm = -g / om12 * (am / 2.0 - (oi * oi + om * om - oi * om) / (2.0 * g) + g * k1dk2 / (2.0 * oi * om));
+                r.fM = 1.0;
+            }
+        }
+    });
+}
+
 }  // namespace cfdwm

@@ -81,6 +81,22 @@ struct SeaTables {
 // Builds all tables.  Returns an empty string on success, else an error text.
 std::string build_tables(const SeaInput& in, SeaTables& out, bool with_direct);
 
+// Smit et al. (2017) s-coordinate coefficients (vertical_theory smit,
+// src/Irregular.cpp:1623-2027).  Per component: A, the profile factors
+// C_k = 1/(1+e^{-2kh}) and C_2k, and the diagonal H from the Sharma-Dean
+// alpha_nn through Eq. 10 (:1672-1678).  Per record of the filtered pair
+// table (same row layout): H of each branch (:919-927, :950-958), the
+// combined wavenumbers and their profile factors, k_i k_m gamma, and 1/0
+// branch-present flags (the branch terms not scaled by H, the P-derivative
+// and the kinetic pressure, are gated by them).
+struct SmitComp {
+    double A, Ck, C2k, Hd;
+};
+struct SmitPair {
+    double Hp, Hm, kp, km, CkP, CkM, k1dk2, fP, fM, pad;
+};
+void build_smit(const SeaTables& t, std::vector<SmitComp>& comp, std::vector<SmitPair>& pair);
+
 // Spectral helpers (Irregular.cpp:1336-1453)
 void spectral_moments(const SeaInput& in, double m[3]);
 double mean_period(const SeaInput& in, int opt);

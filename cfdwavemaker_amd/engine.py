@@ -18,6 +18,8 @@ KINDS = ["eta1", "eta2", "eta", "u1", "v1", "w1", "dp1", "phi1_dxdz", "phi1_dydz
          "phi1_dzdz", "dp1_dz", "u2", "v2", "w2", "dp2", "u", "v", "w", "dp",
          "uvw2_u", "uvw2_v", "uvw2_w"]
 KIND = {k: i for i, k in enumerate(KINDS)}
+# the Smit et al. s-coordinate members (cfdwm_eval kinds 23-26)
+KIND.update({"u_smit": 23, "v_smit": 24, "w_smit": 25, "dp_smit": 26})
 GRID_FIELDS = ["ETA", "ETAdx", "ETAdy", "ETAdt", "UX", "UY", "UZ", "Udt", "Vdt", "Wdt",
                "Udx", "Udy", "Uds", "Vdx", "Vdy", "Vds", "Wdx", "Wdy", "Wds", "IGNORE"]
 FP64, FP32 = 0, 1
@@ -66,6 +68,11 @@ class Sea:
             self.close()
         except Exception:
             pass
+
+    def set_vertical_theory(self, vt: int):
+        """1: u, v, w, p of every point entry follow the Smit et al. members
+        (vertical_theory smit); 0: Sharma & Dean."""
+        check(self.L.cfdwm_sea_set_vertical_theory(self.h, int(vt)), "cfdwm_sea_set_vertical_theory")
 
     def terms(self):
         a, b, c = C.c_int64(), C.c_int64(), C.c_int64()

@@ -152,7 +152,7 @@ def config_sea(cfg: int, state: int = 0) -> SeaState:
 # ------------------------------------------------------------ waveinput.dat
 def write_waveinput(path, sea: SeaState, lsgrid: dict | None = None, ramps: dict | None = None,
                     extrap: int | None = None, bandwidth=None, cutoff=None, normalize=None,
-                    amplify=None, individual_ramp=False, version="@v310") -> None:
+                    amplify=None, individual_ramp=False, vertical_theory=None, version="@v310") -> None:
     """Write a ``waveinput.dat`` that the reference parser accepts.
 
     Components are written one row per component with ndir 0 (already expanded
@@ -182,6 +182,8 @@ def write_waveinput(path, sea: SeaState, lsgrid: dict | None = None, ramps: dict
         L.append(f"cutoff {cutoff!r}")
     if individual_ramp:
         L.append("individual_ramp true")
+    if vertical_theory is not None:
+        L.append(f"vertical_theory {vertical_theory}")
     L += [f"nfreq {sea.n}", "ndir 0"]
     for i in range(sea.n):
         row = [sea.omega[i], sea.A[i], sea.k[i], sea.phase[i], sea.theta[i]]

@@ -138,6 +138,14 @@ CFDWM_EXPORT int cfdwm_sea_create(cfdwm_sea** out, int n, const double* omega, c
                                   const double* ramptime, const double* starttime,
                                   const double* duration, const cfdwm_sea_params* params);
 CFDWM_EXPORT int cfdwm_sea_destroy(cfdwm_sea* sea);
+/* vertical_theory of the sea (Irregular.h:83, waveinput.dat key
+ * "vertical_theory smit", src/CFDwavemaker.cpp:571-585): 1 routes the u, v, w
+ * and p of every point entry (wave_VeloX..wave_DynPres, wave_EvalPoints,
+ * cfdwm_eval kinds 15-18, cfdwm_eval_fields) to the Smit et al. (2017)
+ * s-coordinate members u_smit/v_smit/w_smit/dp_smit (Irregular.cpp:1623-2027);
+ * the elevation stays Sharma & Dean (Irregular::eta).  0 restores Sharma &
+ * Dean.  Returns 0, or -1 for a bad argument. */
+CFDWM_EXPORT int cfdwm_sea_set_vertical_theory(cfdwm_sea* sea, int vertical_theory);
 /* Number of second-order terms kept by the pair filters (plus+minus branches
  * of the pair cache) and of diagonal terms: the algorithmic work per point. */
 CFDWM_EXPORT int cfdwm_sea_terms(cfdwm_sea* sea, int64_t* pair_terms, int64_t* diag_terms,
@@ -146,7 +154,9 @@ CFDWM_EXPORT int cfdwm_sea_terms(cfdwm_sea* sea, int64_t* pair_terms, int64_t* d
 /* Evaluation kinds, one per reference Irregular member:
  *  0 eta1  1 eta2  2 eta  3 u1  4 v1  5 w1  6 dp1  7 phi1_dxdz  8 phi1_dydz  9 phi1_dzdz
  * 10 dp1_dz  11 u2  12 v2  13 w2  14 dp2  15 u  16 v  17 w  18 dp  19 uvw2.u  20 uvw2.v
- * 21 uvw2.w  (22 uvw2.p is computed by the reference but read by no API path: -1).
+ * 21 uvw2.w  (22 uvw2.p is computed by the reference but read by no API path: -1)
+ * 23 u_smit  24 v_smit  25 w_smit  26 dp_smit (Irregular.cpp:1623-2027, whatever the
+ * sea's vertical_theory).
  * Host arrays, n points; z ignored by the z-independent kinds. */
 CFDWM_EXPORT int cfdwm_eval(cfdwm_sea* sea, int kind, int64_t n, const double* t, const double* x,
                             const double* y, const double* z, double* out);

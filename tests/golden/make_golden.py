@@ -16,7 +16,8 @@ Outputs (small npz fixtures, committed):
               queries at each step;
   abi_*.npz   a waveinput.dat and a program of C-ABI calls (ref_abi.py) with
               the reference's answers: direct mode with ramps, s-grid mode
-              with updates, and the reference's t005 focused-wave input.
+              with updates, direct mode with vertical_theory smit, and the
+              reference's t005 focused-wave input.
 """
 import os
 import subprocess
@@ -171,7 +172,18 @@ def make_abi():
     prog = prog_grid(22, lsg, sea.depth)
     rc, out = run_abi(lib, text, prog)
     np.savez_compressed(os.path.join(HERE, "abi_grid.npz"), waveinput=text, rc=rc, out=out, **prog)
-    # 3. the reference's own t005 focused-wave input (data file of its test
+    # 3. direct mode with the Smit et al. vertical theory (experimental in the
+    #    reference): VeloX/Y/Z, DynPres and SurfElev through the dispatch
+    sea = make_sea(16, 2.5, 9.0, 25.0, seed=23, ndir=2)
+    sea.swl, sea.tofmax, sea.x0 = 0.2, 4.0, 5.0
+    with tempfile.NamedTemporaryFile("r", suffix=".dat") as tf:
+        write_waveinput(tf.name, sea, ramps={"time_rampup": (0.0, 3.0)}, bandwidth="0.8",
+                        vertical_theory="smit")
+        text = open(tf.name).read()
+    prog = prog_direct(23, n=40, depth=sea.depth)
+    rc, out = run_abi(lib, text, prog)
+    np.savez_compressed(os.path.join(HERE, "abi_smit.npz"), waveinput=text, rc=rc, out=out, **prog)
+    # 4. the reference's own t005 focused-wave input (data file of its test
     #    suite), sampled where its targetwave.dat was written (x=10, y=0)
     t005 = os.path.join(REF_SRC, "tests", "t005_focused_wave")
     text = open(os.path.join(t005, "waveinput.dat")).read()

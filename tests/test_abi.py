@@ -52,7 +52,7 @@ def run_program(fixture):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("fixture", ["abi_direct.npz", "abi_grid.npz", "abi_t005.npz"])
+@pytest.mark.parametrize("fixture", ["abi_direct.npz", "abi_grid.npz", "abi_smit.npz", "abi_t005.npz"])
 def test_reference_programs_through_abi(fixture):
     g, rc, out = run_program(fixture)
     assert rc == int(g["rc"]) == 0
