@@ -235,12 +235,13 @@ def batched_leg(nstates: int, nsample_err: int) -> dict:
             "per_state_err_vs_fp64": errs}
 
 
-def direct_leg(steps: int) -> dict:
+def direct_leg(steps: int, smit: bool = False) -> dict:
     """Direct (per-point semantics) workload: N=512 long-crested sea, 256 x on
     [0, 500] m x 16 t on [0, 15] s columns, 64 z per column on [-30, eta]:
     eta1 + eta2 per column (FP32: eta2 as the bf16 MFMA quadratic form) and
-    u, v, w, dp (unfiltered second-order sums) at the 262,144 points, FP32 and
-    FP64, errors of FP32 vs FP64."""
+    u, v, w, dp (unfiltered second-order sums; with smit, the Smit et al.
+    s-coordinate members, top level one spacing below the surface) at the
+    262,144 points, FP32 and FP64, errors of FP32 vs FP64."""
     import torch
     from cfdwavemaker_amd import engine
     from cfdwavemaker_amd.seastate import config_sea
@@ -256,12 +257,18 @@ def direct_leg(steps: int) -> dict:
     tc = torch.tensor(T, dtype=torch.float64, device=dev)
     res, fields = {}, {}
     for prec, name in ((engine.FP32, "fp32"), (engine.FP64, "fp64")):
-        es = engine.Sea(d, precision=prec, with_direct=True)
+        es = engine.Sea(d, precision=prec, with_direct=not smit)
+        if smit:
+            es.set_vertical_theory(1)
 
         def run():
             c = es.eval_fields_device(tc, xc, yc, want=("eta1", "eta2"))
             eta = c["eta1"] + c["eta2"]
-            frac = torch.linspace(0.0, 1.0, nz, dtype=torch.float64, device=dev)
+            # Smit velocities jump to 0 above eta + swl: keep the top level one
+            # spacing below the surface, so that FP32 and FP64 elevations do not
+            # put the same point on opposite sides of it
+            top = 1.0 - 1.0 / nz if smit else 1.0
+            frac = torch.linspace(0.0, top, nz, dtype=torch.float64, device=dev)
             z = (-30.0 + (eta[:, None] + 30.0) * frac[None, :]).reshape(-1)
             rep = lambda a: a[:, None].expand(-1, nz).reshape(-1).contiguous()
             f = es.eval_fields_device(rep(tc), rep(xc), rep(yc), z, want=("u", "v", "w", "p"))
@@ -282,8 +289,10 @@ def direct_leg(steps: int) -> dict:
     err = {"eta_max_err_over_hs": float(np.max(np.abs(e32 - e64)) / 6.0)}
     for k in ("u", "w", "p"):
         err[k + "_scaled_err"] = float(np.max(np.abs(f32[k] - f64[k])) / np.max(np.abs(f64[k])))
+    what = ("Smit et al. s-coordinate u, v, w, dp (vertical_theory smit)" if smit
+            else "u, v, w, dp (unfiltered second order)")
     return {"workload": "N=512 long-crested sea, 256 x 16 (x, t) columns x 64 z: eta1+eta2 per column, "
-                        "u, v, w, dp (unfiltered second order) per point", "points": nx * nt * nz,
+                        f"{what} per point", "points": nx * nt * nz,
             "fp32": res["fp32"], "fp64": res["fp64"], "fp32_vs_fp64": err,
             "eta2": "fp32: bf16 MFMA quadratic form (hi/lo split, FP32 accumulation)"}
 
@@ -416,6 +425,7 @@ def main():
     sur = None if a.no_surrogate else surrogate_leg(a.steps, a.warmup)
     bat = batched_leg(a.batched_states, 3) if a.batched_states > 0 else None
     dirl = None if a.no_direct else direct_leg(a.steps)
+    smitl = None if a.no_direct else direct_leg(a.steps, smit=True)
 
     cpu = None
     if not a.no_cpu:
@@ -455,6 +465,7 @@ def main():
         "surrogate": sur,
         "batched": bat,
         "direct": dirl,
+        "direct_smit": smitl,
     }
     print(json.dumps(line), flush=True)
 
