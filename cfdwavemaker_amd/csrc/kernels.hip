@@ -623,16 +623,6 @@ __device__ __forceinline__ R herm(R f0, R f1, R a, R b, R u) {
 template <typename R>
 __device__ __forceinline__ R lin(R f0, R f1, R u) { return f0 * ((R)1 - u) + f1 * u; }
 
-// node index of the 8 corners (i|i+1, j|j+1, l|l+1) with edge clamps (lsgrid_spline.cpp:375-384)
-__device__ __forceinline__ void corners(int64_t* a, int nx, int ny, int nl, int i, int j, int l) {
-    const int i1 = min(i + 1, nx - 1), j1 = min(j + 1, ny - 1), l1 = min(l + 1, nl - 1);
-    const int64_t I0 = (int64_t)i * ny, I1 = (int64_t)i1 * ny;
-    a[0] = (I0 + j) * nl + l;  a[1] = (I0 + j1) * nl + l;
-    a[2] = (I1 + j) * nl + l;  a[3] = (I1 + j1) * nl + l;
-    a[4] = (I0 + j) * nl + l1; a[5] = (I0 + j1) * nl + l1;
-    a[6] = (I1 + j) * nl + l1; a[7] = (I1 + j1) * nl + l1;
-}
-
 // cart2sigmaS (lsgrid_spline.cpp:107-112).  Points in (eta - swl, eta] give
 // sigma > 0 when swl > 0, where the reference indexes with a NaN; they are
 // mapped to the surface here.
@@ -645,55 +635,81 @@ __device__ __forceinline__ double cart2sigmaS(double zpt, double e, double h, do
     return 1 - atan(pow(-sigma * tanab, 1. / tb)) / ta;
 }
 
-// spline_interp_velo (lsgrid_spline.cpp:386-502) for field f of the packed records
+// A packed record of N values (N a multiple of 4) at a 16-byte aligned
+// address, loaded as 16-byte vectors.  Node records hold 16 values (3 fields x
+// {value, d/dt, d/dx, d/dy, d/ds} + pad), column surface records 8.
+template <int N, typename R>
+__device__ __forceinline__ void load_rec(R (&r)[N], const R* __restrict__ p) {
+    if constexpr (sizeof(R) == 4) {
+        const float4* q = reinterpret_cast<const float4*>(p);
+#pragma unroll
+        for (int k = 0; k < N / 4; ++k) {
+            const float4 v = q[k];
+            r[4 * k] = v.x; r[4 * k + 1] = v.y; r[4 * k + 2] = v.z; r[4 * k + 3] = v.w;
+        }
+    } else {
+        const double2* q = reinterpret_cast<const double2*>(p);
+#pragma unroll
+        for (int k = 0; k < N / 2; ++k) {
+            const double2 v = q[k];
+            r[2 * k] = v.x; r[2 * k + 1] = v.y;
+        }
+    }
+}
+
+// x stage of spline_interp_velo (lsgrid_spline.cpp:386-502) for the three
+// fields of the node pair (lo at i, hi at i+1): Hermite value, and linear
+// d/dy, d/dt, d/ds.
 template <typename R>
-__device__ R spline_velo(const R* __restrict__ pC, const R* __restrict__ pD, int nx, int ny, int nl,
-                         int i, int j, int l0, int l1, int f, R xd, R yd, R s0, R s1, R td,
-                         R dx, R dy, R ds, R dt) {
-    int64_t A[8], B[8];
-    corners(A, nx, ny, nl, i, j, l0);
-    corners(B, nx, ny, nl, i, j, l1);
-    R C[8], D[8], Ct[8], Dt[8], Cx[8], Dx[8], Cy[8], Dy[8], Cs[8], Ds[8];
+__device__ __forceinline__ void xstage(R (&xr)[3], R (&yr)[3], R (&tr)[3], R (&sr)[3],
+                                       const R* __restrict__ plo, const R* __restrict__ phi, R xd, R dx) {
+    R a[16], b[16];
+    load_rec(a, plo);
+    load_rec(b, phi);
 #pragma unroll
-    for (int q = 0; q < 8; ++q) {
-        const R* a = pC + 16 * A[q] + 5 * f;
-        const R* b = pD + 16 * B[q] + 5 * f;
-        C[q] = a[0]; Ct[q] = a[1]; Cx[q] = a[2]; Cy[q] = a[3]; Cs[q] = a[4];
-        D[q] = b[0]; Dt[q] = b[1]; Dx[q] = b[2]; Dy[q] = b[3]; Ds[q] = b[4];
+    for (int f = 0; f < 3; ++f) {
+        const int o = 5 * f;
+        const R ac = dx * a[o + 2] - (b[o] - a[o]);
+        const R bc = -dx * b[o + 2] + (b[o] - a[o]);
+        xr[f] = herm(a[o], b[o], ac, bc, xd);
+        yr[f] = lin(a[o + 3], b[o + 3], xd);
+        tr[f] = lin(a[o + 1], b[o + 1], xd);
+        sr[f] = lin(a[o + 4], b[o + 4], xd);
     }
-    const int lo[4] = {0, 1, 4, 5}, hi[4] = {2, 3, 6, 7};
-    R Cxr[4], Dxr[4], Cyr[4], Dyr[4], Ctr[4], Dtr[4], Csr[4], Dsr[4];
+}
+
+// x, y and s stages of spline_interp_velo (lsgrid_spline.cpp:386-502) on one
+// time slot's records for all three fields: value v and d/dt vt at (xd, yd, sd)
+// inside the cell of columns col = {(i,j), (i,j+1), (i+1,j), (i+1,j+1)} between
+// levels l and l+1, both edge clamped (lsgrid_spline.cpp:375-384).  Each record
+// is read once.
+template <typename R>
+__device__ __forceinline__ void slot_velo(R (&v)[3], R (&vt)[3], const R* __restrict__ pk, const int64_t (&col)[4],
+                                          int nl, int l, R xd, R yd, R sd, R dx, R dy, R ds) {
+    const int lp = min(l + 1, nl - 1);
+    R yv[2][3], sv[2][3], tv[2][3];
 #pragma unroll
-    for (int q = 0; q < 4; ++q) {
-        const R ac = dx * Cx[lo[q]] - (C[hi[q]] - C[lo[q]]);
-        const R bc = -dx * Cx[hi[q]] + (C[hi[q]] - C[lo[q]]);
-        const R ad = dx * Dx[lo[q]] - (D[hi[q]] - D[lo[q]]);
-        const R bd = -dx * Dx[hi[q]] + (D[hi[q]] - D[lo[q]]);
-        Cxr[q] = herm(C[lo[q]], C[hi[q]], ac, bc, xd);
-        Dxr[q] = herm(D[lo[q]], D[hi[q]], ad, bd, xd);
-        Cyr[q] = lin(Cy[lo[q]], Cy[hi[q]], xd); Dyr[q] = lin(Dy[lo[q]], Dy[hi[q]], xd);
-        Ctr[q] = lin(Ct[lo[q]], Ct[hi[q]], xd); Dtr[q] = lin(Dt[lo[q]], Dt[hi[q]], xd);
-        Csr[q] = lin(Cs[lo[q]], Cs[hi[q]], xd); Dsr[q] = lin(Ds[lo[q]], Ds[hi[q]], xd);
-    }
-    R Cyv[2], Dyv[2], Csv[2], Dsv[2], Ctv[2], Dtv[2];
+    for (int lv = 0; lv < 2; ++lv) {
+        const int ll = lv ? lp : l;
+        R x0[3], y0[3], t0[3], s0[3], x1[3], y1[3], t1[3], s1[3];
+        xstage(x0, y0, t0, s0, pk + 16 * (col[0] * nl + ll), pk + 16 * (col[2] * nl + ll), xd, dx);
+        xstage(x1, y1, t1, s1, pk + 16 * (col[1] * nl + ll), pk + 16 * (col[3] * nl + ll), xd, dx);
 #pragma unroll
-    for (int q = 0; q < 2; ++q) {
-        const int a0 = 2 * q, a1 = 2 * q + 1;
-        const R ac = dy * Cyr[a0] - (Cxr[a1] - Cxr[a0]);
-        const R bc = -dy * Cyr[a1] + (Cxr[a1] - Cxr[a0]);
-        const R ad = dy * Dyr[a0] - (Dxr[a1] - Dxr[a0]);
-        const R bd = -dy * Dyr[a1] + (Dxr[a1] - Dxr[a0]);
-        Cyv[q] = herm(Cxr[a0], Cxr[a1], ac, bc, yd);
-        Dyv[q] = herm(Dxr[a0], Dxr[a1], ad, bd, yd);
-        Csv[q] = lin(Csr[a0], Csr[a1], yd); Dsv[q] = lin(Dsr[a0], Dsr[a1], yd);
-        Ctv[q] = lin(Ctr[a0], Ctr[a1], yd); Dtv[q] = lin(Dtr[a0], Dtr[a1], yd);
+        for (int f = 0; f < 3; ++f) {
+            const R ac = dy * y0[f] - (x1[f] - x0[f]);
+            const R bc = -dy * y1[f] + (x1[f] - x0[f]);
+            yv[lv][f] = herm(x0[f], x1[f], ac, bc, yd);
+            sv[lv][f] = lin(s0[f], s1[f], yd);
+            tv[lv][f] = lin(t0[f], t1[f], yd);
+        }
     }
-    const R a0 = ds * Csv[0] - (Cyv[1] - Cyv[0]), b0 = -ds * Csv[1] + (Cyv[1] - Cyv[0]);
-    const R a1 = ds * Dsv[0] - (Dyv[1] - Dyv[0]), b1 = -ds * Dsv[1] + (Dyv[1] - Dyv[0]);
-    const R CC = herm(Cyv[0], Cyv[1], a0, b0, s0), DD = herm(Dyv[0], Dyv[1], a1, b1, s1);
-    const R CCt = lin(Ctv[0], Ctv[1], s0), DDt = lin(Dtv[0], Dtv[1], s1);
-    const R a = dt * CCt - (DD - CC), b = -dt * DDt + (DD - CC);
-    return herm(CC, DD, a, b, td);
+#pragma unroll
+    for (int f = 0; f < 3; ++f) {
+        const R a = ds * sv[0][f] - (yv[1][f] - yv[0][f]);
+        const R b = -ds * sv[1][f] + (yv[1][f] - yv[0][f]);
+        v[f] = herm(yv[0][f], yv[1][f], a, b, sd);
+        vt[f] = lin(tv[0][f], tv[1][f], sd);
+    }
 }
 }  // namespace
 
@@ -726,7 +742,8 @@ __global__ __launch_bounds__(256) void k_query(QueryJob<R, TQ> J) {
     R C[4], D[4], Ct[4], Dt[4], Cx[4], Dx[4], Cy[4], Dy[4];
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
-        const R* e = J.pk_eta + 8 * ac[q];
+        R e[8];
+        load_rec(e, J.pk_eta + 8 * ac[q]);
         C[q] = e[0]; D[q] = e[1]; Ct[q] = e[2]; Dt[q] = e[3];
         Cx[q] = e[4]; Dx[q] = e[5]; Cy[q] = e[6]; Dy[q] = e[7];
     }
@@ -759,11 +776,14 @@ __global__ __launch_bounds__(256) void k_query(QueryJob<R, TQ> J) {
     const R ds = (R)g.ds;
     const double rmp = J.ramp_scale ? (double)J.ramp_scale[p] : 1.0;
     J.o_eta[p] = (TQ)((double)eta * rmp);
+    R VC[3], VCt[3], VD[3], VDt[3];
+    slot_velo(VC, VCt, J.pk_C, ac, nl, l0, xd, yd, sd0, dx, dy, ds);
+    slot_velo(VD, VDt, J.pk_D, ac, nl, l1, xd, yd, sd1, dx, dy, ds);
     TQ* outs[3] = {J.o_u, J.o_v, J.o_w};
-#pragma unroll 1
+#pragma unroll
     for (int f = 0; f < 3; ++f) {
-        const R v = spline_velo(J.pk_C, J.pk_D, nx, ny, nl, i, j, l0, l1, f, xd, yd, sd0, sd1, tdR, dx, dy, ds, dt);
-        outs[f][p] = (TQ)((double)v * rmp);
+        const R a = dt * VCt[f] - (VD[f] - VC[f]), b = -dt * VDt[f] + (VD[f] - VC[f]);
+        outs[f][p] = (TQ)((double)herm(VC[f], VD[f], a, b, tdR) * rmp);
     }
 }
 
