@@ -235,6 +235,31 @@ def batched_leg(nstates: int, nsample_err: int) -> dict:
             "per_state_err_vs_fp64": errs}
 
 
+def update_leg(engine, grid, updates: int) -> dict:
+    """Time stepping of the filled grid (CheckTime / update_lsgrid): each
+    force_update advances the grid by one time level and computes that new
+    level (256 x 256 columns x 16 levels: column and level passes, FD
+    gradients, packing); per-update time and grid points per second."""
+    grid.refill()
+    n = grid.force_update(T0 + 1.5 * DT)                # warm-up: one update
+    engine.sync()
+    engine.prof_enable(True)
+    engine.prof_read(reset=True)
+    t0 = time.perf_counter()
+    done = 0
+    for k in range(updates):
+        done += grid.force_update(T0 + (k + 2.5) * DT)
+    engine.sync()
+    el = time.perf_counter() - t0
+    prof = engine.prof_read(reset=True)
+    engine.prof_enable(False)
+    if n != 1 or done != updates:
+        raise RuntimeError(f"update leg: expected one time level per call, got {n} and {done}/{updates}")
+    return {"workload": f"{updates} updates of the {NX}x{NY}x{NL} s-grid, one new time level each",
+            "ms_per_update": el / updates * 1e3, "points_per_s": NX * NY * NL * updates / el,
+            "kernels_ms": {k: v[0] / max(v[1], 1) for k, v in prof.items()}}
+
+
 def direct_leg(steps: int, smit: bool = False) -> dict:
     """Direct (per-point semantics) workload: N=512 long-crested sea, 256 x on
     [0, 500] m x 16 t on [0, 15] s columns, 64 z per column on [-30, eta]:
@@ -304,13 +329,15 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--precision", default="fp32", choices=["fp32", "fp64"])
     ap.add_argument("--queries", type=int, default=10_000_000)
-    ap.add_argument("--cpu-cols", type=int, default=4)
+    ap.add_argument("--cpu-cols", type=int, default=12)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-worker", nargs=2, default=None)
     ap.add_argument("--rocprof-summary", nargs=2, metavar=("DB", "COMMAND"), default=None,
                     help="print a markdown summary of a rocprofv3 results database and exit")
     ap.add_argument("--no-surrogate", action="store_true", help="skip the learned-surrogate leg")
     ap.add_argument("--no-direct", action="store_true", help="skip the direct per-point workload leg")
+    ap.add_argument("--updates", type=int, default=5,
+                    help="time-stepping updates of the grid to time after the fill (0: skip)")
     ap.add_argument("--batched-states", type=int, default=48,
                     help="sea states of the batched leg (512 = the full batched workload; 0 skips it)")
     ap.add_argument("--calib-copy", action="store_true",
@@ -390,7 +417,9 @@ def main():
 
     points = 4 * NX * NY * NL
     value = points * a.steps / el
+    # queries: cell ordering (key pass + radix sort) plus the spline kernel
     q_ms, q_n = prof.get("k_query", (float("nan"), 1))
+    q_ms += prof.get("k_query_sort", (0.0, 1))[0]
     qfi = grid.qform_info()
     if qfi is not None:
         # dominant kernel: the second-order pair sums as bf16 MFMA quadratic forms
@@ -412,6 +441,8 @@ def main():
                 "peak": PEAK[a.precision] / 1e12, "unit": "TFLOP/s",
                 "frac": flop_per_launch / lev_avg / PEAK[a.precision], "traffic": None,
                 "avg_launch_ms": lev_avg * 1e3, "flop_per_launch": flop_per_launch}
+
+    upd = update_leg(engine, grid, a.updates) if a.updates > 0 else None
 
     # accuracy gate: FP32 (or FP64) grid vs an FP64 grid on an 8 x 8 block of the same columns
     dx = (DOMAIN[1] - DOMAIN[0]) / (NX - 1)
@@ -466,6 +497,7 @@ def main():
         "batched": bat,
         "direct": dirl,
         "direct_smit": smitl,
+        "update": upd,
     }
     print(json.dumps(line), flush=True)
 

@@ -805,6 +805,9 @@ int Grid::batch_end() {
 
 void Grid::release_workspace() {
     zbuf_.release();
+    qs_key_.release();
+    qs_rec_.release();
+    qs_tmp_.release();
 }
 
 namespace {
@@ -1129,6 +1132,9 @@ int Grid::write_vtu(const char* path, bool last, const char* time_label) {
     return 0;
 }
 
+// Query batches from this size on are evaluated in cell order.
+constexpr int64_t kQuerySortMin = 65536;
+
 template <typename R, typename TQ>
 int Grid::run_query(int64_t n, const TQ* t, double tconst, const TQ* x, const TQ* y, const TQ* z,
                     TQ* eta, TQ* u, TQ* v, TQ* w, const double* ramp) {
@@ -1141,7 +1147,23 @@ int Grid::run_query(int64_t n, const TQ* t, double tconst, const TQ* x, const TQ
     J.t = t; J.tconst = tconst; J.x = x; J.y = y; J.z = z;
     J.ramp_scale = ramp;
     J.o_eta = eta; J.o_u = u; J.o_v = v; J.o_w = w;
-    PROF("k_query", (launch_query<R, TQ>(J, Engine::get().stream())));
+    hipStream_t st = Engine::get().stream();
+    const int64_t ncol = (int64_t)g.nx * g.ny;
+    if (n >= kQuerySortMin && n <= INT32_MAX) {
+        // large batches run in cell order (qsort.hip)
+        int bits = 1;
+        while (((int64_t)1 << bits) < ncol) ++bits;
+        size_t tmp = 0;
+        CFDWM_HIP(query_sort_tmp_bytes<TQ>(n, bits, &tmp));
+        if (qs_key_.alloc(2 * n * sizeof(uint32_t)) || qs_rec_.alloc(2 * n * sizeof(QRec<TQ>)) ||
+            qs_tmp_.alloc(std::max<size_t>(tmp, 256)))
+            return -3;
+        QRec<TQ>* rec = (QRec<TQ>*)qs_rec_.get();
+        PROF("k_query_sort", (launch_query_sort<TQ>(n, g, x, y, z, bits, (uint32_t*)qs_key_.get(), rec,
+                                                    qs_tmp_.get(), qs_tmp_.size(), st)));
+        J.srec = rec + n;
+    }
+    PROF("k_query", (launch_query<R, TQ>(J, st)));
     return 0;
 }
 

@@ -72,6 +72,20 @@ struct PackJob {
     R *pk_eta, *pk_C, *pk_D;
 };
 
+// A query in cell order: its coordinates and its index in the caller's arrays.
+template <typename TQ>
+struct QRec;
+template <>
+struct QRec<float> {
+    float x, y, z;
+    uint32_t p;
+};
+template <>
+struct QRec<double> {
+    double x, y, z;
+    uint64_t p;
+};
+
 template <typename R, typename TQ>
 struct QueryJob {
     int64_t n = 0;
@@ -83,6 +97,9 @@ struct QueryJob {
     double tconst = 0;
     const double* ramp_scale = nullptr;
     TQ *o_eta = nullptr, *o_u = nullptr, *o_v = nullptr, *o_w = nullptr;
+    // when set: thread p evaluates the query srec[p] (cell order) instead of
+    // (x, y, z)[p] and writes its outputs at the query's own index
+    const QRec<TQ>* srec = nullptr;
 };
 
 template <typename R>
@@ -103,6 +120,15 @@ template <typename R, typename TQ>
 hipError_t launch_query(const QueryJob<R, TQ>& J, hipStream_t st);
 template <typename R>
 hipError_t launch_pack(const PackJob<R>& J, hipStream_t st);
+// Cell order of a query batch (qsort.hip): key[p] = column of query p's cell,
+// rec[p] = (x, y, z, p), then a radix sort of (key, rec) over key_bits bits.
+// keys and recs hold 2 n entries (input half, sorted half), tmp tmp_bytes
+// (query_sort_tmp_bytes).  On return recs + n holds the cell-ordered records.
+template <typename TQ>
+hipError_t query_sort_tmp_bytes(int64_t n, int key_bits, size_t* tmp_bytes);
+template <typename TQ>
+hipError_t launch_query_sort(int64_t n, const GridGeom& g, const TQ* x, const TQ* y, const TQ* z, int key_bits,
+                             uint32_t* keys, QRec<TQ>* recs, void* tmp, size_t tmp_bytes, hipStream_t st);
 // Z operands, then the MFMA quadratic form of the filtered eta2 pair sum
 hipError_t launch_qf_zgen(const QfJob& J, hipStream_t st);
 hipError_t launch_qf_forms(const QfJob& J, hipStream_t st);
@@ -317,6 +343,7 @@ private:
     DevBuf f_[19];
     DevBuf q_in_, q_out_;
     DevBuf pk_eta_, pk_C_, pk_D_;             // query records of the two active slots
+    DevBuf qs_key_, qs_rec_, qs_tmp_;         // cell-ordered query batches
     template <typename R>
     int pack();
     template <typename R>

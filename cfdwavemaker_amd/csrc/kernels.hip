@@ -715,14 +715,24 @@ __device__ __forceinline__ void slot_velo(R (&v)[3], R (&vt)[3], const R* __rest
 
 // get_kinematics_at_point (lsgrid_spline.cpp:505-605).  The cell fractions
 // follow std::modf(double, float*): the coordinate is rounded to float and the
-// fraction taken in float.  TQ: coordinate/output type of the queries.
+// fraction taken in float.  TQ: coordinate/output type of the queries.  With
+// J.srec thread k evaluates the k-th query in cell order; p is always the
+// query's index in the caller's arrays.
 template <typename R, typename TQ>
 __global__ __launch_bounds__(256) void k_query(QueryJob<R, TQ> J) {
-    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= J.n) return;
+    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= J.n) return;
+    int64_t p = k;
+    double xpt, ypt, zpt;
+    if (J.srec) {
+        const QRec<TQ> r = J.srec[k];
+        p = (int64_t)r.p;
+        xpt = (double)r.x; ypt = (double)r.y; zpt = (double)r.z;
+    } else {
+        xpt = (double)J.x[p]; ypt = (double)J.y[p]; zpt = (double)J.z[p];
+    }
     const GridGeom& g = J.g;
     const double tpt = J.t ? (double)J.t[p] : J.tconst;
-    const double xpt = (double)J.x[p], ypt = (double)J.y[p], zpt = (double)J.z[p];
     double cx = (xpt - g.x0) / g.dx;
     cx = cx < (g.nx - 1.) ? cx : (g.nx - 1.);
     cx = cx > 0. ? cx : 0.;

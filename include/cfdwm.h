@@ -223,8 +223,9 @@ typedef struct {
  * time as field data named time_label (NULL: "TimeValue"), velocity vectors at
  * the nodes, node positions from the sigma map, hexahedral cells. */
 CFDWM_EXPORT int cfdwm_grid_write_vtu(cfdwm_grid* grid, const char* path, int last, const char* time_label);
-/* Free the sea's form tables and the grid's operand workspace (rebuilt on the
- * next FP32 fill); the grid's fields are kept. */
+/* Free the sea's form tables, the grid's operand workspace (rebuilt on the
+ * next FP32 fill) and its query-ordering buffers (rebuilt by the next batch of
+ * 65,536 or more queries); the grid's fields are kept. */
 CFDWM_EXPORT int cfdwm_grid_release_workspace(cfdwm_grid* grid);
 CFDWM_EXPORT int cfdwm_grids_refill_batched(cfdwm_grid** grids, int n, double budget_gb, int release,
                                             cfdwm_batch_stats* stats);
