@@ -80,6 +80,20 @@ struct Global {
 };
 Global G;
 
+// Device scratch of wave_EvalPointsDevice (kept between calls).
+struct DevPoints {
+    DevBuf z, e1, e2, z0, q4, e4;
+};
+DevPoints g_dev;
+
+RampDev ramp_dev(const RampCfg& r) {
+    RampDev d;
+    d.init = r.init ? 1 : 0;
+    for (int q = 0; q < 6; ++q) d.on[q] = r.on[q] ? 1 : 0;
+    for (int q = 0; q < 12; ++q) d.se[q] = r.se[q];
+    return d;
+}
+
 void trim(std::string& s) {
     auto ns = [](int ch) { return !std::isspace(ch); };
     s.erase(s.begin(), std::find_if(s.begin(), s.end(), ns));
@@ -626,6 +640,34 @@ static int eval_points_impl(int64_t n, const double* x, const double* y, const d
     if (!G.initialized) { set_error("wave_Initialize has not been called"); return -1; }
     if (n < 0 || !x || !y || !t || ((u || v || w || p) && !z)) { set_error("bad arguments"); return -1; }
     if (n == 0) return 0;
+    if (G.wavetype == 1 && dev) {
+        // device-resident: z clamp, the batch, then elevation sum and ramps
+        // in one pass (the host branch's operations, in its order)
+        Sea& s = *G.csea->s;
+        hipStream_t st = Engine::get().stream();
+        const double* zz = z;
+        if (z) {
+            if (g_dev.z.alloc(n * sizeof(double))) return -3;
+            CFDWM_HIP(launch_clamp_below(n, z, -G.depth, (double*)g_dev.z.get(), st));
+            zz = (const double*)g_dev.z.get();
+        }
+        double *e1 = nullptr, *e2 = nullptr;
+        if (eta) {
+            if (g_dev.e1.alloc(n * sizeof(double)) || g_dev.e2.alloc(n * sizeof(double))) return -3;
+            e1 = (double*)g_dev.e1.get();
+            e2 = G.order == 2 ? (double*)g_dev.e2.get() : nullptr;
+        }
+        const uint32_t fl = order_flags(G.order) | (p ? EV_P : 0u);
+        if (int rc = s.eval_points(n, x, y, zz, t, fl, G.order == 2, e1, e2, nullptr, u, v, w, p, true)) return rc;
+        PointsFinish F;
+        F.n = n; F.ramp = ramp_dev(G.ramp); F.t = t; F.x = x; F.y = y;
+        F.eadd = 1; F.eoff = G.swl; F.ea = e1; F.eb = e2; F.eta = eta;
+        double* outs[4] = {u, v, w, p};
+        for (int c = 0; c < 4; ++c) { F.src[c] = outs[c]; F.dst[c] = outs[c]; }
+        CFDWM_HIP(launch_points_finish(F, st));
+        CFDWM_HIP(hipStreamSynchronize(st));
+        return 0;
+    }
     if (G.wavetype == 1) {
         Sea& s = *G.csea->s;
         std::vector<double> zc, e1, e2;
@@ -662,6 +704,39 @@ static int eval_points_impl(int64_t n, const double* x, const double* y, const d
             return 0;
         };
         if (fin(eta, he) || fin(u, hu) || fin(v, hv) || fin(w, hw) || fin(p, hp)) return -3;
+        return 0;
+    }
+    if (G.wavetype == 5 && dev) {
+        // device-resident s-grid queries: velocities at the clamped z, the
+        // elevation from a query at z = 0, then the ramps
+        Grid& g = *G.cgrid->g;
+        hipStream_t st = Engine::get().stream();
+        double *q4 = nullptr, *e4 = nullptr;
+        if (u || v || w) {
+            if (g_dev.z.alloc(n * sizeof(double)) || g_dev.q4.alloc(4 * n * sizeof(double))) return -3;
+            double* zc = (double*)g_dev.z.get();
+            q4 = (double*)g_dev.q4.get();
+            CFDWM_HIP(launch_clamp_below(n, z, -G.depth, zc, st));
+            if (int rc = g.query_device_f64(n, t, x, y, zc, q4, q4 + n, q4 + 2 * n, q4 + 3 * n)) return rc;
+        }
+        if (eta) {
+            if (g_dev.z0.alloc(n * sizeof(double)) || g_dev.e4.alloc(4 * n * sizeof(double))) return -3;
+            double* z0 = (double*)g_dev.z0.get();
+            e4 = (double*)g_dev.e4.get();
+            CFDWM_HIP(hipMemsetAsync(z0, 0, n * sizeof(double), st));
+            if (int rc = g.query_device_f64(n, t, x, y, z0, e4, e4 + n, e4 + 2 * n, e4 + 3 * n)) return rc;
+        }
+        if (p) { set_error("dynamic pressure is defined for wave type 1 only (CFDwavemaker.cpp:1800-1813)"); }
+        PointsFinish F;
+        F.n = n; F.ramp = ramp_dev(G.ramp); F.t = t; F.x = x; F.y = y;
+        F.eadd = 0; F.ea = e4; F.eta = eta;
+        double* outs[4] = {u, v, w, p};
+        for (int c = 0; c < 4; ++c) {
+            F.dst[c] = outs[c];
+            F.src[c] = (c < 3 && outs[c]) ? q4 + (c + 1) * n : nullptr;    // p: 0
+        }
+        CFDWM_HIP(launch_points_finish(F, st));
+        CFDWM_HIP(hipStreamSynchronize(st));
         return 0;
     }
     if (G.wavetype == 5) {

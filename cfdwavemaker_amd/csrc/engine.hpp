@@ -72,6 +72,35 @@ struct PackJob {
     R *pk_eta, *pk_C, *pk_D;
 };
 
+// Ramp of the reference's Ramp class (Utils.cpp:26-68) for device point
+// batches: axis q (time in, time out, x in, x out, y in, y out) is on when
+// on[q], a linear 0..1 ramp from se[2q] to se[2q + 1] (reversed for the odd,
+// "out" axes); the factor is the smallest over the axes, 1 when init is 0.
+struct RampDev {
+    int init = 0;
+    int on[6] = {0, 0, 0, 0, 0, 0};
+    double se[12] = {};
+};
+
+// Last pass of a device point batch (wave_EvalPointsDevice): with r the ramp
+// factor of (t, x, y)[q],
+//   eta[q] = (eadd ? (eoff + ea[q]) + (eb ? eb[q] : 0) : ea[q]) * r   (eta != null)
+//   dst[c][q] = src[c] ? src[c][q] * r : 0                             (dst[c] != null)
+struct PointsFinish {
+    int64_t n = 0;
+    RampDev ramp{};
+    const double *t = nullptr, *x = nullptr, *y = nullptr;
+    int eadd = 0;
+    double eoff = 0;
+    const double *ea = nullptr, *eb = nullptr;
+    double* eta = nullptr;
+    const double* src[4] = {nullptr, nullptr, nullptr, nullptr};
+    double* dst[4] = {nullptr, nullptr, nullptr, nullptr};
+};
+hipError_t launch_points_finish(const PointsFinish& F, hipStream_t st);
+// out[q] = max(lo, z[q]) in the host's std::max semantics
+hipError_t launch_clamp_below(int64_t n, const double* z, double lo, double* out, hipStream_t st);
+
 // A query in cell order: its coordinates and its index in the caller's arrays.
 template <typename TQ>
 struct QRec;

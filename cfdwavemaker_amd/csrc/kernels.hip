@@ -797,6 +797,49 @@ __global__ __launch_bounds__(256) void k_query(QueryJob<R, TQ> J) {
     }
 }
 
+// ===================================================== device point batches
+namespace {
+// RampCfg::r1d and operator() of the host entries, same operations and order
+__device__ __forceinline__ double ramp_axis(double a, double xs, double xe, bool inv) {
+    const double v = inv ? 1. - ((a - xs) / (xe - xs)) : ((a - xs) / (xe - xs));
+    const double m = (1. < v) ? 1. : v;               // std::min(v, 1.)
+    return (m < 0.) ? 0. : m;                         // std::max(., 0.)
+}
+__device__ __forceinline__ double ramp_factor(const RampDev& R, double t, double x, double y) {
+    if (!R.init) return 1.0;
+    const double a[6] = {t, t, x, x, y, y};
+    double m = 1.0;
+#pragma unroll
+    for (int q = 0; q < 6; ++q) {
+        const double r = R.on[q] ? ramp_axis(a[q], R.se[2 * q], R.se[2 * q + 1], (q & 1) != 0) : 1.0;
+        if (q == 0 || r < m) m = r;                   // std::min_element
+    }
+    return m;
+}
+
+__global__ __launch_bounds__(256) void k_points_finish(PointsFinish F) {
+    const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= F.n) return;
+    const double r = ramp_factor(F.ramp, F.t[q], F.x[q], F.y[q]);
+    if (F.eta) {
+        double e = F.ea[q];
+        if (F.eadd) e = (F.eoff + e) + (F.eb ? F.eb[q] : 0.0);
+        F.eta[q] = e * r;
+    }
+#pragma unroll
+    for (int c = 0; c < 4; ++c)
+        if (F.dst[c]) F.dst[c][q] = F.src[c] ? F.src[c][q] * r : 0.0;
+}
+
+__global__ __launch_bounds__(256) void k_clamp_below(int64_t n, const double* __restrict__ z, double lo,
+                                                     double* __restrict__ out) {
+    const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= n) return;
+    const double v = z[q];
+    out[q] = (lo < v) ? v : lo;                       // std::max(lo, v)
+}
+}  // namespace
+
 // ==================================================== profile-term pruning
 // eta range (minus swl) over the columns written by one fill
 template <typename R>
@@ -939,6 +982,19 @@ hipError_t launch_pack(const PackJob<R>& J, hipStream_t st) {
 }
 template hipError_t launch_pack<double>(const PackJob<double>&, hipStream_t);
 template hipError_t launch_pack<float>(const PackJob<float>&, hipStream_t);
+
+hipError_t launch_points_finish(const PointsFinish& F, hipStream_t st) {
+    if (F.n <= 0) return hipSuccess;
+    if (!F.t || !F.x || !F.y || (F.eta && !F.ea)) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_points_finish, dim3(nblk(F.n)), dim3(256), 0, st, F);
+    return hipGetLastError();
+}
+
+hipError_t launch_clamp_below(int64_t n, const double* z, double lo, double* out, hipStream_t st) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_clamp_below, dim3(nblk(n)), dim3(256), 0, st, n, z, lo, out);
+    return hipGetLastError();
+}
 
 template <typename R, typename TQ>
 hipError_t launch_query(const QueryJob<R, TQ>& J, hipStream_t st) {

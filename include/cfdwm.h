@@ -113,7 +113,9 @@ CFDWM_EXPORT int wave_PrefetchPoints(int64_t n, const double* x, const double* y
 CFDWM_EXPORT int wave_PrefetchClear(void);
 CFDWM_EXPORT int wave_PrefetchStats(int64_t* hits, int64_t* misses);
 /* As wave_EvalPoints, all pointers in device memory (the call synchronises
- * the library's stream before returning). */
+ * the library's stream before returning).  The points never leave the
+ * device: the z clamp, the batch, the elevation sum and the ramps run as
+ * device passes, with the same results as wave_EvalPoints bit for bit. */
 CFDWM_EXPORT int wave_EvalPointsDevice(int64_t n, const double* x, const double* y, const double* z,
                                        const double* t, double* eta, double* u, double* v, double* w,
                                        double* p, uint32_t flags);
