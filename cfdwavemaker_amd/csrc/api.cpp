@@ -7,6 +7,7 @@
 #include <cstring>
 #include <atomic>
 #include <unordered_map>
+#include <mutex>
 #include <shared_mutex>
 #include <fstream>
 #include <iostream>
@@ -80,8 +81,10 @@ struct Global {
 };
 Global G;
 
-// Device scratch of wave_EvalPointsDevice (kept between calls).
+// Device scratch of wave_EvalPointsDevice (kept between calls; one caller at
+// a time).
 struct DevPoints {
+    std::mutex mu;
     DevBuf z, e1, e2, z0, q4, e4;
 };
 DevPoints g_dev;
@@ -645,6 +648,7 @@ static int eval_points_impl(int64_t n, const double* x, const double* y, const d
         // in one pass (the host branch's operations, in its order)
         Sea& s = *G.csea->s;
         hipStream_t st = Engine::get().stream();
+        std::lock_guard<std::mutex> lk(g_dev.mu);
         const double* zz = z;
         if (z) {
             if (g_dev.z.alloc(n * sizeof(double))) return -3;
@@ -711,6 +715,7 @@ static int eval_points_impl(int64_t n, const double* x, const double* y, const d
         // elevation from a query at z = 0, then the ramps
         Grid& g = *G.cgrid->g;
         hipStream_t st = Engine::get().stream();
+        std::lock_guard<std::mutex> lk(g_dev.mu);
         double *q4 = nullptr, *e4 = nullptr;
         if (u || v || w) {
             if (g_dev.z.alloc(n * sizeof(double)) || g_dev.q4.alloc(4 * n * sizeof(double))) return -3;

@@ -1193,12 +1193,14 @@ int Grid::query(int64_t n, const double* t, const double* x, const double* y, co
 
 int Grid::query_device_f32(int64_t n, double t, const float* x, const float* y, const float* z,
                            float* eta, float* u, float* v, float* w) {
+    std::lock_guard<std::mutex> lk(sea->mu);          // the grid's query-ordering buffers
     if (prec == kFP32) return run_query<float, float>(n, nullptr, t, x, y, z, eta, u, v, w, nullptr);
     return run_query<double, float>(n, nullptr, t, x, y, z, eta, u, v, w, nullptr);
 }
 
 int Grid::query_device_f64(int64_t n, const double* t, const double* x, const double* y, const double* z,
                            double* eta, double* u, double* v, double* w) {
+    std::lock_guard<std::mutex> lk(sea->mu);          // the grid's query-ordering buffers
     if (prec == kFP32) return run_query<float, double>(n, t, 0.0, x, y, z, eta, u, v, w, nullptr);
     return run_query<double, double>(n, t, 0.0, x, y, z, eta, u, v, w, nullptr);
 }

@@ -320,11 +320,18 @@ __device__ __forceinline__ void levels_body(const SeaDev<R>& S, const LevelJob<R
             gy += (double)(c.gyz * cr);
             gz += (double)(c.gzz * sr);
             if (WP) gp += (double)(c.gpz * cr);
+            // FP32: the diagonal's 2k profile is the square of the k profile
+            // (p2k = 2 pk, p2g = 2 pg), so the k exponentials serve both
+            constexpr bool SQ = sizeof(R) == 4;
+            R Ek[LC], Gk[LC];
+            if (lin || (SQ && sec)) {
+#pragma unroll
+                for (int l = 0; l < LC; ++l) prof(c.pk, c.pg, zp[l], Ek[l], Gk[l]);
+            }
             if (lin) {
 #pragma unroll
                 for (int l = 0; l < LC; ++l) {
-                    R E, G;
-                    prof(c.pk, c.pg, zp[l], E, G);
+                    const R E = Ek[l], G = Gk[l];
                     const R ch = E + G, sh = E - G;
                     U[l] += (double)(c.lu * cr * ch);
                     V[l] += (double)(c.lv * cr * ch);
@@ -339,7 +346,12 @@ __device__ __forceinline__ void levels_body(const SeaDev<R>& S, const LevelJob<R
 #pragma unroll
                 for (int l = 0; l < LC; ++l) {
                     R E, G;
-                    prof(c.p2k, c.p2g, zp[l], E, G);
+                    if constexpr (SQ) {
+                        E = Ek[l] * Ek[l];
+                        G = Gk[l] * Gk[l];
+                    } else {
+                        prof(c.p2k, c.p2g, zp[l], E, G);
+                    }
                     const R ch = E + G, sh = E - G;
                     ur[l] = du * c2 * ch;
                     vr[l] = dv * c2 * ch;
