@@ -333,9 +333,17 @@ __device__ __forceinline__ void levels_body(const SeaDev<R>& S, const LevelJob<R
                 for (int l = 0; l < LC; ++l) {
                     const R E = Ek[l], G = Gk[l];
                     const R ch = E + G, sh = E - G;
-                    U[l] += (double)(c.lu * cr * ch);
-                    V[l] += (double)(c.lv * cr * ch);
-                    W[l] += (double)(c.lw * sr * sh);
+                    if constexpr (SQ) {
+                        // FP32: the component's terms meet in ur/vr/wr and are
+                        // folded into the FP64 totals once, below
+                        ur[l] = c.lu * cr * ch;
+                        vr[l] = c.lv * cr * ch;
+                        wr[l] = c.lw * sr * sh;
+                    } else {
+                        U[l] += (double)(c.lu * cr * ch);
+                        V[l] += (double)(c.lv * cr * ch);
+                        W[l] += (double)(c.lw * sr * sh);
+                    }
                     if (WP) P += (double)(c.lp * cr * ch);
                 }
             }
@@ -353,9 +361,15 @@ __device__ __forceinline__ void levels_body(const SeaDev<R>& S, const LevelJob<R
                         prof(c.p2k, c.p2g, zp[l], E, G);
                     }
                     const R ch = E + G, sh = E - G;
-                    ur[l] = du * c2 * ch;
-                    vr[l] = dv * c2 * ch;
-                    wr[l] = dw * s2 * sh;
+                    if constexpr (SQ) {
+                        ur[l] += du * c2 * ch;
+                        vr[l] += dv * c2 * ch;
+                        wr[l] += dw * s2 * sh;
+                    } else {
+                        ur[l] = du * c2 * ch;
+                        vr[l] = dv * c2 * ch;
+                        wr[l] = dw * s2 * sh;
+                    }
                     if (WP && direct) pr = c.ddp * c2 * ch;
                 }
             }
