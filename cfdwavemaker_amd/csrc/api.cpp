@@ -1017,7 +1017,7 @@ int cfdwm_grid_release_workspace(cfdwm_grid* grid) {
     return 0;
 }
 int cfdwm_grids_refill_batched(cfdwm_grid** grids, int n, double budget_gb, int release, cfdwm_batch_stats* out) {
-    if (!grids || n < 0) { set_error("bad arguments"); return -1; }
+    if (n < 0 || (n > 0 && !grids)) { set_error("bad arguments"); return -1; }
     std::vector<Grid*> gs(n);
     for (int k = 0; k < n; ++k) {
         if (!grids[k]) { set_error("bad arguments"); return -1; }

@@ -1,0 +1,75 @@
+"""Edge cases of the batched paths: empty batches, batch sizes around the
+thresholds where the implementation changes path (2,048 points: the eta2 pair
+sum as the MFMA form in FP32; 65,536 queries: cell order), single points,
+and an empty batched refill."""
+import numpy as np
+import pytest
+
+from tests.conftest import golden, scaled_err, sea_from_golden
+from tests.golden.cases import GRID_CASE
+
+pytestmark = pytest.mark.gpu
+
+
+def _sea(prec, case="B"):
+    from cfdwavemaker_amd import engine as eng
+    return eng.Sea(sea_from_golden(golden("points.npz"), case), precision=prec, with_direct=True)
+
+
+def test_empty_batches():
+    import torch
+    from cfdwavemaker_amd import engine as eng
+    for prec in (eng.FP64, eng.FP32):
+        s = _sea(prec)
+        e = np.zeros(0)
+        for kind in ("eta1", "eta2", "eta", "u", "v", "w", "dp"):
+            out = s.eval(kind, e, e, e, e if kind not in ("eta1", "eta2", "eta") else None)
+            assert out.shape == (0,)
+        c = GRID_CASE
+        g = s.grid(c["domain"], c["nx"], c["ny"], c["nl"], c["t0"], c["dt"], c["tan_a"], c["tan_b"], c["depth"])
+        assert g.query(e, e, e, e).shape == (0, 4)
+        buf = torch.zeros(4, dtype=torch.float32, device="cuda")
+        g.query_device_f32(0, 0.0, *([buf.data_ptr()] * 7))
+        eng.sync()
+        f = s.eval_fields_device(*(torch.zeros(0, dtype=torch.float64, device="cuda") for _ in range(4)))
+        assert all(v.numel() == 0 for v in f.values())
+    st = eng.refill_batched([])
+    assert st["batched"] == 0 and st["single"] == 0
+
+
+@pytest.mark.parametrize("n", [1, 255, 257, 2047, 2048, 2049, 4097])
+def test_point_batch_sizes_fp32_vs_fp64(n):
+    """FP32 point batches below and above the MFMA-form threshold agree with
+    the FP64 evaluation (golden-pinned) to the FP32 bound."""
+    from cfdwavemaker_amd import engine as eng
+    rng = np.random.default_rng(n)
+    x = rng.uniform(-50, 150, n)
+    y = rng.uniform(-20, 20, n)
+    t = rng.uniform(0, 30, n)
+    z = rng.uniform(-25, 0.5, n)
+    s64, s32 = _sea(eng.FP64), _sea(eng.FP32)
+    for kind in ("eta2", "eta", "u", "w"):
+        zz = None if kind.startswith("eta") else z
+        a, b = s32.eval(kind, t, x, y, zz), s64.eval(kind, t, x, y, zz)
+        ref = s64.eval(kind, *(np.resize(q, 4096) for q in (t, x, y)), None if zz is None else np.resize(zz, 4096))
+        # scale: the field's magnitude over a fixed 4096-point sample (a single
+        # point's own value can be arbitrarily small)
+        err = float(np.max(np.abs(a - b)) / max(np.max(np.abs(ref)), 1e-300))
+        assert err <= 1e-5, (kind, n, err)
+
+
+@pytest.mark.parametrize("n", [65535, 65536, 65537])
+def test_query_sizes_around_cell_order_threshold(n):
+    """Batches just below, at and above the cell-order threshold return, bit
+    for bit, what the same queries return in small batches."""
+    from cfdwavemaker_amd import engine as eng
+    s = _sea(eng.FP32)
+    c = GRID_CASE
+    g = s.grid(c["domain"], c["nx"], c["ny"], c["nl"], c["t0"], c["dt"], c["tan_a"], c["tan_b"], c["depth"])
+    rng = np.random.default_rng(n)
+    x0, x1, y0, y1 = c["domain"]
+    pts = (rng.uniform(c["t0"], c["t0"] + c["dt"], n), rng.uniform(x0, x1, n), rng.uniform(y0, y1, n),
+           rng.uniform(-c["depth"], 0.5, n))
+    big = g.query(*pts)
+    small = np.concatenate([g.query(*(q[a:a + 10000] for q in pts)) for a in range(0, n, 10000)])
+    assert np.array_equal(big, small)
