@@ -28,6 +28,13 @@ sys.path.insert(0, ROOT)
 
 PEAK = {"fp32": 157.3e12, "fp64": 78.6e12}   # MI355X vector peaks (FLOP/s)
 PEAK_MFMA_BF16 = 2.5e15                        # MI355X dense bf16 matrix peak (FLOP/s)
+PEAK_HBM = 8.0e12                              # MI355X HBM3E (B/s)
+# algorithmic work of the secondary kernels: a spline query moves 12 B in + 16 B
+# out + its share of one pass over the packed grid (128 MB over 1e7 queries);
+# the column linear sums take 4 N (3 nl + 4) FLOP per column; the surrogate's
+# 29-256-256-256-65 network 2 (29*256 + 3*256^2 + 256*65) FLOP per column
+QUERY_BYTES = 41.0
+MLP_FLOP_PER_COLUMN = 2 * (29 * 256 + 3 * 256 * 256 + 256 * 65)
 NX = NY = 256
 NL = 16
 DOMAIN = (0.0, 1000.0, 0.0, 1000.0)
@@ -60,6 +67,20 @@ def cpu_worker(ncols: int, out_path: str) -> None:
         res = dict(kind="port", seconds=tot, points=4 * ncols * ncols * NL)
     with open(out_path, "w") as f:
         json.dump(res, f)
+
+
+def host_cpu() -> str:
+    """CPU model and logical CPU count of this host."""
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return f"{model}, {os.cpu_count()} logical CPUs"
 
 
 def cpu_baseline(ncols: int, threads: int):
@@ -107,8 +128,9 @@ def profiled_traffic(kernel: str):
 
 def rocprof_summary(db_path: str, command: str) -> str:
     """Markdown summary of a rocprofv3 results database: per kernel and launch
-    size the dispatch count and durations, and the mean of any PMC counter
-    (KiB per dispatch)."""
+    size the dispatch count and durations, and the mean of any PMC counter per
+    dispatch (FETCH_SIZE / WRITE_SIZE in KiB, instruction counters in
+    instructions, busy counters in cycles summed over the units)."""
     import sqlite3
     con = sqlite3.connect(db_path)
     lines = [f"command: `{command}`", ""]
@@ -125,7 +147,7 @@ def rocprof_summary(db_path: str, command: str) -> str:
                       "from counters_collection group by kernel_name, grid_size, counter_name "
                       "order by avg(duration) desc").fetchall()
     if pmc:
-        lines += ["", "| kernel | grid_size | counter | dispatches | mean KiB/dispatch | mean ms |", "|---|---|---|---|---|---|"]
+        lines += ["", "| kernel | grid_size | counter | dispatches | mean per dispatch | mean ms |", "|---|---|---|---|---|---|"]
         for nm, gs, cn, n, v, d in pmc:
             short = nm.replace("(anonymous namespace)::", "").split("(")[0].replace("void ", "")
             lines.append(f"| `{short}` | {gs} | {cn} | {n} | {v:.1f} | {d / 1e6:.3f} |")
@@ -162,6 +184,14 @@ def surrogate_leg(steps: int, warmup: int) -> dict:
     engine.sync()
     t_sur = (time.perf_counter() - t0) / steps
     pts = n * n * S.NLEV
+    # the network alone on the same features (bf16 MFMA GEMMs with fused bias + GELU)
+    X, _ = S.linear_fields(es, xd, yd, t, float(sea.depth), S.peak_wavenumber(sea))
+    sur.mlp.forward(X)
+    t3 = time.perf_counter()
+    for _ in range(steps):
+        sur.mlp.forward(X)
+    t_mlp = (time.perf_counter() - t3) / steps
+    mlp_flop = MLP_FLOP_PER_COLUMN * n * n
     eta = out[:, 0].cpu().numpy().astype(float)
     sample = np.arange(0, n * n, 256)
     gate = 1e-3 * HS
@@ -179,6 +209,9 @@ def surrogate_leg(steps: int, warmup: int) -> dict:
     eff = pts / (t_sur + t_chk + (0.0 if ok else t_exact))
     return {"workload": "N=1024 short-crested sea, 512x512 columns x 16 levels, one time level",
             "points": pts, "surrogate_points_per_s": pts / t_sur, "ms_surrogate": t_sur * 1e3,
+            "mlp": {"bound": "mfma", "flop_per_column": MLP_FLOP_PER_COLUMN, "ms": t_mlp * 1e3,
+                    "achieved": mlp_flop / t_mlp / 1e12, "peak": PEAK_MFMA_BF16 / 1e12, "unit": "TFLOP/s",
+                    "frac": mlp_flop / t_mlp / PEAK_MFMA_BF16},
             "residual_check": {"columns": int(sample.size), "eta_max_err_m": err, "eta_err_over_hs": err / HS,
                                "gate": "eta error <= 1e-3 Hs vs exact", "pass": bool(ok), "ms": t_chk * 1e3},
             "exact_points_per_s": pts / t_exact, "ms_exact_per_time_level": t_exact * 1e3,
@@ -442,6 +475,21 @@ def main():
                 "frac": flop_per_launch / lev_avg / PEAK[a.precision], "traffic": None,
                 "avg_launch_ms": lev_avg * 1e3, "flop_per_launch": flop_per_launch}
 
+    # rooflines of the other kernels of the step
+    lev = prof.get("k_levels_grid", (float("nan"), 1))
+    col = prof.get("k_columns_grid", (float("nan"), 1))
+    t_lin = (lev[0] / max(lev[1], 1) + col[0] / max(col[1], 1)) * 1e-3
+    lin_flop = 4.0 * len(s["omega"]) * (3 * NL + 4) * 4 * NX * NY
+    t_q = q_ms / max(q_n, 1) * 1e-3
+    rooflines = {
+        "spline_query": {"kernels": "k_query_sort + k_query", "bound": "hbm", "bytes_per_query": QUERY_BYTES,
+                         "achieved": QUERY_BYTES * nq / t_q / 1e9, "peak": PEAK_HBM / 1e9, "unit": "GB/s",
+                         "frac": QUERY_BYTES * nq / t_q / PEAK_HBM},
+        "column_linear_sums": {"kernels": "k_columns_grid + k_levels_grid", "bound": "valu",
+                               "flop_per_column": 4.0 * len(s["omega"]) * (3 * NL + 4),
+                               "achieved": lin_flop / t_lin / 1e12, "peak": PEAK[a.precision] / 1e12,
+                               "unit": "TFLOP/s", "frac": lin_flop / t_lin / PEAK[a.precision]},
+    }
     upd = update_leg(engine, grid, a.updates) if a.updates > 0 else None
 
     # accuracy gate: FP32 (or FP64) grid vs an FP64 grid on an 8 x 8 block of the same columns
@@ -465,6 +513,7 @@ def main():
         try:
             c = cpu_baseline(a.cpu_cols, threads)
             cpu = {"value": c["points"] / c["seconds"], "unit": "points/s", "cores": threads, "kind": c["kind"],
+                   "host": host_cpu(),
                    "sample": f"s-grid fill of {a.cpu_cols}x{a.cpu_cols} columns x {NL} levels x 4 time levels "
                              f"({c['points']} points, same N=1024 sea and spacing), {c['seconds']:.2f} s"}
         except Exception as e:  # reported, not fatal
@@ -488,6 +537,7 @@ def main():
                    "pair_terms": pair_terms, "diag_terms": diag_terms,
                    "points_per_step": points, "queries_per_step": nq, "precision": a.precision},
         "roofline": roof,
+        "rooflines_other": rooflines,
         "kernels_ms": {k: v[0] / max(v[1], 1) for k, v in prof.items()},
         "query_rate": nq / (q_ms / max(q_n, 1) * 1e-3) if q_ms == q_ms else None,
         "accuracy": {"eta_max_err_m": eta_err, "eta_err_over_hs": eta_err / HS, "ux_scaled_err": u_err,

@@ -54,6 +54,26 @@ __device__ __forceinline__ float gelu_d(float x) {
     return 0.5f * (1.f + erff(x * 0.70710678118654752f)) + x * 0.39894228040143268f * expf(-0.5f * x * x);
 }
 
+// epilogue of a 32 x 32 accumulator tile (rows m0.., column cn of this lane)
+__device__ __forceinline__ void mm_store(const MmArgs& a, const f32x16& acc, int m0, int cn, int hl) {
+#pragma unroll
+    for (int r = 0; r < 16; ++r) {
+        const int m = m0 + 8 * (r >> 2) + 4 * hl + (r & 3);
+        if (m >= a.M || cn >= a.N) continue;
+        float v = acc[r];
+        const int64_t o = (int64_t)m * a.scm + cn;
+        if (a.epi == 1) {
+            v += a.bias[cn];
+            a.Z[o] = v;
+            a.C[o] = gelu(v);
+        } else if (a.epi == 2) {
+            a.C[o] = v + a.bias[cn];
+        } else {
+            a.C[o] = v;
+        }
+    }
+}
+
 // Block 256 threads = 4 waves, 64 x 64 outputs; wave (w & 1, w >> 1) takes a
 // 32 x 32 tile (one v_mfma_f32_32x32x16_bf16 accumulator).  Operand element j
 // of lane l is row (col) l % 32 at k = k0 + 8 (l / 32) + j; out-of-range
@@ -84,23 +104,42 @@ __global__ __launch_bounds__(256) void k_mm(MmArgs a) {
         }
         acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(A8, B8, acc, 0, 0, 0);
     }
-    const int hl = lane >> 5;
+    mm_store(a, acc, m0, cn, lane >> 5);
+}
+
+// The same product when both operands are contiguous in k (sak = sbk = 1),
+// K is a multiple of 16 and rows start 16-byte aligned: each lane's 8 k
+// values of A and of B come as two 16-byte loads, and the next k step's are
+// requested before this step's product.
+__device__ __forceinline__ bf16x8 pack8(const float4& u, const float4& v) {
+    bf16x8 r;
+    const float f[8] = {u.x, u.y, u.z, u.w, v.x, v.y, v.z, v.w};
 #pragma unroll
-    for (int r = 0; r < 16; ++r) {
-        const int m = m0 + 8 * (r >> 2) + 4 * hl + (r & 3);
-        if (m >= a.M || cn >= a.N) continue;
-        float v = acc[r];
-        const int64_t o = (int64_t)m * a.scm + cn;
-        if (a.epi == 1) {
-            v += a.bias[cn];
-            a.Z[o] = v;
-            a.C[o] = gelu(v);
-        } else if (a.epi == 2) {
-            a.C[o] = v + a.bias[cn];
-        } else {
-            a.C[o] = v;
-        }
+    for (int j = 0; j < 8; ++j) r[j] = __builtin_bit_cast(__bf16, to_bf16(f[j]));
+    return r;
+}
+
+__global__ __launch_bounds__(256) void k_mm_vec(MmArgs a) {
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int m0 = blockIdx.y * 64 + (w & 1) * 32;
+    const int n0 = blockIdx.x * 64 + (w >> 1) * 32;
+    const int rm = m0 + (lane & 31), cn = n0 + (lane & 31);
+    const int kh = (lane >> 5) * 8;
+    // rows past M / columns past N read row 0 / column 0 (in bounds) and are
+    // not stored
+    const float4* pa = reinterpret_cast<const float4*>(a.A + (int64_t)(rm < a.M ? rm : 0) * a.sam + kh);
+    const float4* pb = reinterpret_cast<const float4*>(a.B + (int64_t)(cn < a.N ? cn : 0) * a.sbn + kh);
+    f32x16 acc;
+#pragma unroll
+    for (int r = 0; r < 16; ++r) acc[r] = 0.f;
+    float4 a0 = pa[0], a1 = pa[1], b0 = pb[0], b1 = pb[1];
+    for (int k0 = 16; k0 < a.K; k0 += 16) {
+        const float4 na0 = pa[k0 / 4], na1 = pa[k0 / 4 + 1], nb0 = pb[k0 / 4], nb1 = pb[k0 / 4 + 1];
+        acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(pack8(a0, a1), pack8(b0, b1), acc, 0, 0, 0);
+        a0 = na0; a1 = na1; b0 = nb0; b1 = nb1;
     }
+    acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(pack8(a0, a1), pack8(b0, b1), acc, 0, 0, 0);
+    mm_store(a, acc, m0, cn, lane >> 5);
 }
 
 // X_hat = (X - mean) / std over n rows of d features (row-major)
@@ -227,7 +266,12 @@ int Mlp::reserve(int64_t rows) {
 
 static hipError_t mm(const MmArgs& a, hipStream_t st) {
     if (a.M <= 0 || a.N <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_mm, dim3((unsigned)((a.N + 63) / 64), (unsigned)((a.M + 63) / 64)), dim3(256), 0, st, a);
+    const dim3 grid((unsigned)((a.N + 63) / 64), (unsigned)((a.M + 63) / 64));
+    auto al16 = [](const void* p) { return (reinterpret_cast<uintptr_t>(p) & 15) == 0; };
+    const bool vec = a.sak == 1 && a.sbk == 1 && a.K >= 16 && a.K % 16 == 0 && a.sam % 4 == 0 && a.sbn % 4 == 0 &&
+                     al16(a.A) && al16(a.B);
+    if (vec) hipLaunchKernelGGL(k_mm_vec, grid, dim3(256), 0, st, a);
+    else hipLaunchKernelGGL(k_mm, grid, dim3(256), 0, st, a);
     return hipGetLastError();
 }
 
