@@ -580,7 +580,11 @@ int Grid::init() {
     const int cap = prec == kFP32 ? 8 : 4;
     level_chunk = 1;
     while (level_chunk < d.nl && level_chunk < cap) level_chunk *= 2;
-    if (const char* e = std::getenv("CFDWM_LEVEL_CHUNK")) level_chunk = std::atoi(e);
+    if (const char* e = std::getenv("CFDWM_LEVEL_CHUNK")) {
+        // only the compiled chunk sizes (buffers are sized for the chunk the kernel uses)
+        const int v = std::atoi(e);
+        if (v == 1 || v == 2 || v == 4 || v == 8 || v == 16) level_chunk = v;
+    }
     if (const char* e = std::getenv("CFDWM_PRUNE")) prune_ = std::atoi(e) != 0;
     if (const char* e = std::getenv("CFDWM_QFORM")) qform_ = std::atoi(e) != 0;
     return refill();

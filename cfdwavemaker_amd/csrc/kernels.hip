@@ -961,10 +961,7 @@ static hipError_t launch_levels_t(const SeaDev<R>& S, const LevelJob<R>& J, hipS
     const int64_t n = level_threads(J.nslot, J.ncol, J.g.nl, LC, MODE == 0);
     if (J.nslot * J.ncol <= 0) return hipSuccess;
     if constexpr (MODE == 0 && sizeof(R) == 4 && (LC == 8 || LC == 16)) {
-        if (!std::getenv("CFDWM_NO_WAVES_HINT"))
-            hipLaunchKernelGGL((k_levels_grid32<LC>), dim3(nblk(n), J.nseg), dim3(256), 0, st, S, J);
-        else
-            hipLaunchKernelGGL((k_levels<R, LC, MODE>), dim3(nblk(n), J.nseg), dim3(256), 0, st, S, J);
+        hipLaunchKernelGGL((k_levels_grid32<LC>), dim3(nblk(n), J.nseg), dim3(256), 0, st, S, J);
     } else {
         hipLaunchKernelGGL((k_levels<R, LC, MODE>), dim3(nblk(n), J.nseg), dim3(256), 0, st, S, J);
     }
