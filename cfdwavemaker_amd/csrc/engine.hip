@@ -248,6 +248,27 @@ const Segment* Sea::segments(bool direct, int nseg) {
     return r;
 }
 
+const Segment* Sea::comp_segments(int nseg) {
+    auto it = cseg_cache_.find(nseg);
+    if (it != cseg_cache_.end()) return (const Segment*)it->second->get();
+    const int N = tab.comp.n;
+    std::vector<Segment> sg(nseg);
+    for (int s = 0; s < nseg; ++s) {
+        Segment& g = sg[s];
+        g.own_lo = (int)((int64_t)N * s / nseg);
+        g.own_hi = (int)((int64_t)N * (s + 1) / nseg);
+        g.rlo = g.rhi = g.own_lo;
+        g.mstart = g.mend = 0;
+        g.pad0 = g.pad1 = 0;
+    }
+    auto buf = std::make_unique<DevBuf>();
+    if (buf->upload(sg.data(), sg.size() * sizeof(Segment), Engine::get().stream())) return nullptr;
+    if (hipStreamSynchronize(Engine::get().stream()) != hipSuccess) return nullptr;
+    const Segment* r = (const Segment*)buf->get();
+    cseg_cache_[nseg] = std::move(buf);
+    return r;
+}
+
 namespace {
 // Irregular::freq_ramp (Irregular.cpp:200-204), host side
 double freq_ramp_host(double t, double rt, double ts, double du) {
@@ -612,12 +633,16 @@ int Grid::fill_slots(int nslot, const double* slot_t, const int* slot_dst, bool 
         const int rc = qform_fill(nslot, slot_t, C.ncol, C.cols, C, L);
         if (rc < 0) return rc;
         qf_used = rc == 1;
-        if (qf_used) C.nseg = 1;             // pair sums done on the matrix cores
     }
-    C.seg = sea->segments(false, C.nseg);
+    // with the pair sums on the matrix cores the column and level passes are
+    // O(N) per column: split them over component segments when the grid alone
+    // does not fill the GPU
+    const int64_t ncomp = sea->tab.comp.n;
+    if (qf_used) C.nseg = pick_nseg(C.ncol * nslot, 64 * ncomp);
+    C.seg = qf_used ? sea->comp_segments(C.nseg) : sea->segments(false, C.nseg);
     if (!C.seg) return -3;
     const int64_t lthr = level_threads(nslot, C.ncol, g.nl, level_chunk, true);
-    const int lseg = qf_used ? 1 : pick_nseg(lthr, nrec);
+    const int lseg = pick_nseg(lthr, qf_used ? 64 * ncomp : nrec);
     {
         const size_t need = std::max((size_t)C.nseg * nslot * C.ncol * 6, (size_t)lseg * lthr * (3 * level_chunk + 6));
         if ((C.nseg > 1 || lseg > 1) && partial_.alloc(need * sizeof(double))) return -3;
@@ -632,7 +657,7 @@ int Grid::fill_slots(int nslot, const double* slot_t, const int* slot_dst, bool 
     L.o_u = (R*)f_[4].get(); L.o_v = (R*)f_[5].get(); L.o_w = (R*)f_[6].get();
     L.flags = EV_LIN | EV_SEC | EV_TAYLOR;
     L.nseg = lseg;
-    L.seg = sea->segments(false, lseg);
+    L.seg = qf_used ? sea->comp_segments(lseg) : sea->segments(false, lseg);
     if (!L.seg) return -3;
     if (lseg > 1) L.partial = (double*)partial_.get();
     if (prune_ && !qf_used) {

@@ -261,6 +261,9 @@ public:
     const SeaDev<R>& dev() const;
     // balanced split of a pair table into nseg segments (cached on the device)
     const Segment* segments(bool direct, int nseg);
+    // nseg segments of the components alone (no pair rows), for passes whose
+    // pair sums come from the quadratic forms
+    const Segment* comp_segments(int nseg);
     int64_t table_records(bool direct) const {
         return direct ? (int64_t)tab.direct.rec.size() : (int64_t)tab.filt.rec.size();
     }
@@ -296,6 +299,7 @@ public:
 private:
     DevBuf ph_, cr_, act_, frow_, fmhi_, fuvw_, fx_, drow_, duvw_, dx_;
     std::map<std::pair<bool, int>, std::unique_ptr<DevBuf>> seg_cache_;
+    std::map<int, std::unique_ptr<DevBuf>> cseg_cache_;
     std::unique_ptr<QfTables> qf_;
     std::unique_ptr<SmitDev> smit_;
     SeaDev<double> d64_{};

@@ -457,8 +457,9 @@ __device__ __forceinline__ void levels_body(const SeaDev<R>& S, const LevelJob<R
         if (WP && direct) P2 += (double)pr;
     }
     if constexpr (GRID) {
-        if (J.qf) {
+        if (J.qf && blockIdx.y == 0) {
             // pair sums from the quadratic forms of the separable depth basis
+            // (once per point: segment 0)
             R su[LC], sv[LC], sw[LC];
 #pragma unroll
             for (int l = 0; l < LC; ++l) su[l] = sv[l] = sw[l] = 0;
