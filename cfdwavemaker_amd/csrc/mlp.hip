@@ -142,6 +142,121 @@ __global__ __launch_bounds__(256) void k_mm_vec(MmArgs a) {
     mm_store(a, acc, m0, cn, lane >> 5);
 }
 
+// ------------------------------------------------------------ fused inference
+// The whole network for 64 rows per block: the standardised input and each
+// hidden layer's output stay in LDS as bf16 (the operand rounding the
+// layer-wise kernels apply when they load them), weights come from the FP32
+// master parameters, and only the unstandardised outputs reach HBM.  Each
+// wave takes 32-column blocks of a layer in turn, both 32-row halves at once
+// (k order, products and epilogue as k_mm / k_mm_vec), so the outputs equal
+// the layer-wise path's bit for bit.  Limits: at most kFiL layers, widths up to kFiW.
+constexpr int kFiR = 64, kFiW = 256, kFiS = kFiW + 8, kFiL = 8;   // rows, max width, LDS row stride
+
+struct MlpFused {
+    int64_t n = 0;
+    int L = 0, nin = 0, nout = 0;
+    int din[kFiL] = {}, dout[kFiL] = {};
+    const float* W[kFiL] = {};            // W_l[j][k], dout x din
+    const float* b[kFiL] = {};
+    const float *mu_in = nullptr, *sd_in = nullptr, *mu_out = nullptr, *sd_out = nullptr;
+    const float* X = nullptr;             // n x nin
+    float* Y = nullptr;                   // n x nout
+};
+
+__global__ __launch_bounds__(256) void k_mlp_fused(MlpFused a) {
+    __shared__ __attribute__((aligned(16))) uint16_t H[2][kFiR * kFiS];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int64_t r0 = (int64_t)blockIdx.x * kFiR;
+    // layer 0 input, standardised (as k_standardise), zero past nin up to a multiple of 16
+    const int k0p = (a.nin + 15) & ~15;
+    for (int idx = threadIdx.x; idx < kFiR * k0p; idx += 256) {
+        const int r = idx / k0p, c = idx - r * k0p;
+        float v = 0.f;
+        if (c < a.nin && r0 + r < a.n) v = (a.X[(r0 + r) * a.nin + c] - a.mu_in[c]) / a.sd_in[c];
+        H[0][r * kFiS + c] = to_bf16(v);
+    }
+    __syncthreads();
+    const int kh = (lane >> 5) * 8, hl = lane >> 5;
+    int cur = 0;
+    for (int l = 0; l < a.L; ++l) {
+        const int K = a.din[l], N = a.dout[l];
+        const int Kp = (K + 15) & ~15;
+        const bool last = l == a.L - 1;
+        const float* __restrict__ W = a.W[l];
+        const float* __restrict__ bias = a.b[l];
+        // 16-byte weight loads when every k step lies inside the row (K a multiple of 16)
+        const bool wvec = (K & 15) == 0 && (reinterpret_cast<uintptr_t>(W) & 15) == 0;
+        const int nct = (N + 31) / 32;
+        // column block ct of both 32-row halves: two accumulators sharing each B operand
+        for (int ct = w; ct < nct; ct += 4) {
+            const int col = ct * 32 + (lane & 31);
+            const float* __restrict__ wc = W + (int64_t)(col < N ? col : 0) * K;
+            const uint16_t* h0 = &H[cur][(lane & 31) * kFiS + kh];
+            const uint16_t* h1 = h0 + 32 * kFiS;
+            f32x16 acc0, acc1;
+#pragma unroll
+            for (int r = 0; r < 16; ++r) acc0[r] = acc1[r] = 0.f;
+            if (wvec) {
+                // 16-byte weight loads, the next k step's requested before this step's products
+                const float4* q = reinterpret_cast<const float4*>(wc + kh);
+                float4 u = q[0], v = q[1];
+                for (int k0 = 0; k0 < Kp; k0 += 16) {
+                    const int kn = k0 + 16 < Kp ? k0 + 16 : k0;
+                    const float4 un = q[kn / 4], vn = q[kn / 4 + 1];
+                    const float f[8] = {u.x, u.y, u.z, u.w, v.x, v.y, v.z, v.w};
+                    bf16x8 B8;
+#pragma unroll
+                    for (int j = 0; j < 8; ++j) B8[j] = __builtin_bit_cast(__bf16, to_bf16(col < N ? f[j] : 0.f));
+                    const bf16x8 A0 = *reinterpret_cast<const bf16x8*>(h0 + k0);
+                    const bf16x8 A1 = *reinterpret_cast<const bf16x8*>(h1 + k0);
+                    acc0 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(A0, B8, acc0, 0, 0, 0);
+                    acc1 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(A1, B8, acc1, 0, 0, 0);
+                    u = un;
+                    v = vn;
+                }
+            } else {
+                for (int k0 = 0; k0 < Kp; k0 += 16) {
+                    bf16x8 B8;
+#pragma unroll
+                    for (int j = 0; j < 8; ++j) {
+                        const int k = k0 + kh + j;
+                        B8[j] = __builtin_bit_cast(__bf16, to_bf16(k < K && col < N ? wc[k] : 0.f));
+                    }
+                    const bf16x8 A0 = *reinterpret_cast<const bf16x8*>(h0 + k0);
+                    const bf16x8 A1 = *reinterpret_cast<const bf16x8*>(h1 + k0);
+                    acc0 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(A0, B8, acc0, 0, 0, 0);
+                    acc1 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(A1, B8, acc1, 0, 0, 0);
+                }
+            }
+            if (col >= N) continue;
+            const float bc = bias[col];
+#pragma unroll
+            for (int rt = 0; rt < 2; ++rt) {
+#pragma unroll
+                for (int r = 0; r < 16; ++r) {
+                    const int row = rt * 32 + 8 * (r >> 2) + 4 * hl + (r & 3);
+                    const float v = (rt ? acc1[r] : acc0[r]) + bc;
+                    if (!last) {
+                        H[cur ^ 1][row * kFiS + col] = to_bf16(gelu(v));
+                    } else if (r0 + row < a.n) {
+                        a.Y[(r0 + row) * a.nout + col] = v * a.sd_out[col] + a.mu_out[col];
+                    }
+                }
+            }
+        }
+        if (!last) {
+            // columns past N read as zeros by the next layer's last k step
+            const int Np = (N + 15) & ~15;
+            for (int idx = threadIdx.x; idx < kFiR * (Np - N); idx += 256) {
+                const int r = idx / (Np - N), c = N + idx - r * (Np - N);
+                H[cur ^ 1][r * kFiS + c] = 0;
+            }
+        }
+        __syncthreads();
+        cur ^= 1;
+    }
+}
+
 // X_hat = (X - mean) / std over n rows of d features (row-major)
 __global__ void k_standardise(int64_t n, int d, const float* __restrict__ X, const float* __restrict__ mu,
                               const float* __restrict__ sd, float* __restrict__ Y) {
@@ -443,6 +558,23 @@ int cfdwm_mlp_forward_device(cfdwm_mlp* h, int64_t n, const float* X, float* Y) 
     if (!h || n < 0 || (n && (!X || !Y))) return -1;
     Mlp& M = h->m;
     hipStream_t st = Engine::get().stream();
+    // large batches: the fused kernel (256 blocks of 64 rows fill the GPU);
+    // smaller ones run layer by layer, whose 64 x 64 output tiles give more blocks
+    bool fits = M.layers() <= kFiL && M.nin <= kFiW;
+    for (int l = 0; l < M.layers(); ++l) fits = fits && M.dout[l] <= kFiW;
+    if (fits && n >= 256 * kFiR) {
+        MlpFused F;
+        F.n = n; F.L = M.layers(); F.nin = M.nin; F.nout = M.nout;
+        for (int l = 0; l < F.L; ++l) {
+            F.din[l] = M.din[l]; F.dout[l] = M.dout[l];
+            F.W[l] = M.P() + M.woff[l]; F.b[l] = M.P() + M.boff[l];
+        }
+        F.mu_in = M.nin_mu(); F.sd_in = M.nin_sd(); F.mu_out = M.nout_mu(); F.sd_out = M.nout_sd();
+        F.X = X; F.Y = Y;
+        hipLaunchKernelGGL(k_mlp_fused, dim3((unsigned)((n + kFiR - 1) / kFiR)), dim3(256), 0, st, F);
+        CFDWM_HIP(hipGetLastError());
+        return 0;
+    }
     const int64_t chunk = 65536;
     if (M.reserve(std::min(n, chunk))) return -3;
     float* A = (float*)M.act.get();

@@ -96,3 +96,26 @@ def test_gpu_training_reduces_loss():
     print(f"loss before {l0:.3e} after {l1:.3e}; curve {curve[curve > 0]}", flush=True)
     assert l1 < 0.05 * l0
     m.close()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("shape", [(29, 256, 3, 65), (7, 96, 2, 5), (30, 24, 2, 40)])
+def test_fused_inference_equals_layerwise(shape):
+    """Batches of 16,384 rows or more run the whole network in one fused
+    kernel; smaller ones layer by layer.  Both give the same outputs bit for
+    bit (same operand rounding, k order and epilogue)."""
+    import torch
+    from cfdwavemaker_amd.mlp import Mlp
+    nin, width, nh, nout = shape
+    dims, p = _net(nin, width, nh, nout, 8)
+    n = 20000
+    X, Y, norm = _data(n, nin, nout, 9)
+    m = Mlp(nin, width, nh, nout)
+    m.set_params(p)
+    m.set_norm(*norm)
+    Xd = torch.tensor(X, dtype=torch.float32, device="cuda")
+    big = m.forward(Xd).cpu().numpy()
+    small = np.concatenate([m.forward(Xd[a:a + 5000]).cpu().numpy() for a in range(0, n, 5000)])
+    assert np.all(np.isfinite(big))
+    assert np.array_equal(big, small), float(np.max(np.abs(big - small)))
+    m.close()
