@@ -89,22 +89,38 @@ __global__ __launch_bounds__(256) void k_surr_linear(SeaDev<R> S, SurrJob J) {
             zt[l] = z > 0. ? z : 0.;
             U[l] = V[l] = W[l] = P[l] = 0.;
         }
-        for (int i = 0; i < S.n; ++i) {
-            const PhaseRec ph = S.ph[i];
-            const CompRec<R>& c = S.cr[i];
-            R ci, si;
-            s_cs(ph.k * (ph.cth * dx + ph.sth * dy) - ph.omega * t + ph.ph0, ci, si);
-            const R ri = (R)s_ramp(t, ph.rt, ph.ts, ph.du);
-            const R cr = ci * ri, sr = si * ri;
+        // FP32: the terms of 8 components meet in FP32 before each FP64 fold
+        // (the outputs are FP32); FP64: every term folds into the totals
+        constexpr int BLK = sizeof(R) == 4 ? 8 : 1;
+        for (int i0 = 0; i0 < S.n; i0 += BLK) {
+            R Ub[8], Vb[8], Wb[8], Pb[8];
+#pragma unroll
+            for (int l = 0; l < 8; ++l) Ub[l] = Vb[l] = Wb[l] = Pb[l] = 0;
+            const int i1 = i0 + BLK < S.n ? i0 + BLK : S.n;
+            for (int i = i0; i < i1; ++i) {
+                const PhaseRec ph = S.ph[i];
+                const CompRec<R>& c = S.cr[i];
+                R ci, si;
+                s_cs(ph.k * (ph.cth * dx + ph.sth * dy) - ph.omega * t + ph.ph0, ci, si);
+                const R ri = (R)s_ramp(t, ph.rt, ph.ts, ph.du);
+                const R cr = ci * ri, sr = si * ri;
+#pragma unroll
+                for (int l = 0; l < 8; ++l) {
+                    R E, G;
+                    s_prof(c.pk, c.pg, zp[l], E, G);
+                    const R ch = E + G, sh = E - G;
+                    Ub[l] += c.lu * cr * ch;
+                    Vb[l] += c.lv * cr * ch;
+                    Wb[l] += c.lw * sr * sh;
+                    Pb[l] += c.lp * cr * ch;
+                }
+            }
 #pragma unroll
             for (int l = 0; l < 8; ++l) {
-                R E, G;
-                s_prof(c.pk, c.pg, zp[l], E, G);
-                const R ch = E + G, sh = E - G;
-                U[l] += (double)(c.lu * cr * ch);
-                V[l] += (double)(c.lv * cr * ch);
-                W[l] += (double)(c.lw * sr * sh);
-                P[l] += (double)(c.lp * cr * ch);
+                U[l] += (double)Ub[l];
+                V[l] += (double)Vb[l];
+                W[l] += (double)Wb[l];
+                P[l] += (double)Pb[l];
             }
         }
 #pragma unroll
