@@ -271,6 +271,80 @@ def batched_leg(nstates: int, nsample_err: int) -> dict:
             "per_state_err_vs_fp64": errs}
 
 
+def abi_worker(out_path: str) -> None:
+    """The batched C-ABI entries on the s-grid of the N=1024 workload, in a
+    process of their own (the ABI keeps one process-global wave field):
+    waveinput.dat with the same sea and [lsgrid] block, FP32, then 1e6 points
+    (eta at z = 0 and u, v, w at z) through wave_EvalPoints (host arrays,
+    PCIe copies included) and wave_EvalPointsDevice (device arrays)."""
+    import ctypes as C
+    import tempfile
+    import torch
+    from cfdwavemaker_amd._lib import load
+    from cfdwavemaker_amd.seastate import config_sea, write_waveinput
+    n, reps = 1_000_000, 5
+    rng = np.random.default_rng(7)
+    x = rng.uniform(DOMAIN[0], DOMAIN[1], n)
+    y = rng.uniform(DOMAIN[2], DOMAIN[3], n)
+    z = rng.uniform(-DEPTH, 0.0, n)
+    t = np.full(n, T0 + 0.25)
+    res = {}
+    with tempfile.TemporaryDirectory() as wd:
+        write_waveinput(os.path.join(wd, "waveinput.dat"), config_sea(3),
+                        lsgrid=dict(bounds=DOMAIN, nx=NX, ny=NY, nl=NL, t0=T0, dt=DT, tan_a=TAN_A, tan_b=TAN_B))
+        os.environ["CFDWM_PRECISION"] = "fp32"
+        L = load(os.path.join(ROOT, "cfdwavemaker_amd", "libcfdwm.so"))
+        cwd = os.getcwd()
+        os.chdir(wd)
+        t0 = time.perf_counter()
+        rc = L.wave_Initialize()
+        res["initialize_s"] = time.perf_counter() - t0
+        os.chdir(cwd)
+        if rc != 0:
+            raise RuntimeError(f"wave_Initialize returned {rc}")
+    flags = 0x1 | 0x2 | 0x4 | 0x8
+    host = [np.zeros(n) for _ in range(4)]
+    P = C.POINTER(C.c_double)
+    ptr = lambda a: a.ctypes.data_as(P)
+
+    def run_host():
+        return L.wave_EvalPoints(n, ptr(x), ptr(y), ptr(z), ptr(t), *[ptr(h) for h in host], None, flags)
+    dev = torch.device("cuda:0")
+    X, Y, Z, T = (torch.tensor(a, dtype=torch.float64, device=dev) for a in (x, y, z, t))
+    D = [torch.empty(n, dtype=torch.float64, device=dev) for _ in range(4)]
+
+    def run_dev():
+        return L.wave_EvalPointsDevice(n, X.data_ptr(), Y.data_ptr(), Z.data_ptr(), T.data_ptr(),
+                                       *[d.data_ptr() for d in D], None, flags)
+    for name, fn in (("host_arrays", run_host), ("device_arrays", run_dev)):
+        if fn() != 0:
+            raise RuntimeError(f"{name}: batched entry failed")
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            fn()
+        torch.cuda.synchronize()
+        el = (time.perf_counter() - t0) / reps
+        res[name] = {"points_per_s": n / el, "ms": el * 1e3}
+    res["bitwise_equal"] = bool(all(np.array_equal(h, d.cpu().numpy()) for h, d in zip(host, D)))
+    res["workload"] = (f"wave_Initialize on the {NX}x{NY}x{NL} s-grid of the N=1024 sea (FP32), then "
+                       f"{n} points: eta at z = 0 and u, v, w at z, FP64 arrays")
+    with open(out_path, "w") as f:
+        json.dump(res, f)
+
+
+def abi_leg() -> dict:
+    import tempfile
+    with tempfile.TemporaryDirectory() as td:
+        out = os.path.join(td, "abi.json")
+        r = subprocess.run([sys.executable, os.path.abspath(__file__), "--abi-worker", out],
+                           stdout=subprocess.DEVNULL, stderr=subprocess.PIPE, text=True, timeout=600)
+        if r.returncode != 0:
+            return {"error": r.stderr[-500:]}
+        with open(out) as f:
+            return json.load(f)
+
+
 def update_leg(engine, grid, updates: int) -> dict:
     """Time stepping of the filled grid (CheckTime / update_lsgrid): each
     force_update advances the grid by one time level and computes that new
@@ -368,6 +442,8 @@ def main():
     ap.add_argument("--cpu-cols", type=int, default=12)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-worker", nargs=2, default=None)
+    ap.add_argument("--abi-worker", default=None)
+    ap.add_argument("--no-abi", action="store_true", help="skip the batched C-ABI leg")
     ap.add_argument("--rocprof-summary", nargs=2, metavar=("DB", "COMMAND"), default=None,
                     help="print a markdown summary of a rocprofv3 results database and exit")
     ap.add_argument("--no-surrogate", action="store_true", help="skip the learned-surrogate leg")
@@ -384,6 +460,9 @@ def main():
         return
     if a.cpu_worker:
         cpu_worker(int(a.cpu_worker[0]), a.cpu_worker[1])
+        return
+    if a.abi_worker:
+        abi_worker(a.abi_worker)
         return
     if a.gpus != 1:
         print(f"bench.py runs on one GPU: the s-grid fill shards by columns across GPUs "
@@ -508,6 +587,7 @@ def main():
     bat = batched_leg(a.batched_states, 3) if a.batched_states > 0 else None
     dirl = None if a.no_direct else direct_leg(a.steps)
     smitl = None if a.no_direct else direct_leg(a.steps, smit=True)
+    abil = None if a.no_abi else abi_leg()
 
     cpu = None
     if not a.no_cpu:
@@ -551,6 +631,7 @@ def main():
         "direct": dirl,
         "direct_smit": smitl,
         "update": upd,
+        "abi_batched": abil,
     }
     print(json.dumps(line), flush=True)
 

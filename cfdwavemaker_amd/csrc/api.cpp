@@ -89,6 +89,13 @@ struct DevPoints {
 };
 DevPoints g_dev;
 
+// Device staging of wave_EvalPoints on the s-grid (inputs, outputs).
+struct StagePoints {
+    std::mutex mu;
+    DevBuf in, out;
+};
+StagePoints g_stage;
+
 RampDev ramp_dev(const RampCfg& r) {
     RampDev d;
     d.init = r.init ? 1 : 0;
@@ -710,6 +717,29 @@ static int eval_points_impl(int64_t n, const double* x, const double* y, const d
         if (fin(eta, he) || fin(u, hu) || fin(v, hv) || fin(w, hw) || fin(p, hp)) return -3;
         return 0;
     }
+    if (G.wavetype == 5 && !dev) {
+        // host arrays: one copy up, the device path below, one copy down
+        // (the device path's results equal the host computation bit for bit)
+        std::lock_guard<std::mutex> lk(g_stage.mu);
+        const size_t nb = (size_t)n * sizeof(double);
+        if (g_stage.in.alloc(4 * nb) || g_stage.out.alloc(5 * nb)) return -3;
+        double* din = (double*)g_stage.in.get();
+        double* dout = (double*)g_stage.out.get();
+        double *dx = din, *dy = din + n, *dt = din + 2 * n, *dz = z ? din + 3 * n : nullptr;
+        CFDWM_HIP(hipMemcpy(dx, x, nb, hipMemcpyHostToDevice));
+        CFDWM_HIP(hipMemcpy(dy, y, nb, hipMemcpyHostToDevice));
+        CFDWM_HIP(hipMemcpy(dt, t, nb, hipMemcpyHostToDevice));
+        if (z) CFDWM_HIP(hipMemcpy(dz, z, nb, hipMemcpyHostToDevice));
+        double* host_out[5] = {eta, u, v, w, p};
+        double* dev_out[5];
+        for (int c = 0; c < 5; ++c) dev_out[c] = host_out[c] ? dout + c * n : nullptr;
+        if (int rc = eval_points_impl(n, dx, dy, dz, dt, dev_out[0], dev_out[1], dev_out[2], dev_out[3], dev_out[4],
+                                      flags, true))
+            return rc;
+        for (int c = 0; c < 5; ++c)
+            if (host_out[c]) CFDWM_HIP(hipMemcpy(host_out[c], dev_out[c], nb, hipMemcpyDeviceToHost));
+        return 0;
+    }
     if (G.wavetype == 5 && dev) {
         // device-resident s-grid queries: velocities at the clamped z, the
         // elevation from a query at z = 0, then the ramps
@@ -742,48 +772,6 @@ static int eval_points_impl(int64_t n, const double* x, const double* y, const d
         }
         CFDWM_HIP(launch_points_finish(F, st));
         CFDWM_HIP(hipStreamSynchronize(st));
-        return 0;
-    }
-    if (G.wavetype == 5) {
-        // s-grid: eta at z = 0 (wave_SurfElev) and velocities at z (wave_Velo*)
-        std::vector<double> hx(x, x + (dev ? 0 : n)), hy, ht, hz;
-        if (dev) {
-            hx.resize(n); hy.resize(n); ht.resize(n);
-            CFDWM_HIP(hipMemcpy(hx.data(), x, n * 8, hipMemcpyDeviceToHost));
-            CFDWM_HIP(hipMemcpy(hy.data(), y, n * 8, hipMemcpyDeviceToHost));
-            CFDWM_HIP(hipMemcpy(ht.data(), t, n * 8, hipMemcpyDeviceToHost));
-            if (z) { hz.resize(n); CFDWM_HIP(hipMemcpy(hz.data(), z, n * 8, hipMemcpyDeviceToHost)); }
-        } else {
-            hy.assign(y, y + n); ht.assign(t, t + n);
-            if (z) hz.assign(z, z + n);
-        }
-        std::vector<double> o4(4 * n), o0;
-        Grid& g = *G.cgrid->g;
-        if (u || v || w) {
-            std::vector<double> zc(n);
-            for (int64_t q = 0; q < n; ++q) zc[q] = std::max(-G.depth, hz[q]);
-            if (int rc = g.query(n, ht.data(), hx.data(), hy.data(), zc.data(), o4.data(), nullptr, nullptr)) return rc;
-        }
-        if (eta) {
-            std::vector<double> z0(n, 0.0);
-            o0.resize(4 * n);
-            if (int rc = g.query(n, ht.data(), hx.data(), hy.data(), z0.data(), o0.data(), nullptr, nullptr)) return rc;
-        }
-        if (p) { set_error("dynamic pressure is defined for wave type 1 only (CFDwavemaker.cpp:1800-1813)"); }
-        std::vector<double> col(n);
-        auto fin = [&](double* dst, const std::vector<double>& src, int c) -> int {
-            if (!dst) return 0;
-            for (int64_t q = 0; q < n; ++q) col[q] = src[4 * q + c] * G.ramp(ht[q], hx[q], hy[q]);
-            if (dev) CFDWM_HIP(hipMemcpy(dst, col.data(), n * 8, hipMemcpyHostToDevice));
-            else std::memcpy(dst, col.data(), n * 8);
-            return 0;
-        };
-        if (fin(eta, o0, 0) || fin(u, o4, 1) || fin(v, o4, 2) || fin(w, o4, 3)) return -3;
-        if (p) {
-            std::vector<double> zero(n, 0.0);
-            if (dev) CFDWM_HIP(hipMemcpy(p, zero.data(), n * 8, hipMemcpyHostToDevice));
-            else std::memcpy(p, zero.data(), n * 8);
-        }
         return 0;
     }
     set_error("wave type not supported by this build");
