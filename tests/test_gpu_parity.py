@@ -39,6 +39,20 @@ def test_point_kinds_fp64_vs_golden(eng, case):
         assert err <= FP64_TOL, (case, kind, err)
 
 
+def test_linear_section_vs_golden(eng):
+    """The reference's CPU linear case: N=256, order 0, a 64 x 32 vertical
+    section at t = 0 (eta, u, w, dp); FP64 to the golden vectors, FP32 to the
+    FP32 bound."""
+    g = golden("linear.npz")
+    t, x, y, z = g["L__pts"]
+    for prec, tol in ((eng.FP64, FP64_TOL), (eng.FP32, 1e-5)):
+        sea = eng.Sea(sea_from_golden(g, "L"), precision=prec)
+        for kind in ("eta", "u", "w", "dp"):
+            err = scaled_err(sea.eval(kind, t, x, y, z), g[f"L__{kind}"])
+            print(f"linear prec {prec} {kind:4s} scaled err {err:.2e}", flush=True)
+            assert err <= tol, (prec, kind, err)
+
+
 @pytest.mark.parametrize("ign", [0, 1])
 def test_sgrid_fp64_vs_golden(eng, ign):
     g = golden("grid.npz")

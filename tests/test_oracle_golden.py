@@ -25,6 +25,16 @@ def test_point_functions(oracle, case):
         assert scaled_err(got, ref) <= TOL, (case, kind, scaled_err(got, ref))
 
 
+def test_linear_section(oracle):
+    """The reference's CPU linear case (order 0: no clamp, u1(z - swl))."""
+    g = golden("linear.npz")
+    sea = oracle.sea(sea_from_golden(g, "L"))
+    t, x, y, z = g["L__pts"]
+    for kind in ("eta", "u", "w", "dp"):
+        err = scaled_err(sea.eval(kind, t, x, y, z), g[f"L__{kind}"])
+        assert err <= TOL, (kind, err)
+
+
 @pytest.mark.parametrize("case", CASES)
 def test_pair_cache(oracle, case):
     g = golden("pairs.npz")
