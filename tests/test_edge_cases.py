@@ -73,3 +73,22 @@ def test_query_sizes_around_cell_order_threshold(n):
     big = g.query(*pts)
     small = np.concatenate([g.query(*(q[a:a + 10000] for q in pts)) for a in range(0, n, 10000)])
     assert np.array_equal(big, small)
+
+
+def test_fill_in_several_chunks_matches_single_chunk():
+    """FP32 fills whose Z operands exceed one chunk (2 GiB: 131,072 column
+    slots at N = 2048) are processed in point chunks; the shared columns of a
+    200 x 200 grid (two chunks) and a 40 x 40 grid (one) agree."""
+    from cfdwavemaker_amd import engine as eng
+    from cfdwavemaker_amd.seastate import make_sea
+    s = eng.Sea(make_sea(2048, 4.0, 10.0, 50.0, seed=21).as_dict(), precision=eng.FP32, with_direct=False)
+    dx = 4.0
+    big = s.grid((0.0, 199 * dx, 0.0, 199 * dx), 200, 200, 4, 0.0, 0.5, 0.7, 1.5, 50.0)
+    assert big.qform_info() is not None
+    small = s.grid((0.0, 39 * dx, 0.0, 39 * dx), 40, 40, 4, 0.0, 0.5, 0.7, 1.5, 50.0)
+    eb = big.export("ETA").reshape(4, 200, 200)[:, :40, :40]
+    es = small.export("ETA").reshape(4, 40, 40)
+    assert float(np.max(np.abs(eb - es))) <= 1e-6 * float(np.max(np.abs(es)))
+    ub = big.export("UX").reshape(4, 200, 200, 4)[:, :40, :40, :]
+    us = small.export("UX").reshape(4, 40, 40, 4)
+    assert float(np.max(np.abs(ub - us))) <= 1e-6 * float(np.max(np.abs(us)))
