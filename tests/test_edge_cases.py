@@ -92,3 +92,19 @@ def test_fill_in_several_chunks_matches_single_chunk():
     ub = big.export("UX").reshape(4, 200, 200, 4)[:, :40, :40, :]
     us = small.export("UX").reshape(4, 40, 40, 4)
     assert float(np.max(np.abs(ub - us))) <= 1e-6 * float(np.max(np.abs(us)))
+
+
+def test_point_batch_eta2_in_several_chunks():
+    """FP32 point batches take eta2 from the MFMA form in chunks of 131,072
+    points at N = 2048: one batch of 150,000 points (two chunks) agrees with
+    three batches of 50,000 (one chunk each)."""
+    from cfdwavemaker_amd import engine as eng
+    from cfdwavemaker_amd.seastate import make_sea
+    s = eng.Sea(make_sea(2048, 4.0, 10.0, 50.0, seed=22).as_dict(), precision=eng.FP32, with_direct=False)
+    rng = np.random.default_rng(22)
+    n = 150_000
+    t, x, y = rng.uniform(0, 20, n), rng.uniform(0, 500, n), rng.uniform(-100, 100, n)
+    big = s.eval("eta2", t, x, y)
+    small = np.concatenate([s.eval("eta2", t[a:a + 50000], x[a:a + 50000], y[a:a + 50000])
+                            for a in range(0, n, 50000)])
+    assert float(np.max(np.abs(big - small))) <= 1e-6 * float(np.max(np.abs(small)))
