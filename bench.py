@@ -28,6 +28,7 @@ sys.path.insert(0, ROOT)
 
 PEAK = {"fp32": 157.3e12, "fp64": 78.6e12}   # MI355X vector peaks (FLOP/s)
 PEAK_MFMA_BF16 = 2.5e15                        # MI355X dense bf16 matrix peak (FLOP/s)
+PEAK_SCLK_MHZ = 2400.0                         # its engine clock: 256 CUs x 4 x 1024 FLOP/clk x 2.4 GHz
 PEAK_HBM = 8.0e12                              # MI355X HBM3E (B/s)
 # algorithmic work of the secondary kernels: a spline query moves 12 B in + 16 B
 # out + its share of one pass over the packed grid (128 MB over 1e7 queries);
@@ -548,6 +549,17 @@ def main():
                 "traffic_source": "profiles/*_fetch_size.md + *_write_size.md (rocprofv3 --pmc, calibrated)",
                 "avg_launch_ms": avg * 1e3, "flop_per_launch": flop_launch,
                 "forms": qfi["nform"], "depth_basis_rank": qfi["rank"], "depth_basis_err": qfi["basis_err"]}
+        # shader clock during the forms launch, from one more fill (untimed)
+        # with the in-kernel clock measurement on; the dense peak assumes the
+        # peak engine clock, and MFMA power with these operands lowers it
+        grid.qform_clock(True)
+        grid.refill()
+        engine.sync()
+        sclk = grid.qform_clock(False)
+        if sclk:
+            roof["sclk_mhz"] = sclk
+            roof["peak_sclk_mhz"] = PEAK_SCLK_MHZ
+            roof["frac_at_sclk"] = roof["frac"] * PEAK_SCLK_MHZ / sclk
     else:
         lev_ms, lev_n = prof.get("k_levels_grid", (float("nan"), 1))
         lev_avg = lev_ms / max(lev_n, 1) * 1e-3

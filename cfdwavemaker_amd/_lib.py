@@ -37,6 +37,7 @@ BATCH_API = [
     "cfdwm_mlp_loss_grad_device", "cfdwm_mlp_train_device", "cfdwm_surrogate_linear", "cfdwm_surrogate_add",
     "cfdwm_grids_refill_batched", "cfdwm_grid_release_workspace", "wave_PrefetchPoints", "wave_PrefetchClear",
     "wave_PrefetchStats", "cfdwm_eval_fields", "cfdwm_grid_write_vtu", "cfdwm_sea_set_vertical_theory",
+    "cfdwm_grid_qform_clock",
 ]
 
 
@@ -142,6 +143,8 @@ def load(path: str | None = None) -> C.CDLL:
     L.cfdwm_prof_read.argtypes = [C.c_char_p, _dp, C.POINTER(_i64), C.c_int, C.c_int]
     L.cfdwm_grid_qform_info.restype = C.c_int
     L.cfdwm_grid_qform_info.argtypes = [C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_int), _dp, _dp]
+    L.cfdwm_grid_qform_clock.restype = C.c_int
+    L.cfdwm_grid_qform_clock.argtypes = [C.c_void_p, C.c_int, _dp]
     L.cfdwm_depth_basis.restype = C.c_int
     L.cfdwm_depth_basis.argtypes = [_d, _d, _d, C.c_int, _dp, _dp, _dp]
     _fp = C.POINTER(C.c_float)

@@ -994,6 +994,12 @@ int cfdwm_grid_qform_info(cfdwm_grid* grid, int* nform, int* rank, double* mfma_
     if (basis_err) *basis_err = g.qf_basis_err;
     return g.qf_used ? 1 : 0;
 }
+int cfdwm_grid_qform_clock(cfdwm_grid* grid, int enable, double* mhz) {
+    if (!grid) { set_error("bad arguments"); return -1; }
+    const int r = grid->g->qform_clock(mhz);
+    grid->g->qform_clock_enable(enable != 0);
+    return r;
+}
 int cfdwm_grid_write_vtu(cfdwm_grid* grid, const char* path, int last, const char* time_label) {
     if (!grid || !path) { set_error("bad arguments"); return -1; }
     return grid->g->write_vtu(path, last != 0, time_label);

@@ -740,6 +740,13 @@ int Grid::qform_prepare(int nslot, const double* slot_t, int64_t ncol, const int
 int Grid::qform_run() {
     hipStream_t st = Engine::get().stream();
     QfJob Q = qj_;
+    qf_clock_valid_ = false;
+    if (qf_clock_on_) {
+        if (qf_clk_.alloc(2 * sizeof(unsigned long long))) return -3;
+        CFDWM_HIP(hipMemsetAsync(qf_clk_.get(), 0, 2 * sizeof(unsigned long long), st));
+        Q.clk = (unsigned long long*)qf_clk_.get();
+        qf_clock_valid_ = true;
+    }
     for (int64_t p0 = 0; p0 < qf_P_; p0 += qf_chunk_) {
         Q.p0 = p0;
         Q.P = std::min(qf_chunk_, qf_P_ - p0);
@@ -748,6 +755,21 @@ int Grid::qform_run() {
         PROF("k_qf_forms", launch_qf_forms(Q, st));
         qf_flop += qf_fpp_ * (double)Q.ppad * qj_.nform;
     }
+    return 0;
+}
+
+// Mean shader clock over the blocks of the last fill's forms launches:
+// cycles / wall-clock ticks x the wall-clock rate.
+int Grid::qform_clock(double* mhz) {
+    if (mhz) *mhz = 0;
+    if (!qf_clock_valid_ || !qf_clk_.get()) return -1;
+    unsigned long long c[2] = {0, 0};
+    CFDWM_HIP(hipStreamSynchronize(Engine::get().stream()));
+    CFDWM_HIP(hipMemcpy(c, qf_clk_.get(), sizeof(c), hipMemcpyDeviceToHost));
+    int khz = 0;
+    CFDWM_HIP(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, 0));
+    if (c[1] == 0 || khz <= 0) return -1;
+    if (mhz) *mhz = (double)c[0] / (double)c[1] * khz * 1e-3;
     return 0;
 }
 

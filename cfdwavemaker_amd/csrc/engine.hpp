@@ -344,6 +344,9 @@ public:
     bool qf_used = false;
     int qf_nform = 0, qf_rank = 0;
     double qf_flop = 0, qf_basis_err = 0;
+    // shader clock of the forms launches of later fills (diagnostic)
+    void qform_clock_enable(bool on) { qf_clock_on_ = on; }
+    int qform_clock(double* mhz);
 private:
     int prec;
     std::vector<int32_t> ignore_;
@@ -357,6 +360,8 @@ private:
     DevBuf qout_;                             // form values [nform][points of the fill]
     QfJob qj_{};                              // prepared forms launch (first chunk)
     int64_t qf_chunk_ = 0, qf_P_ = 0;
+    bool qf_clock_on_ = false, qf_clock_valid_ = false;
+    DevBuf qf_clk_;                           // 2 counters of the clock measurement
     double qf_fpp_ = 0, qf_scale_[4] = {1, 1, 1, 1};
     bool batch_pending_ = false;
     int qform_prepare(int nslot, const double* slot_t, int64_t ncol, const int32_t* cols);

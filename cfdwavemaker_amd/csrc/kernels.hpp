@@ -104,6 +104,9 @@ struct QfJob {
     const int32_t* kend = nullptr;        // per 64-wide tile of i: end of its m range (multiple of 16)
     float* out = nullptr;                 // [nform][ostride]
     int64_t ostride = 0;
+    // optional clock measurement of the forms launch: [0] += shader-clock
+    // cycles and [1] += constant-rate wall-clock ticks of every block
+    unsigned long long* clk = nullptr;
 };
 
 // One tile of a batched forms launch: job (sea state), point tile, form.

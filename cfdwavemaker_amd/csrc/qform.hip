@@ -322,7 +322,16 @@ __global__ __launch_bounds__(256) void k_qf_forms_il(QfJob J, int npt) {
     const int ptile = pg * kQfGP + in % kQfGP;
     const int form = fg * kQfGF + in / kQfGP;
     if (ptile >= npt || form >= J.nform) return;      // whole block, before any barrier
+    if (!J.clk) {
+        qf_tile_il(J, ptile, form);
+        return;
+    }
+    const unsigned long long c0 = clock64(), w0 = wall_clock64();
     qf_tile_il(J, ptile, form);
+    if (threadIdx.x == 0) {
+        atomicAdd(J.clk, clock64() - c0);
+        atomicAdd(J.clk + 1, wall_clock64() - w0);
+    }
 }
 
 // Persistent launch over the tiles of many jobs (sea states): block b takes

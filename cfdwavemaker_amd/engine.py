@@ -181,6 +181,13 @@ class Grid:
             return None
         return dict(nform=nf.value, rank=rk.value, mfma_flop=fl.value, basis_err=er.value)
 
+    def qform_clock(self, enable):
+        """Mean shader clock (MHz) of the last fill's form launches, or None
+        if that fill was not measured; then measure later fills or not."""
+        mhz = C.c_double()
+        r = self.L.cfdwm_grid_qform_clock(self.h, 1 if enable else 0, C.byref(mhz))
+        return mhz.value if r == 0 else None
+
     def query_device_f32(self, n, t, x_ptr, y_ptr, z_ptr, eta_ptr, u_ptr, v_ptr, w_ptr):
         """Device pointers (e.g. torch tensors' data_ptr()), enqueued on the library stream."""
         check(self.L.cfdwm_grid_query_device_f32(self.h, int(n), float(t), x_ptr, y_ptr, z_ptr,

@@ -207,6 +207,13 @@ CFDWM_EXPORT int cfdwm_grid_query_device_f32(cfdwm_grid* grid, int64_t n, double
  * the pair loop. */
 CFDWM_EXPORT int cfdwm_grid_qform_info(cfdwm_grid* grid, int* nform, int* rank, double* mfma_flop,
                                        double* basis_err);
+/* Shader clock of the MFMA form launches (no reference counterpart: a
+ * measurement aid).  Writes the mean shader clock in MHz over the blocks of
+ * the last fill's form launches, if that fill was measured (returns 0; else -1
+ * and 0.0), then switches the measurement on (enable != 0) or off for later
+ * fills of this grid.  Each block adds its shader-clock cycles and wall-clock
+ * ticks to two device counters; results of the fill do not change. */
+CFDWM_EXPORT int cfdwm_grid_qform_clock(cfdwm_grid* grid, int enable, double* mhz);
 /* Batched refill of many s-grids (no reference counterpart: the batched
  * workload of many sea states).  Equivalent to cfdwm_grid_refill on each grid;
  * FP32 grids have their second-order forms evaluated in persistent launches

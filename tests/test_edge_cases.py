@@ -108,3 +108,24 @@ def test_point_batch_eta2_in_several_chunks():
     small = np.concatenate([s.eval("eta2", t[a:a + 50000], x[a:a + 50000], y[a:a + 50000])
                             for a in range(0, n, 50000)])
     assert float(np.max(np.abs(big - small))) <= 1e-6 * float(np.max(np.abs(small)))
+
+
+def test_forms_clock_measurement_leaves_fill_unchanged():
+    """The shader-clock measurement of the form launches (a diagnostic) leaves
+    the fill bit for bit unchanged and reports a clock in the GPU's range;
+    fills after it is switched off are not measured."""
+    from cfdwavemaker_amd import engine as eng
+    from cfdwavemaker_amd.seastate import make_sea
+    s = eng.Sea(make_sea(256, 4.0, 10.0, 50.0, seed=5).as_dict(), precision=eng.FP32, with_direct=False)
+    g = s.grid((0.0, 124.0, 0.0, 124.0), 32, 32, 4, 0.0, 0.5, 0.7, 1.5, 50.0)
+    assert g.qform_info() is not None
+    names = ("ETA", "UX", "UZ")
+    ref = [g.export(k) for k in names]
+    assert g.qform_clock(True) is None            # the fill at creation was not measured
+    g.refill()
+    assert all(np.array_equal(a, g.export(k)) for a, k in zip(ref, names))
+    mhz = g.qform_clock(False)
+    assert mhz is not None and 100.0 < mhz < 3500.0, mhz
+    g.refill()
+    assert g.qform_clock(False) is None
+    assert all(np.array_equal(a, g.export(k)) for a, k in zip(ref, names))
