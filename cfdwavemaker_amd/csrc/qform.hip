@@ -43,6 +43,11 @@ __device__ __forceinline__ uint32_t bf16_bits(float f) {
     return u >> 16;
 }
 __device__ __forceinline__ float bf16_val(uint32_t b) { return __uint_as_float(b << 16); }
+// bf16 element e (< 4) of four packed in a uint2, as float
+__device__ __forceinline__ float bf16_half(uint2 v, int e) {
+    const uint32_t w = e < 2 ? v.x : v.y;
+    return __uint_as_float((e & 1) ? (w & 0xFFFF0000u) : (w << 16));
+}
 
 // ------------------------------------------------------------------ Z operands
 // One thread per (point, 8 consecutive components): FP64 phase and range
@@ -101,32 +106,22 @@ __global__ __launch_bounds__(256) void k_qf_zgen(QfJob J) {
     *reinterpret_cast<uint4*>(J.zs_lo + o) = pack(sl);
 }
 
-// butterfly reduce-scatter over the 32 lanes of each half-wave: on return
-// v[0] of lane l holds the half-wave sum of the lane values v[l & 31]
-template <int O>
-__device__ __forceinline__ void bfly(float* v, int lane) {
-    const bool up = (lane & O) != 0;
-#pragma unroll
-    for (int k = 0; k < O; ++k) {
-        const float send = up ? v[k] : v[k + O];
-        const float keep = up ? v[k + O] : v[k];
-        v[k] = keep + __shfl_xor(send, O);
-    }
-}
-
 // ----------------------------------------------------------- quadratic forms
 // Form f of a launch:  Q_f(p) = sum_i e1_pi sum_m B1_f[i][m] c_pm + e2_pi sum_m B2_f[i][m] s_pm
 // with (e1, e2) = (c, s) for cos-type forms and (s, c) for sin-type forms.
 // One tile of work is (128 points, form): 4 waves, one per SIMD, walk the i
 // range in tiles of 128; wave (w & 1, w >> 1) takes 64 points x 64 i as
-// 2 x 2 MFMA 32 x 32 blocks for each of the two products.  The k (= m)
+// 2 x 2 MFMA 32 x 32 blocks for each of the two products, computed with
+// the table operand first, so that each lane's accumulator holds one point
+// and 16 values of i.  The k (= m)
 // range of an i tile runs from its first row (pairs have m > i) to the end
 // of its filtered rows.  Per k step (16 values of m) the block stages the A
 // operands of its 128 points and the B operands of its 128 i (4 arrays x 4
 // row blocks each, 32 KiB) in LDS, each operand tile read from L2 once per
-// block.  Per point the i sums stay in lane registers across tiles; one
-// cross-lane reduction and one LDS exchange between the two waves of a point
-// range at the end write Q_f(p).  Single-job launches swizzle blocks in
+// block.  The epilogue of an i tile multiplies the accumulators by c or s
+// of the same (point, i), read as runs of four consecutive i (8 bytes), and
+// sums over i in the lane; one exchange between the two half-waves and one
+// between the two waves of a point range at the end write Q_f(p).  Single-job launches swizzle blocks in
 // groups of 8 forms x 32 point tiles so that the Z and B tables a group
 // reads stay in the last-level cache.
 constexpr int kQfGF = 8, kQfGP = 32;
@@ -177,7 +172,7 @@ __device__ __forceinline__ void qf_tile_il(const QfJob& J, int ptile, int form) 
     const int64_t P0 = PB + wp * 64;
     const int hl = lane >> 5, cl32 = lane & 31;
     const int loff = ((lane >> 5) * 256 + (lane & 31) * 8) * 2;
-    float tot = 0.f;
+    float tp[2] = {0.f, 0.f};                     // per lane: points P0 + cl32 and P0 + 32 + cl32
     const int wv = __builtin_amdgcn_readfirstlane(w);
     const int lo8 = lane * 8;
     typedef __attribute__((address_space(3))) void* lds_vp;
@@ -220,7 +215,7 @@ __device__ __forceinline__ void qf_tile_il(const QfJob& J, int ptile, int form) 
             const int lv = s >> 3, q = s & 7, cs = q & 1, a2 = q >> 2, b = (q >> 1) & 1;
             const int fa = a2 * 4 + 2 * cs + (lv == 2 ? 1 : 0);
             const int fb = 8 + b * 4 + 2 * cs + (lv == 1 ? 1 : 0);
-            acc[q] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(op[fa], op[fb], acc[q], 0, 0, 0);
+            acc[q] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(op[fb], op[fa], acc[q], 0, 0, 0);
         };
         // one step: products of operand set x, reads of step j + 1 into set y
         // from stage s1, requests of step kbd into stage s3; 8 chunks of 3
@@ -266,48 +261,46 @@ __device__ __forceinline__ void qf_tile_il(const QfJob& J, int ptile, int form) 
         // the next i tile's requests reuse stages 0 .. 2
         QF4_SYNC(0);
         if (wk1 > wk0) {                              // wave-uniform
-            float v[32];
+            // accumulator element r of block (a2, b): point P0 + a2 * 32 + cl32,
+            // i = I0 + b * 32 + 8 j + 4 hl + e (j = r / 4, e = r % 4); the four
+            // i of one j are consecutive k of one half-tile row (8 bytes)
 #pragma unroll
             for (int a2 = 0; a2 < 2; ++a2) {
-                float sum[16];
-#pragma unroll
-                for (int r = 0; r < 16; ++r) sum[r] = 0.f;
+                const uint16_t* zrow = J.zc_hi + ((P0 >> 5) + a2) * nkb * 512 + cl32 * 8 + 4 * hl;
+                float s = 0.f;
 #pragma unroll
                 for (int b = 0; b < 2; ++b) {
-                    const int i = I0 + b * 32 + cl32;
-                    const uint16_t* zc = J.zc_hi + (((P0 >> 5) + a2) * nkb + (i >> 4)) * 512 + ((i >> 3) & 1) * 256
-                                       + 4 * hl * 8 + (i & 7);
-                    const uint16_t* zcl = zc + J.zn;
-                    const uint16_t* zs = zc + 2 * J.zn;
-                    const uint16_t* zsl = zc + 3 * J.zn;
 #pragma unroll
-                    for (int r = 0; r < 16; ++r) {
-                        const int ro = (8 * (r >> 2) + (r & 3)) * 8;
-                        const float c = bf16_val(zc[ro]) + bf16_val(zcl[ro]);
-                        const float sv = bf16_val(zs[ro]) + bf16_val(zsl[ro]);
-                        sum[r] += acc[(a2 * 2 + b) * 2][r] * (sint ? sv : c) + acc[(a2 * 2 + b) * 2 + 1][r] * (sint ? c : sv);
-                        if ((r & 3) == 3) __asm__ volatile("" ::: "memory");
+                    for (int j = 0; j < 4; ++j) {
+                        const uint16_t* zp = zrow + ((I0 >> 4) + 2 * b + (j >> 1)) * 512 + (j & 1) * 256;
+                        const uint2 ch = *reinterpret_cast<const uint2*>(zp);
+                        const uint2 cl = *reinterpret_cast<const uint2*>(zp + J.zn);
+                        const uint2 sh = *reinterpret_cast<const uint2*>(zp + 2 * J.zn);
+                        const uint2 sl = *reinterpret_cast<const uint2*>(zp + 3 * J.zn);
+#pragma unroll
+                        for (int e = 0; e < 4; ++e) {
+                            const int r = 4 * j + e;
+                            const float c = bf16_half(ch, e) + bf16_half(cl, e);
+                            const float sv = bf16_half(sh, e) + bf16_half(sl, e);
+                            s += acc[(a2 * 2 + b) * 2][r] * (sint ? sv : c) + acc[(a2 * 2 + b) * 2 + 1][r] * (sint ? c : sv);
+                        }
                     }
                 }
-#pragma unroll
-                for (int r = 0; r < 16; ++r) v[a2 * 16 + r] = sum[r];
+                tp[a2] += s;
             }
-            bfly<16>(v, lane);
-            bfly<8>(v, lane);
-            bfly<4>(v, lane);
-            bfly<2>(v, lane);
-            bfly<1>(v, lane);
-            tot += v[0];
         }
     }
 #undef QF4_SYNC
+    // lane l ends with point P0 + l: its own half's i sums plus the other half's
+    const float t0 = tp[0] + __shfl_xor(tp[0], 32);
+    const float t1 = tp[1] + __shfl_xor(tp[1], 32);
+    const float tot = hl ? t1 : t0;
     float* red = reinterpret_cast<float*>(lds[0]);
     __syncthreads();
     if (wi) red[wp * 64 + lane] = tot;
     __syncthreads();
     if (!wi) {
-        const int k = cl32;
-        const int64_t p = P0 + (k >> 4) * 32 + 8 * ((k & 15) >> 2) + 4 * hl + (k & 3);
+        const int64_t p = P0 + lane;
         if (p < J.P) J.out[(int64_t)form * J.ostride + J.p0 + p] = tot + red[wp * 64 + lane];
     }
 }
