@@ -12,8 +12,8 @@
 // times components x components) on v_mfma_f32_32x32x16_bf16; every operand is
 // split into bf16 hi + lo parts and the three products hi*hi + hi*lo + lo*hi
 // are accumulated in FP32, which keeps about 16 significant bits per product.
-// The outer sum over i is a fused epilogue (accumulator times c or s) with a
-// cross-lane reduction at the end; the column kernel folds the result into
+// The outer sum over i is a fused epilogue (accumulator times c or s, summed
+// in the lane, two exchanges at the end); the column kernel folds the result into
 // its FP64 total.  The same kernel evaluates the z-independent forms of the
 // velocity pair sums under the separable depth basis (depth_basis.hpp): per
 // skeleton wavenumber mu_q one cos-type form for u and v and one sin-type
