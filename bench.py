@@ -27,6 +27,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 PEAK = {"fp32": 157.3e12, "fp64": 78.6e12}   # MI355X vector peaks (FLOP/s)
+PEAK_UNPACKED = {"fp32": 78.6e12, "fp64": 78.6e12}   # without packed FP32 instructions
 PEAK_MFMA_BF16 = 2.5e15                        # MI355X dense bf16 matrix peak (FLOP/s)
 PEAK_SCLK_MHZ = 2400.0                         # its engine clock: 256 CUs x 4 x 1024 FLOP/clk x 2.4 GHz
 PEAK_HBM = 8.0e12                              # MI355X HBM3E (B/s)
@@ -582,7 +583,11 @@ def main():
         "column_linear_sums": {"kernels": "k_columns_grid + k_levels_grid", "bound": "valu",
                                "flop_per_column": 4.0 * len(s["omega"]) * (3 * NL + 4),
                                "achieved": lin_flop / t_lin / 1e12, "peak": PEAK[a.precision] / 1e12,
-                               "unit": "TFLOP/s", "frac": lin_flop / t_lin / PEAK[a.precision]},
+                               "unit": "TFLOP/s", "frac": lin_flop / t_lin / PEAK[a.precision],
+                               # the FP32 peak counts packed FP32 FMAs, which this build does not
+                               # emit; one FMA per lane and cycle gives half of it
+                               "peak_unpacked": PEAK_UNPACKED[a.precision] / 1e12,
+                               "frac_unpacked": lin_flop / t_lin / PEAK_UNPACKED[a.precision]},
     }
     upd = update_leg(engine, grid, a.updates) if a.updates > 0 else None
 
