@@ -254,7 +254,8 @@ __device__ __forceinline__ void lev_store(const SeaDev<R>& S, const LevelJob<R>&
 // Accumulation: each row i (pairs (i, m), m > i) is summed in the working
 // precision R and folded into FP64 totals, which keeps the FP32 error at the
 // level of the per-term rounding instead of growing over ~N^2/2 terms.
-template <typename R, int LC, int MODE>
+// F8: FP32 grid fill whose pair sums come from the quadratic forms (J.qf).
+template <typename R, int LC, int MODE, bool F8 = false>
 __device__ __forceinline__ void levels_body(const SeaDev<R>& S, const LevelJob<R>& J) {
     constexpr bool GRID = MODE == 0;
     constexpr bool WP = MODE == 2;
@@ -300,6 +301,17 @@ __device__ __forceinline__ void levels_body(const SeaDev<R>& S, const LevelJob<R
 
     const UvwRec<R>* __restrict__ tab = direct ? S.duvw : S.fuvw;
     const int ia = min(sg.rlo, sg.own_lo), ib = max(sg.rhi, sg.own_hi);
+    // The terms of a component meet in ur/vr/wr and are folded into the FP64
+    // totals.  FP32 grid fills whose pair sums come from the quadratic forms
+    // (F8: a component then adds its linear and diagonal terms only) fold
+    // once per 8 components; elsewhere the FP32 row sums of the pair terms
+    // are folded per component.  The component loop is wave-uniform.
+    constexpr bool SQ = sizeof(R) == 4;
+    constexpr int fold_n = (SQ && GRID && F8) ? 8 : 1;
+    R ur[LC], vr[LC], wr[LC];
+#pragma unroll
+    for (int l = 0; l < LC; ++l) ur[l] = vr[l] = wr[l] = 0;
+    int nfold = 0;
     for (int i = ia; i < ib; ++i) {
         const bool own = i >= sg.own_lo && i < sg.own_hi;
         const bool sec = secf && (direct || S.rowActive[i]);
@@ -311,10 +323,11 @@ __device__ __forceinline__ void levels_body(const SeaDev<R>& S, const LevelJob<R
         cs_of(phase_of(pi, dxp, dyp, t), ci, si);
         const R ri = (R)freq_ramp(t, pi.rt, pi.ts, pi.du);
         const R cr = ci * ri, sr = si * ri;
-        R ur[LC], vr[LC], wr[LC];
         R pr = 0;
+        if constexpr (fold_n == 1) {
 #pragma unroll
-        for (int l = 0; l < LC; ++l) ur[l] = vr[l] = wr[l] = 0;
+            for (int l = 0; l < LC; ++l) ur[l] = vr[l] = wr[l] = 0;
+        }
         if (own) {
             gx += (double)(c.gxz * cr);
             gy += (double)(c.gyz * cr);
@@ -322,7 +335,6 @@ __device__ __forceinline__ void levels_body(const SeaDev<R>& S, const LevelJob<R
             if (WP) gp += (double)(c.gpz * cr);
             // FP32: the diagonal's 2k profile is the square of the k profile
             // (p2k = 2 pk, p2g = 2 pg), so the k exponentials serve both
-            constexpr bool SQ = sizeof(R) == 4;
             R Ek[LC], Gk[LC];
             if (lin || (SQ && sec)) {
 #pragma unroll
@@ -334,11 +346,9 @@ __device__ __forceinline__ void levels_body(const SeaDev<R>& S, const LevelJob<R
                     const R E = Ek[l], G = Gk[l];
                     const R ch = E + G, sh = E - G;
                     if constexpr (SQ) {
-                        // FP32: the component's terms meet in ur/vr/wr and are
-                        // folded into the FP64 totals once, below
-                        ur[l] = c.lu * cr * ch;
-                        vr[l] = c.lv * cr * ch;
-                        wr[l] = c.lw * sr * sh;
+                        ur[l] = fmaR(c.lu * cr, ch, ur[l]);
+                        vr[l] = fmaR(c.lv * cr, ch, vr[l]);
+                        wr[l] = fmaR(c.lw * sr, sh, wr[l]);
                     } else {
                         U[l] += (double)(c.lu * cr * ch);
                         V[l] += (double)(c.lv * cr * ch);
@@ -448,13 +458,25 @@ __device__ __forceinline__ void levels_body(const SeaDev<R>& S, const LevelJob<R
                 }
             }
         }
+        if (++nfold == fold_n) {
+#pragma unroll
+            for (int l = 0; l < LC; ++l) {
+                U[l] += (double)ur[l];
+                V[l] += (double)vr[l];
+                W[l] += (double)wr[l];
+                if constexpr (fold_n > 1) ur[l] = vr[l] = wr[l] = 0;
+            }
+            nfold = 0;
+        }
+        if (WP && direct) P2 += (double)pr;
+    }
+    if (fold_n > 1 && nfold > 0) {
 #pragma unroll
         for (int l = 0; l < LC; ++l) {
             U[l] += (double)ur[l];
             V[l] += (double)vr[l];
             W[l] += (double)wr[l];
         }
-        if (WP && direct) P2 += (double)pr;
     }
     if constexpr (GRID) {
         if (J.qf && blockIdx.y == 0) {
@@ -513,10 +535,10 @@ __global__ __launch_bounds__(256) void k_levels(SeaDev<R> S, LevelJob<R> J) {
 // FP32 s-grid fill: the register budget is steered to 128 VGPRs so that four
 // waves share each SIMD (the kernel is issue-bound; more waves hide the
 // scalar-load and transcendental latencies).
-template <int LC>
+template <int LC, bool F8>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8)))
 void k_levels_grid32(SeaDev<float> S, LevelJob<float> J) {
-    levels_body<float, LC, 0>(S, J);
+    levels_body<float, LC, 0, F8>(S, J);
 }
 
 // sum of the segment partials of k_levels
@@ -951,7 +973,8 @@ hipError_t launch_columns(const SeaDev<R>& S, const ColumnJob<R>& J, bool grid, 
     const dim3 g1(nblk(n), J.nseg);
     if (grid) hipLaunchKernelGGL((k_columns<R, true>), g1, dim3(256), 0, st, S, J);
     else hipLaunchKernelGGL((k_columns<R, false>), g1, dim3(256), 0, st, S, J);
-    if (hipGetLastError() != hipSuccess || !J.partial) return hipGetLastError();
+    const hipError_t e = hipGetLastError();
+    if (e != hipSuccess || !J.partial) return e;
     if (grid) hipLaunchKernelGGL((k_columns_reduce<R, true>), dim3(nblk(n)), dim3(256), 0, st, S, J);
     else hipLaunchKernelGGL((k_columns_reduce<R, false>), dim3(nblk(n)), dim3(256), 0, st, S, J);
     return hipGetLastError();
@@ -962,11 +985,15 @@ static hipError_t launch_levels_t(const SeaDev<R>& S, const LevelJob<R>& J, hipS
     const int64_t n = level_threads(J.nslot, J.ncol, J.g.nl, LC, MODE == 0);
     if (J.nslot * J.ncol <= 0) return hipSuccess;
     if constexpr (MODE == 0 && sizeof(R) == 4 && (LC == 8 || LC == 16)) {
-        hipLaunchKernelGGL((k_levels_grid32<LC>), dim3(nblk(n), J.nseg), dim3(256), 0, st, S, J);
+        if (J.qf)
+            hipLaunchKernelGGL((k_levels_grid32<LC, true>), dim3(nblk(n), J.nseg), dim3(256), 0, st, S, J);
+        else
+            hipLaunchKernelGGL((k_levels_grid32<LC, false>), dim3(nblk(n), J.nseg), dim3(256), 0, st, S, J);
     } else {
         hipLaunchKernelGGL((k_levels<R, LC, MODE>), dim3(nblk(n), J.nseg), dim3(256), 0, st, S, J);
     }
-    if (hipGetLastError() != hipSuccess || !J.partial) return hipGetLastError();
+    const hipError_t e = hipGetLastError();
+    if (e != hipSuccess || !J.partial) return e;
     hipLaunchKernelGGL((k_levels_reduce<R, LC, MODE>), dim3(nblk(n)), dim3(256), 0, st, S, J);
     return hipGetLastError();
 }
@@ -1034,7 +1061,8 @@ hipError_t launch_prune(const R* eta, int64_t ncell, const int32_t* cols, int64_
     const int4 dst = make_int4(slot_dst[0], nslot > 1 ? slot_dst[1] : 0, nslot > 2 ? slot_dst[2] : 0,
                                nslot > 3 ? slot_dst[3] : 0);
     hipLaunchKernelGGL((k_eta_bounds<R>), dim3(1), dim3(1024), 0, st, eta, ncell, cols, ncol, nslot, dst, swl, eb);
-    if (hipGetLastError() != hipSuccess) return hipGetLastError();
+    const hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
     if (nrec <= 0) return hipSuccess;
     const double T = sizeof(R) == 4 ? 30.0 * 1.4426950408889634 : 40.0;
     hipLaunchKernelGGL((k_prune<R>), dim3(nblk(nrec)), dim3(256), 0, st, rec, nrec, eb, sigma, nl, h, T, lvl);
