@@ -680,6 +680,7 @@ int Grid::fill_slots(int nslot, const double* slot_t, const int* slot_dst, bool 
 // 0 when not, < 0 on error.
 int Grid::qform_prepare(int nslot, const double* slot_t, int64_t ncol, const int32_t* cols) {
     qf_used = false;
+    qf_clock_valid_ = false;                  // a measurement belongs to one fill
     if (!qform_ || sea->prec != kFP32 || sea->nfilt() == 0) return 0;
     const CompTable& c = sea->tab.comp;
     for (int s = 0; s < 4; ++s) qf_scale_[s] = 1.0;
