@@ -11,7 +11,11 @@ eta2 + Taylor gradients; level pass: first + second order + Taylor at every
 level; FD gradients) plus the 1e7 queries, all on the GPU with inputs resident
 in HBM.  value = grid points (4 x 256 x 256 x 16) per second of step time.
 
-usage: python bench.py [--gpus 1] [--steps 3] [--warmup 1] [--precision fp32|fp64]
+usage: python bench.py [--gpus 1] [--steps 3] [--warmup 1] [--precision bf16|fp32|fp64]
+
+Precision: bf16 (default) = FP32 sums with FP64 phase reduction whose MFMA
+quadratic forms take one bf16 product per term; fp32 = the same with three
+bf16 products per term (hi/lo split); fp64 = FP64 throughout.
 """
 import argparse
 import json
@@ -26,8 +30,8 @@ import numpy as np
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
-PEAK = {"fp32": 157.3e12, "fp64": 78.6e12}   # MI355X vector peaks (FLOP/s)
-PEAK_UNPACKED = {"fp32": 78.6e12, "fp64": 78.6e12}   # without packed FP32 instructions
+PEAK = {"fp32": 157.3e12, "bf16": 157.3e12, "fp64": 78.6e12}   # MI355X vector peaks (FLOP/s)
+PEAK_UNPACKED = {"fp32": 78.6e12, "bf16": 78.6e12, "fp64": 78.6e12}   # without packed FP32 instructions
 PEAK_MFMA_BF16 = 2.5e15                        # MI355X dense bf16 matrix peak (FLOP/s)
 PEAK_SCLK_MHZ = 2400.0                         # its engine clock: 256 CUs x 4 x 1024 FLOP/clk x 2.4 GHz
 PEAK_HBM = 8.0e12                              # MI355X HBM3E (B/s)
@@ -220,57 +224,85 @@ def surrogate_leg(steps: int, warmup: int) -> dict:
             "effective_points_per_s": eff, "fallback_to_exact": not ok}
 
 
-def batched_leg(nstates: int, nsample_err: int) -> dict:
-    """Batched workload: the first nstates of the 512 random sea states (N
-    128-2048, Hs 1-8 m, Tp 6-16 s, depth 20-200 m, half long-crested, half
-    16-direction), each on its own 64 x 64 x 8 s-grid over [0, 20 lambda_p]^2
-    with 4 time levels, refilled together (persistent forms launch, autotuned;
-    the seas' form tables built in the setup and kept resident); per-state
-    error of a sample against FP64 fills."""
+def batched_leg(nstates: int, prec_name: str, err_cols: int = 4) -> dict:
+    """Batched workload: nstates of the 512 random sea states (N 128-2048, Hs
+    1-8 m, Tp 6-16 s, depth 20-200 m, half long-crested, half 16-direction),
+    each on its own 64 x 64 x 8 s-grid over [0, 20 lambda_p]^2 with 4 time
+    levels, refilled in one call: per group of states (bounded by a device
+    memory budget) the form tables are built, the Z operands generated, the
+    forms of every state of the group evaluated in one persistent launch per
+    product mode (tile order, grouping and grid size from the tuner), the
+    column / level / gradient / packing passes run, and the tables released,
+    all inside the timed call.  Per state, the error of an err_cols x err_cols
+    column subsample (all levels and time levels) against an FP64 fill of the
+    same columns."""
     from cfdwavemaker_amd import engine
     from cfdwavemaker_amd import surrogate as S
     from cfdwavemaker_amd.seastate import config_sea
 
+    prec = {"bf16": engine.BF16, "fp32": engine.FP32, "fp64": engine.FP64}[prec_name]
+    os.environ.setdefault("CFDWM_QUIET", "1")
     seas, grids = [], []
     t0 = time.perf_counter()
     for st in range(nstates):
         sea = config_sea(4, st)
         lam = 2 * np.pi / S.peak_wavenumber(sea)
-        es = engine.Sea(sea.as_dict(), precision=engine.FP32, with_direct=False)
-        g = es.grid((0.0, 20 * lam, 0.0, 20 * lam), 64, 64, 8, 0.0, 0.5, 0.7, 1.5, sea.depth)
-        g.release_workspace()
-        seas.append((st, sea, es))
+        es = engine.Sea(sea.as_dict(), precision=prec, with_direct=False)
+        g = es.grid((0.0, 20 * lam, 0.0, 20 * lam), 64, 64, 8, 0.0, 0.5, 0.7, 1.5, sea.depth, defer_fill=True)
+        seas.append((st, sea, es, lam))
         grids.append(g)
-    # first call: autotune, and build every sea's form tables, which then stay
-    # resident (as a single sea's do between its fills)
-    engine.refill_batched(grids, release=False)
-    engine.sync()
+        if (st + 1) % 128 == 0:
+            print(f"[bench] batched: {st + 1}/{nstates} seas set up", file=sys.stderr, flush=True)
     t_setup = time.perf_counter() - t0
+    # first call: the tuner times its candidate launch shapes on the first
+    # group of each product mode (and the call fills every grid)
     t1 = time.perf_counter()
-    stats = engine.refill_batched(grids, release=False)
+    tune = engine.refill_batched(grids, release=True)
+    engine.sync()
+    t_first = time.perf_counter() - t1
+    print(f"[bench] batched: first call (tuning) {t_first:.2f} s", file=sys.stderr, flush=True)
+    t1 = time.perf_counter()
+    stats = engine.refill_batched(grids, release=True)
     engine.sync()
     el = time.perf_counter() - t1
+    print(f"[bench] batched: timed call {el:.2f} s", file=sys.stderr, flush=True)
     pts = nstates * 64 * 64 * 8 * 4
+    # per-state error: FP64 fill of err_cols^2 columns that coincide with grid
+    # columns 0, 21, 42, 63 (for err_cols 4) of the 64 x 64 grid
+    step = 63 // (err_cols - 1)
+    idx = np.arange(err_cols) * step
     errs = []
-    for k in range(min(nsample_err, nstates)):
-        st, sea, _ = seas[k]
-        lam = 2 * np.pi / S.peak_wavenumber(sea)
-        g64 = engine.Sea(sea.as_dict(), precision=engine.FP64, with_direct=False).grid(
-            (0.0, 20 * lam, 0.0, 20 * lam), 64, 64, 8, 0.0, 0.5, 0.7, 1.5, sea.depth)
-        e = {"state": st, "components": len(sea.omega)}
-        for f in ("ETA", "UX", "UZ"):
+    t2 = time.perf_counter()
+    for k, (st, sea, es, lam) in enumerate(seas):
+        dx = 20 * lam / 63
+        e64 = es.clone(engine.FP64)
+        g64 = e64.grid((0.0, idx[-1] * dx, 0.0, idx[-1] * dx), err_cols, err_cols, 8, 0.0, 0.5, 0.7, 1.5, sea.depth)
+        e = {"state": st, "components": int(len(sea.omega))}
+        for f in ("ETA", "UX", "UY", "UZ"):
             ref = g64.export(f)
-            e[f] = float(np.max(np.abs(grids[k].export(f) - ref)) / np.max(np.abs(ref)))
-        e["eta_err_over_hs"] = float(np.max(np.abs(grids[k].export("ETA") - g64.export("ETA"))) / sea.hs)
+            got = grids[k].export(f)
+            if f == "ETA":
+                got = got.reshape(4, 64, 64)[:, idx][:, :, idx].ravel()
+                e["eta_err_over_hs"] = float(np.max(np.abs(got - ref)) / sea.hs)
+            else:
+                got = got.reshape(4, 64, 64, 8)[:, idx][:, :, idx].ravel()
+                e[f + "_scaled"] = float(np.max(np.abs(got - ref)) / max(np.max(np.abs(ref)), 1e-300))
         errs.append(e)
-    return {"workload": f"{nstates} of the 512 random sea states, 64x64x8 s-grid x 4 time levels each",
+        g64.close()
+        e64.close()
+    t_err = time.perf_counter() - t2
+    worst = {key: max(e[key] for e in errs) for key in ("eta_err_over_hs", "UX_scaled", "UY_scaled", "UZ_scaled")}
+    return {"workload": f"{nstates} of the 512 random sea states (N 128-2048), 64x64x8 s-grid x 4 time levels each, "
+                        f"{prec_name}; form tables built inside the timed call",
             "states": nstates, "points": pts, "points_per_s": pts / el, "ms_refill_all": el * 1e3,
-            "setup_s": t_setup, "forms_ms": stats["forms_ms"],
-            "forms_tflops": stats["mfma_flop"] / (stats["forms_ms"] * 1e-3) / 1e12 if stats["forms_ms"] else None,
-            "forms_frac_bf16_peak": stats["mfma_flop"] / (stats["forms_ms"] * 1e-3) / PEAK_MFMA_BF16
-            if stats["forms_ms"] else None,
-            "groups": stats["groups"], "launch": {"blocks": stats["blocks"], "order": stats["order"]},
-            "per_state_err_vs_fp64": errs}
+            "setup_s": t_setup, "first_call_s": t_first, "tables_ms": stats["tables_ms"],
+            "forms_ms": stats["forms_ms"],
+            "forms_exec_tflops": stats["mfma_flop"] / (stats["forms_ms"] * 1e-3) / 1e12 if stats["forms_ms"] else None,
+            "groups": stats["groups"], "tiles": stats["tiles"],
+            "launch": {"blocks": stats["blocks"], "order": stats["order"], "group_forms": stats["group_forms"],
+                       "group_ptiles": stats["group_ptiles"], "tuned_on_first_call": bool(tune["tuned"])},
+            "per_state_err_vs_fp64": {"columns_per_state": err_cols * err_cols, "worst": worst, "states": errs,
+                                      "seconds": t_err}}
 
 
 def abi_worker(out_path: str) -> None:
@@ -439,7 +471,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--precision", default="fp32", choices=["fp32", "fp64"])
+    ap.add_argument("--precision", default="bf16", choices=["bf16", "fp32", "fp64"])
     ap.add_argument("--queries", type=int, default=10_000_000)
     ap.add_argument("--cpu-cols", type=int, default=12)
     ap.add_argument("--no-cpu", action="store_true")
@@ -452,7 +484,7 @@ def main():
     ap.add_argument("--no-direct", action="store_true", help="skip the direct per-point workload leg")
     ap.add_argument("--updates", type=int, default=5,
                     help="time-stepping updates of the grid to time after the fill (0: skip)")
-    ap.add_argument("--batched-states", type=int, default=48,
+    ap.add_argument("--batched-states", type=int, default=512,
                     help="sea states of the batched leg (512 = the full batched workload; 0 skips it)")
     ap.add_argument("--calib-copy", action="store_true",
                     help="run one plain 256 MiB device copy first (reference for PMC byte counters)")
@@ -475,7 +507,7 @@ def main():
     from cfdwavemaker_amd import engine
     from cfdwavemaker_amd.seastate import config_sea
 
-    prec = engine.FP32 if a.precision == "fp32" else engine.FP64
+    prec = {"bf16": engine.BF16, "fp32": engine.FP32, "fp64": engine.FP64}[a.precision]
     s = config_sea(3).as_dict()
     print(f"[bench] building N={len(s['omega'])} sea ({a.precision})", file=sys.stderr, flush=True)
     sea = engine.Sea(s, precision=prec, with_direct=False)
@@ -543,13 +575,24 @@ def main():
         f_ms, f_n = prof.get("k_qf_forms", (float("nan"), 1))
         per_fill = max(f_n // max(a.steps, 1), 1)
         avg = f_ms / max(f_n, 1) * 1e-3
+        # algorithmic work of the surface forms: 5 quadratic forms x 2 (2N)^2
+        # = 40 N^2 FLOP per column-slot, over the 4 x 256 x 256 column-slots of
+        # one launch; the kernel executes more: every form of the depth basis
+        # (1 + 3 R of them), nlv bf16 products per term and the pair triangle
+        # in tiles (exec_flop_per_launch)
+        ncomp = len(s["omega"])
+        alg_launch = 40.0 * ncomp * ncomp * 4 * NX * NY / per_fill
         flop_launch = qfi["mfma_flop"] / per_fill
-        roof = {"bound": "mfma", "kernel": "k_qf_forms", "achieved": flop_launch / avg / 1e12,
-                "peak": PEAK_MFMA_BF16 / 1e12, "unit": "TFLOP/s", "frac": flop_launch / avg / PEAK_MFMA_BF16,
+        roof = {"bound": "mfma", "kernel": "k_qf_forms", "achieved": alg_launch / avg / 1e12,
+                "peak": PEAK_MFMA_BF16 / 1e12, "unit": "TFLOP/s", "frac": alg_launch / avg / PEAK_MFMA_BF16,
                 "traffic": profiled_traffic("k_qf_forms"),
                 "traffic_source": "profiles/*_fetch_size.md + *_write_size.md (rocprofv3 --pmc, calibrated)",
-                "avg_launch_ms": avg * 1e3, "flop_per_launch": flop_launch,
-                "forms": qfi["nform"], "depth_basis_rank": qfi["rank"], "depth_basis_err": qfi["basis_err"]}
+                "avg_launch_ms": avg * 1e3, "flop_per_launch": alg_launch,
+                "flop_rule": "40 N^2 per column-slot (5 forms x 2 (2N)^2)",
+                "exec_flop_per_launch": flop_launch, "exec_achieved": flop_launch / avg / 1e12,
+                "exec_frac": flop_launch / avg / PEAK_MFMA_BF16, "exec_over_alg": flop_launch / alg_launch,
+                "forms": qfi["nform"], "bf16_products": qfi["products"], "depth_basis_rank": qfi["rank"],
+                "depth_basis_err": qfi["basis_err"], "depth_basis_tol": qfi["basis_tol"]}
         # shader clock during the forms launch, from one more fill (untimed)
         # with the in-kernel clock measurement on; the dense peak assumes the
         # peak engine clock, and MFMA power with these operands lowers it
@@ -561,6 +604,7 @@ def main():
             roof["sclk_mhz"] = sclk
             roof["peak_sclk_mhz"] = PEAK_SCLK_MHZ
             roof["frac_at_sclk"] = roof["frac"] * PEAK_SCLK_MHZ / sclk
+            roof["exec_frac_at_sclk"] = roof["exec_frac"] * PEAK_SCLK_MHZ / sclk
     else:
         lev_ms, lev_n = prof.get("k_levels_grid", (float("nan"), 1))
         lev_avg = lev_ms / max(lev_n, 1) * 1e-3
@@ -601,7 +645,8 @@ def main():
     u_err = float(np.max(np.abs(gxx.export("UX") - uref)) / np.max(np.abs(uref)))
 
     sur = None if a.no_surrogate else surrogate_leg(a.steps, a.warmup)
-    bat = batched_leg(a.batched_states, 3) if a.batched_states > 0 else None
+    bat = batched_leg(a.batched_states, a.precision if a.precision != "fp64" else "fp32") \
+        if a.batched_states > 0 else None
     dirl = None if a.no_direct else direct_leg(a.steps)
     smitl = None if a.no_direct else direct_leg(a.steps, smit=True)
     abil = None if a.no_abi else abi_leg()
@@ -630,7 +675,7 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32" if a.precision == "fp32" else "f64",
+        "dtype": {"bf16": "bf16/f32", "fp32": "f32", "fp64": "f64"}[a.precision],
         "data": "synthetic (JONSWAP + cos^2s spreading, seeded)",
         "config": {"workload": "N=1024 short-crested sea, s-grid 256x256 columns x 16 levels x 4 time "
                                "levels + 1e7 spline queries", "components": len(s["omega"]),

@@ -37,13 +37,14 @@ BATCH_API = [
     "cfdwm_mlp_loss_grad_device", "cfdwm_mlp_train_device", "cfdwm_surrogate_linear", "cfdwm_surrogate_add",
     "cfdwm_grids_refill_batched", "cfdwm_grid_release_workspace", "wave_PrefetchPoints", "wave_PrefetchClear",
     "wave_PrefetchStats", "cfdwm_eval_fields", "cfdwm_grid_write_vtu", "cfdwm_sea_set_vertical_theory",
-    "cfdwm_grid_qform_clock",
+    "cfdwm_grid_qform_clock", "cfdwm_grid_qform_mode", "cfdwm_sea_clone",
 ]
 
 
 class BatchStatsC(C.Structure):
     _fields_ = [(n, C.c_int) for n in ("groups", "batched", "single", "tiles", "blocks", "order", "tuned")] + \
-               [("forms_ms", C.c_double), ("mfma_flop", C.c_double)]
+               [("forms_ms", C.c_double), ("mfma_flop", C.c_double)] + \
+               [("group_forms", C.c_int), ("group_ptiles", C.c_int), ("tables_ms", C.c_double)]
 
 
 class SeaParams(C.Structure):
@@ -55,7 +56,7 @@ class SeaParams(C.Structure):
 class GridDescC(C.Structure):
     _fields_ = [("domain", _d * 4), ("nx", C.c_int), ("ny", C.c_int), ("nl", C.c_int),
                 ("t0", _d), ("dt", _d), ("tan_a", _d), ("tan_b", _d), ("water_depth", _d),
-                ("swl", _d), ("domain_ignore", _d * 4), ("ignore_at_init", C.c_int)]
+                ("swl", _d), ("domain_ignore", _d * 4), ("ignore_at_init", C.c_int), ("defer_fill", C.c_int)]
 
 
 _LIB = None
@@ -114,6 +115,8 @@ def load(path: str | None = None) -> C.CDLL:
     L.cfdwm_sea_create.restype = C.c_int
     L.cfdwm_sea_create.argtypes = [C.POINTER(C.c_void_p), C.c_int] + [_dp] * 8 + [C.POINTER(SeaParams)]
     L.cfdwm_sea_destroy.argtypes = [C.c_void_p]
+    L.cfdwm_sea_clone.restype = C.c_int
+    L.cfdwm_sea_clone.argtypes = [C.POINTER(C.c_void_p), C.c_void_p, C.c_int]
     L.cfdwm_sea_set_vertical_theory.restype = C.c_int
     L.cfdwm_sea_set_vertical_theory.argtypes = [C.c_void_p, C.c_int]
     L.cfdwm_sea_terms.argtypes = [C.c_void_p] + [C.POINTER(_i64)] * 3
@@ -145,6 +148,8 @@ def load(path: str | None = None) -> C.CDLL:
     L.cfdwm_grid_qform_info.argtypes = [C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_int), _dp, _dp]
     L.cfdwm_grid_qform_clock.restype = C.c_int
     L.cfdwm_grid_qform_clock.argtypes = [C.c_void_p, C.c_int, _dp]
+    L.cfdwm_grid_qform_mode.restype = C.c_int
+    L.cfdwm_grid_qform_mode.argtypes = [C.c_void_p, C.POINTER(C.c_int), _dp, C.POINTER(C.c_int)]
     L.cfdwm_depth_basis.restype = C.c_int
     L.cfdwm_depth_basis.argtypes = [_d, _d, _d, C.c_int, _dp, _dp, _dp]
     _fp = C.POINTER(C.c_float)

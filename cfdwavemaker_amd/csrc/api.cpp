@@ -328,9 +328,14 @@ void dump_spectral_components() {
     std::fclose(fp);
 }
 
+// CFDWM_PRECISION: fp64 (default), fp32, or bf16 (FP32 with the
+// one-product bf16 quadratic forms); the values of cfdwm_sea_params.precision
 int env_precision() {
     const char* e = std::getenv("CFDWM_PRECISION");
-    return (e && (!std::strcmp(e, "fp32") || !std::strcmp(e, "FP32"))) ? kFP32 : kFP64;
+    if (!e) return 0;
+    if (!std::strcmp(e, "fp32") || !std::strcmp(e, "FP32")) return 1;
+    if (!std::strcmp(e, "bf16") || !std::strcmp(e, "BF16")) return 2;
+    return 0;
 }
 
 [[noreturn]] void gpu_fatal(const char* where) {
@@ -854,10 +859,26 @@ int cfdwm_sea_create(cfdwm_sea** out, int n, const double* omega, const double* 
     }
     auto cs = std::make_unique<cfdwm_sea>();
     cs->s = std::make_unique<Sea>();
-    cs->s->prec = P->precision == 1 ? kFP32 : kFP64;
+    if (P->precision < 0 || P->precision > 2) { set_error("precision must be 0 (FP64), 1 (FP32) or 2 (BF16 forms)"); return -1; }
+    cs->s->prec = P->precision >= 1 ? kFP32 : kFP64;
+    cs->s->qf_products = P->precision == 2 ? 1 : 3;
     cs->with_direct = P->with_direct != 0;
     std::string err = build_tables(in, cs->s->tab, cs->with_direct);
     if (!err.empty()) { set_error(err); return -1; }
+    if (int rc = cs->s->upload(cs->with_direct)) return rc;
+    *out = cs.release();
+    return 0;
+}
+
+int cfdwm_sea_clone(cfdwm_sea** out, const cfdwm_sea* src, int precision) {
+    if (!out || !src || precision < 0 || precision > 2) { set_error("bad arguments"); return -1; }
+    auto cs = std::make_unique<cfdwm_sea>();
+    cs->s = std::make_unique<Sea>();
+    cs->s->tab = src->s->tab;                 // host tables: no second O(N^2) coefficient pass
+    cs->s->prec = precision >= 1 ? kFP32 : kFP64;
+    cs->s->qf_products = precision == 2 ? 1 : 3;
+    cs->s->vertical_theory = src->s->vertical_theory;
+    cs->with_direct = src->with_direct;
     if (int rc = cs->s->upload(cs->with_direct)) return rc;
     *out = cs.release();
     return 0;
@@ -958,7 +979,7 @@ int cfdwm_grid_create(cfdwm_grid** out, cfdwm_sea* sea, const cfdwm_grid_desc* d
     gd.ignore_at_init = d->ignore_at_init;
     auto cg = std::make_unique<cfdwm_grid>();
     cg->g = std::make_unique<Grid>(sea->s.get(), gd);
-    if (int rc = cg->g->init()) return rc;
+    if (int rc = cg->g->init(d->defer_fill == 0)) return rc;
     *out = cg.release();
     return 0;
 }
@@ -994,6 +1015,20 @@ int cfdwm_grid_qform_info(cfdwm_grid* grid, int* nform, int* rank, double* mfma_
     if (basis_err) *basis_err = g.qf_basis_err;
     return g.qf_used ? 1 : 0;
 }
+int cfdwm_grid_qform_mode(cfdwm_grid* grid, int* products, double* basis_tol, int* reason) {
+    if (!grid) { set_error("bad arguments"); return -1; }
+    const Grid& g = *grid->g;
+    if (reason) *reason = g.qf_skip;
+    const Sea::QfTables* T = nullptr;
+    if (g.qf_used && g.sea->qform_tables(&T) == 1) {
+        if (products) *products = T->nlv;
+        if (basis_tol) *basis_tol = T->basis_tol;
+        return 1;
+    }
+    if (products) *products = 0;
+    if (basis_tol) *basis_tol = 0;
+    return 0;
+}
 int cfdwm_grid_qform_clock(cfdwm_grid* grid, int enable, double* mhz) {
     if (!grid) { set_error("bad arguments"); return -1; }
     const int r = grid->g->qform_clock(mhz);
@@ -1024,6 +1059,8 @@ int cfdwm_grids_refill_batched(cfdwm_grid** grids, int n, double budget_gb, int 
         out->groups = S.groups; out->batched = S.batched; out->single = S.single; out->tiles = S.tiles;
         out->blocks = S.blocks; out->order = S.order; out->tuned = S.tuned;
         out->forms_ms = S.forms_ms; out->mfma_flop = S.mfma_flop;
+        out->group_forms = S.group_forms; out->group_ptiles = S.group_ptiles;
+        out->tables_ms = S.tables_ms;
     }
     return 0;
 }

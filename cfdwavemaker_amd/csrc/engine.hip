@@ -4,6 +4,7 @@
 #include "depth_basis.hpp"
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstring>
 #include <cstdio>
@@ -286,21 +287,27 @@ double freq_ramp_host(double t, double rt, double ts, double du) {
 // (coefficients cP = c_q(kP), cM = c_q(kM)):  u: B1 = cuP cP + cuM cM,
 // B2 = cuM cM - cuP cP;  v likewise with cv;  w (sin-type): B1 = cwP cP + cwM cM,
 // B2 = cwP cP - cwM cM.
-const Sea::QfTables* Sea::qform_tables() {
-    if (qf_) return qf_.get();
-    if (prec != kFP32) return nullptr;
+int Sea::qform_tables(const QfTables** out) {
+    *out = nullptr;
+    if (qf_) { *out = qf_.get(); return 1; }
+    if (prec != kFP32) return 0;
     const PairTable& P = tab.filt;
     const int N = tab.comp.n;
     const double h = tab.in.depth;
     double kmx = 0;
     for (const PairRec64& r : P.rec) kmx = std::max(kmx, std::max(std::fabs(r.kP), std::fabs(r.kM)));
-    double tol = 1e-6;
+    int nlv = qf_products == 1 ? 1 : 3;
+    if (const char* e = std::getenv("CFDWM_QF_PRODUCTS")) nlv = std::atoi(e) == 1 ? 1 : 3;
+    double tol = qf_tol > 0 ? qf_tol : (nlv == 1 ? 1e-4 : 1e-6);
     if (const char* e = std::getenv("CFDWM_QF_TOL")) tol = std::atof(e);
     DepthBasis DB;
-    build_depth_basis_cached(kmx * (1.0 + 1e-9) + 1e-12, 2.0 * h, tol, 32, DB);
+    const int st_basis = build_depth_basis_cached(kmx * (1.0 + 1e-9) + 1e-12, 2.0 * h, tol, 32, DB);
     const int R = DB.R;
-    if (R < 1 || R > 32) return nullptr;
+    // no basis within the tolerance at rank <= 32: the fills take the pair loop
+    qf_fallback = st_basis != 0 || !(DB.err <= tol) || R < 1 || R > 32;
+    if (qf_fallback) return 0;
     const int nform = 1 + 3 * R;
+    const int nbarr = nlv == 1 ? 2 : 4;
     const int npad = (N + 127) / 128 * 128;
     const size_t nn = (size_t)npad * npad;
     std::vector<int8_t> ftype(nform, 0);
@@ -312,25 +319,34 @@ const Sea::QfTables* Sea::qform_tables() {
     }
     auto T = std::make_unique<QfTables>();
     T->npad = npad; T->nform = nform; T->rank = R; T->mu = DB.mu; T->basis_err = DB.err;
+    T->basis_tol = tol; T->nlv = nlv; T->nbarr = nbarr;
     T->kend_h = kend;
     hipStream_t st = Engine::get().stream();
     DevBuf tau, einv;
-    const size_t bytes = (size_t)nform * 4 * nn * sizeof(uint16_t);
+    // one product per term: the velocity forms (hi parts) in b, the eta2 form
+    // (hi and lo parts, three products) in beta
+    const bool split = nlv == 1;
+    const size_t bytes = (size_t)(split ? nform - 1 : nform) * nbarr * nn * sizeof(uint16_t);
+    const size_t ebytes = split ? 4 * nn * sizeof(uint16_t) : 0;
+    if (split && T->beta.alloc(ebytes)) return -3;
+    if (split) CFDWM_HIP(hipMemsetAsync(T->beta.get(), 0, ebytes, st));
     if (T->b.alloc(bytes) || tau.upload(DB.tau.data(), R * sizeof(double), st) ||
         einv.upload(DB.einv.data(), (size_t)R * R * sizeof(double), st) ||
         T->ftype.upload(ftype.data(), ftype.size(), st) ||
         T->kend.upload(kend.data(), kend.size() * sizeof(int32_t), st))
-        return nullptr;
-    if (hipMemsetAsync(T->b.get(), 0, bytes, st) != hipSuccess) return nullptr;
+        return -3;
+    CFDWM_HIP(hipMemsetAsync(T->b.get(), 0, bytes, st));
     QfBuild Bq;
-    Bq.n = N; Bq.npad = npad; Bq.R = R; Bq.nform = nform;
+    Bq.n = N; Bq.npad = npad; Bq.R = R; Bq.nform = nform; Bq.nbarr = nbarr;
     Bq.frow = d32_.frow; Bq.fmhi = d32_.fmhi; Bq.uvw = d32_.fuvw; Bq.x = d32_.fx;
     Bq.tau = (const double*)tau.get(); Bq.einv = (const double*)einv.get();
     Bq.b = (uint16_t*)T->b.get();
-    if (N > 0 && launch_qf_build(Bq, st) != hipSuccess) return nullptr;
-    if (hipStreamSynchronize(st) != hipSuccess) return nullptr;
+    Bq.beta = split ? (uint16_t*)T->beta.get() : nullptr;
+    if (N > 0) CFDWM_HIP(launch_qf_build(Bq, st));
+    CFDWM_HIP(hipStreamSynchronize(st));
     qf_ = std::move(T);
-    return qf_.get();
+    *out = qf_.get();
+    return 1;
 }
 
 void Sea::release_qform() { qf_.reset(); }
@@ -379,8 +395,8 @@ int points_eta2_form(Sea& s, int64_t n, const double* dx, const double* dy, cons
     const CompTable& c = s.tab.comp;
     for (int i = 1; i < c.n; ++i)
         if (c.rt[i] != c.rt[0] || c.ts[i] != c.ts[0] || c.du[i] != c.du[0]) return 0;
-    const Sea::QfTables* T = s.qform_tables();
-    if (!T) return -3;
+    const Sea::QfTables* T = nullptr;
+    if (const int rc = s.qform_tables(&T); rc != 1) return rc;
     const int npad = T->npad;
     const int64_t ppad_all = (n + 127) / 128 * 128;
     const int64_t cap = std::max<int64_t>(128, ((int64_t)1 << 31) / ((int64_t)npad * 8) / 128 * 128);
@@ -395,9 +411,11 @@ int points_eta2_form(Sea& s, int64_t n, const double* dx, const double* dy, cons
     uint16_t* z = (uint16_t*)s.qz.get();
     Q.zc_hi = z; Q.zc_lo = z + zn; Q.zs_hi = z + 2 * zn; Q.zs_lo = z + 3 * zn;
     Q.zn = (int64_t)zn;
-    Q.b = (const uint16_t*)T->b.get();
+    // eta2 only, with three products per term in every mode
+    Q.b = (const uint16_t*)(T->nlv == 1 ? T->beta.get() : T->b.get());
+    Q.nbarr = 4; Q.nlv = 3;
     Q.ftype = (const int8_t*)T->ftype.get();
-    Q.nform = 1;                                       // eta2 only
+    Q.nform = 1;
     Q.kend = (const int32_t*)T->kend.get();
     Q.out = (float*)s.qo.get();
     Q.ostride = ppad_all;
@@ -556,7 +574,7 @@ int64_t Grid::field_count(int which) const {
     }
 }
 
-int Grid::init() {
+int Grid::init(bool fill) {
     if (d.nx < 1 || d.ny < 1 || d.nl < 1) {
         set_error("s-grid needs nx, ny, nl >= 1");
         return -1;
@@ -608,7 +626,7 @@ int Grid::init() {
     }
     if (const char* e = std::getenv("CFDWM_PRUNE")) prune_ = std::atoi(e) != 0;
     if (const char* e = std::getenv("CFDWM_QFORM")) qform_ = std::atoi(e) != 0;
-    return refill();
+    return fill ? refill() : 0;
 }
 
 template <typename R>
@@ -681,22 +699,39 @@ int Grid::fill_slots(int nslot, const double* slot_t, const int* slot_dst, bool 
 int Grid::qform_prepare(int nslot, const double* slot_t, int64_t ncol, const int32_t* cols) {
     qf_used = false;
     qf_clock_valid_ = false;                  // a measurement belongs to one fill
-    if (!qform_ || sea->prec != kFP32 || sea->nfilt() == 0) return 0;
+    qf_skip = QF_FP64;
+    if (sea->prec != kFP32) return 0;
+    qf_skip = QF_OFF;
+    if (!qform_) return 0;
+    qf_skip = QF_NO_PAIRS;
+    if (sea->nfilt() == 0) return 0;
     const CompTable& c = sea->tab.comp;
     for (int s = 0; s < 4; ++s) qf_scale_[s] = 1.0;
     for (int s = 0; s < nslot; ++s) {
         const double r0 = freq_ramp_host(slot_t[s], c.rt[0], c.ts[0], c.du[0]);
         for (int i = 1; i < c.n; ++i)
-            if (freq_ramp_host(slot_t[s], c.rt[i], c.ts[i], c.du[i]) != r0) return 0;
+            if (freq_ramp_host(slot_t[s], c.rt[i], c.ts[i], c.du[i]) != r0) {
+                qf_skip = QF_RAMPS;
+                warn_pair_loop("the components' frequency ramps differ at a slot time");
+                return 0;
+            }
         qf_scale_[s] = r0;
     }
     const double hs = sea->tab.in.depth;
     // the depth basis covers t = -z in [0, 2 hs] and t = 2 hs + z: the grid's levels must lie above -2 hs
-    if (g.h > 2.0 * hs) return 0;
-    const Sea::QfTables* T = sea->qform_tables();
-    if (!T) return -3;
+    qf_skip = QF_DEPTH;
+    if (g.h > 2.0 * hs) {
+        warn_pair_loop("the grid's water depth exceeds twice the sea's");
+        return 0;
+    }
+    const Sea::QfTables* T = nullptr;
+    if (const int rc = sea->qform_tables(&T); rc != 1) {
+        qf_skip = QF_BASIS;
+        if (rc == 0) warn_pair_loop("no depth basis within the tolerance at rank <= 32");
+        return rc;
+    }
+    qf_skip = QF_USED;
     const int npad = T->npad, R = T->rank;
-    if (R > 32) return 0;
     const int64_t P = (int64_t)nslot * ncol;
     const int64_t ppad_all = (P + 127) / 128 * 128;
     // Z operands of one chunk of points: 8 bytes per (point, component), at most 2 GiB
@@ -716,25 +751,49 @@ int Grid::qform_prepare(int nslot, const double* slot_t, int64_t ncol, const int
     uint16_t* z = (uint16_t*)zbuf_.get();
     Q.zc_hi = z; Q.zc_lo = z + zn; Q.zs_hi = z + 2 * zn; Q.zs_lo = z + 3 * zn;
     Q.zn = (int64_t)zn;
-    Q.b = (const uint16_t*)T->b.get();
     Q.ftype = (const int8_t*)T->ftype.get();
-    Q.nform = T->nform;
     Q.kend = (const int32_t*)T->kend.get();
     Q.out = (float*)qout_.get();
     Q.ostride = ppad_all;
     Q.p0 = 0;
     Q.P = std::min(qf_chunk_, P);
     Q.ppad = (Q.P + 127) / 128 * 128;
+    qf_split_ = T->nlv == 1;
+    if (!qf_split_) {                             // every form, three products per term
+        Q.b = (const uint16_t*)T->b.get();
+        Q.nbarr = 4; Q.nlv = 3;
+        Q.nform = T->nform;
+    } else {                                      // eta2 at three products; velocity forms at one
+        Q.b = (const uint16_t*)T->beta.get();
+        Q.nbarr = 4; Q.nlv = 3;
+        Q.nform = 1;
+        QfJob& V = qj2_;
+        V = Q;
+        V.b = (const uint16_t*)T->b.get();
+        V.nbarr = T->nbarr; V.nlv = 1;
+        V.ftype = Q.ftype + 1;
+        V.nform = T->nform - 1;
+        V.out = Q.out + ppad_all;
+    }
     // executed MFMA flops per padded point and form: per 64-wide i tile,
-    // 64 x (kend - i0) products, two operand pairs (c and s), three bf16 products
-    qf_fpp_ = 0;
-    for (size_t t = 0; t < T->kend_h.size(); ++t)
-        qf_fpp_ += 64.0 * std::max(0, T->kend_h[t] - (int)(64 * t)) * 2 * 3 * 2;
+    // 64 x (kend - i0) products, two operand pairs (c and s), 3 or 1 bf16 products
+    double tri = 0;
+    for (size_t t = 0; t < T->kend_h.size(); ++t) tri += 64.0 * std::max(0, T->kend_h[t] - (int)(64 * t)) * 2 * 2;
+    qf_fpp_ = 3 * tri;
+    qf_fpp1_ = tri;
+    qf_nlv = T->nlv;
     qf_nform = T->nform;
     qf_rank = R;
     qf_basis_err = T->basis_err;
     qf_flop = 0;
     return 1;
+}
+
+void Grid::warn_pair_loop(const char* why) {
+    if (qf_warned_ || std::getenv("CFDWM_QUIET")) return;
+    qf_warned_ = true;
+    std::fprintf(stderr, "cfdwm: FP32 s-grid fill runs the second-order pair loop instead of the matrix-core "
+                         "forms (%s); expect it to be much slower\n", why);
 }
 
 // Z operands and forms of every chunk of the prepared fill (one grid).
@@ -748,13 +807,27 @@ int Grid::qform_run() {
         Q.clk = (unsigned long long*)qf_clk_.get();
         qf_clock_valid_ = true;
     }
+    QfJob V = qj2_;
+    V.clk = Q.clk;
     for (int64_t p0 = 0; p0 < qf_P_; p0 += qf_chunk_) {
         Q.p0 = p0;
         Q.P = std::min(qf_chunk_, qf_P_ - p0);
         Q.ppad = (Q.P + 127) / 128 * 128;
         PROF("k_qf_zgen", launch_qf_zgen(Q, st));
-        PROF("k_qf_forms", launch_qf_forms(Q, st));
-        qf_flop += qf_fpp_ * (double)Q.ppad * qj_.nform;
+        if (qf_split_) {
+            // eta2 at three products and the velocity forms at one: the two
+            // launches are timed as one forms step
+            V.p0 = Q.p0; V.P = Q.P; V.ppad = Q.ppad;
+            Engine::get().prof_begin();
+            hipError_t e1 = launch_qf_forms(Q, st);
+            hipError_t e2 = e1 == hipSuccess ? launch_qf_forms(V, st) : e1;
+            Engine::get().prof_end("k_qf_forms");
+            CFDWM_HIP(e2);
+            qf_flop += (qf_fpp_ * Q.nform + qf_fpp1_ * V.nform) * (double)Q.ppad;
+        } else {
+            PROF("k_qf_forms", launch_qf_forms(Q, st));
+            qf_flop += qf_fpp_ * (double)Q.ppad * Q.nform;
+        }
     }
     return 0;
 }
@@ -776,7 +849,8 @@ int Grid::qform_clock(double* mhz) {
 
 // Point the column and level passes of the fill at the form values.
 void Grid::qform_attach(ColumnJob<float>& C, LevelJob<float>& L) {
-    const Sea::QfTables* T = sea->qform_tables();
+    const Sea::QfTables* T = nullptr;
+    (void)sea->qform_tables(&T);              // cached: qform_prepare built them
     const int R = qf_rank;
     const double hs = sea->tab.in.depth;
     qf_used = true;
@@ -835,7 +909,7 @@ int Grid::batch_begin() {
     if (rc != 1) return rc;
     if (qf_chunk_ < qf_P_) return 0;          // needs chunking: not batched
     PROF("k_qf_zgen", launch_qf_zgen(qj_, Engine::get().stream()));
-    qf_flop = qf_fpp_ * (double)qj_.ppad * qj_.nform;
+    qf_flop = (qf_fpp_ * qj_.nform + (qf_split_ ? qf_fpp1_ * qj2_.nform : 0.0)) * (double)qj_.ppad;
     batch_pending_ = true;
     return 1;
 }
@@ -852,7 +926,9 @@ int Grid::batch_end() {
     batch_pending_ = false;
     if (rc) return rc;
     if ((rc = gradients<float>())) return rc;
-    return pack<float>();
+    if ((rc = pack<float>())) return rc;
+    filled = true;
+    return 0;
 }
 
 void Grid::release_workspace() {
@@ -863,42 +939,64 @@ void Grid::release_workspace() {
 }
 
 namespace {
+// Shape of a batched persistent launch: tile order, grid size, and (order 1)
+// the group of gf forms x gp point tiles that consecutive tiles walk.
+struct BatchShape {
+    int order, blocks, gf, gp;
+};
+
 // Tile lists of a batched launch.  order 0: all tiles by decreasing cost
 // (point-tile cost ~ the job's i-range work), so round-robin blocks get equal
-// shares; order 1: job by job (decreasing cost), inside a job groups of 8
-// forms x 32 point tiles (cache locality of the job's tables and operands).
-std::vector<QfTile> batch_tiles(const std::vector<QfJob>& jobs, const std::vector<double>& cost, int order) {
+// shares; order 1: job by job (decreasing cost), inside a job groups of gf
+// forms x gp point tiles (cache locality of the job's tables and operands:
+// consecutive blocks share a group's B tables and Z tiles).
+std::vector<QfTile> batch_tiles(const std::vector<QfJob>& jobs, const std::vector<double>& cost, const BatchShape& b) {
     std::vector<int> idx(jobs.size());
     for (size_t j = 0; j < jobs.size(); ++j) idx[j] = (int)j;
-    std::stable_sort(idx.begin(), idx.end(), [&](int a, int b) { return cost[a] > cost[b]; });
+    std::stable_sort(idx.begin(), idx.end(), [&](int a, int c) { return cost[a] > cost[c]; });
     std::vector<QfTile> t;
-    if (order == 0) {
+    if (b.order == 0) {
         for (int j : idx) {
             const int npt = (int)(jobs[j].ppad / 128);
             for (int f = 0; f < jobs[j].nform; ++f)
                 for (int p = 0; p < npt; ++p) t.push_back(QfTile{j, p, f, 0});
         }
-        std::stable_sort(t.begin(), t.end(), [&](const QfTile& a, const QfTile& b) { return cost[a.job] > cost[b.job]; });
+        std::stable_sort(t.begin(), t.end(), [&](const QfTile& a, const QfTile& c) { return cost[a.job] > cost[c.job]; });
     } else {
+        const int gf = std::max(1, b.gf), gp = std::max(1, b.gp);
         for (int j : idx) {
             const int npt = (int)(jobs[j].ppad / 128), nf = jobs[j].nform;
-            for (int f0 = 0; f0 < nf; f0 += 8)
-                for (int p0 = 0; p0 < npt; p0 += 32)
-                    for (int f = f0; f < std::min(nf, f0 + 8); ++f)
-                        for (int p = p0; p < std::min(npt, p0 + 32); ++p) t.push_back(QfTile{j, p, f, 0});
+            for (int f0 = 0; f0 < nf; f0 += gf)
+                for (int p0 = 0; p0 < npt; p0 += gp)
+                    for (int f = f0; f < std::min(nf, f0 + gf); ++f)
+                        for (int p = p0; p < std::min(npt, p0 + gp); ++p) t.push_back(QfTile{j, p, f, 0});
         }
     }
     return t;
 }
 
+// Candidates of the batched-launch tuner: cost-ordered tiles, or job-ordered
+// tiles in groups of 8 x 32, 4 x 64 or 16 x 16 (forms x point tiles), each at
+// one, two or four blocks per CU of the 256 CUs.
+const std::vector<BatchShape>& batch_candidates() {
+    static const std::vector<BatchShape> c = {
+        {0, 256, 0, 0}, {0, 1024, 0, 0},
+        {1, 256, 8, 32}, {1, 512, 8, 32}, {1, 1024, 8, 32},
+        {1, 256, 4, 64}, {1, 1024, 4, 64},
+        {1, 256, 16, 16}, {1, 1024, 16, 16},
+    };
+    return c;
+}
+
 struct TuneKey {
-    int jobs_log2, tiles_log2;
+    int nlv, jobs_log2, tiles_log2;
     bool operator<(const TuneKey& o) const {
+        if (nlv != o.nlv) return nlv < o.nlv;
         return jobs_log2 != o.jobs_log2 ? jobs_log2 < o.jobs_log2 : tiles_log2 < o.tiles_log2;
     }
 };
-std::map<TuneKey, std::pair<int, int>>& tune_cache() {
-    static std::map<TuneKey, std::pair<int, int>> m;
+std::map<TuneKey, BatchShape>& tune_cache() {
+    static std::map<TuneKey, BatchShape> m;
     return m;
 }
 std::mutex& tune_mutex() {
@@ -930,50 +1028,69 @@ int refill_batched(Grid** grids, int n, int64_t budget_bytes, bool release, Batc
             Grid* g = grids[i];
             const int64_t npad = (g->sea->tab.comp.n + 127) / 128 * 128;
             const int64_t pts = 4LL * g->g.nx * g->g.ny;
-            const int64_t est = 61 * 8 * npad * npad + pts * npad * 8 + 61 * pts * 4;
+            const int64_t tb = g->sea->qf_products == 1 ? 4 : 8;    // table bytes per entry and form
+            const int64_t est = 61 * tb * npad * npad + pts * npad * 8 + 61 * pts * 4;
             if (!grp.empty() && bytes + est > budget_bytes) break;
             grp.push_back(g);
             bytes += est;
             ++i;
         }
         ++S.groups;
-        std::vector<QfJob> jobs;
-        std::vector<double> cost;
+        // per product mode (3: hi/lo split, 1: hi only) the jobs of the group
+        // and their costs; a grid whose velocity forms take one product adds
+        // its eta2 form to the three-product list and its velocity forms to
+        // the one-product list
+        std::vector<QfJob> jobs3, jobs1;
+        std::vector<double> cost3, cost1;
         std::vector<char> joined(grp.size(), 0);
         for (size_t k = 0; k < grp.size(); ++k) {
+            if (grp[k]->sea->prec == kFP32) {
+                // the sea's form tables (built here unless still resident)
+                const auto c0 = std::chrono::steady_clock::now();
+                const Sea::QfTables* T = nullptr;
+                if (grp[k]->sea->qform_tables(&T) < 0) { cleanup(); return -3; }
+                S.tables_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - c0).count();
+            }
             const int rc = grp[k]->batch_begin();
             if (rc < 0) { cleanup(); return rc; }
             if (rc == 1) {
                 joined[k] = 1;
-                jobs.push_back(grp[k]->batch_job());
                 double c = 0;
-                const auto& T = *grp[k]->sea->qform_tables();
-                for (size_t t = 0; t < T.kend_h.size(); ++t) c += std::max(0, T.kend_h[t] - (int)(64 * t));
-                cost.push_back(c);
+                const Sea::QfTables* T = nullptr;
+                (void)grp[k]->sea->qform_tables(&T);  // cached by batch_begin
+                for (size_t t = 0; t < T->kend_h.size(); ++t) c += std::max(0, T->kend_h[t] - (int)(64 * t));
+                jobs3.push_back(grp[k]->batch_job());
+                cost3.push_back(c);
+                if (grp[k]->batch_split()) {
+                    jobs1.push_back(grp[k]->batch_job2());
+                    cost1.push_back(c);
+                }
                 S.mfma_flop += grp[k]->qf_flop;
             }
         }
-        if (!jobs.empty()) {
-            if (djobs.upload(jobs.data(), jobs.size() * sizeof(QfJob), st)) { cleanup(); return -3; }
+        // one persistent launch per product mode, its shape from the tuner
+        auto launch_list = [&](const std::vector<QfJob>& jobs, const std::vector<double>& cost, int nlv) -> int {
+            if (jobs.empty()) return 0;
+            if (djobs.upload(jobs.data(), jobs.size() * sizeof(QfJob), st)) return -3;
             int64_t ntile = 0;
             for (const QfJob& J : jobs) ntile += (J.ppad / 128) * J.nform;
-            const TuneKey key{ilog2((int64_t)jobs.size()), ilog2(ntile)};
-            std::pair<int, int> choice{-1, -1};
+            const TuneKey key{nlv, ilog2((int64_t)jobs.size()), ilog2(ntile)};
+            BatchShape choice{-1, 0, 0, 0};
             {
                 std::lock_guard<std::mutex> lk(tune_mutex());
                 auto it = tune_cache().find(key);
                 if (it != tune_cache().end()) choice = it->second;
             }
             if (const char* e = std::getenv("CFDWM_BATCH_SHAPE")) {
-                int o = 0, b = 256;
-                if (std::sscanf(e, "%d,%d", &o, &b) == 2) choice = {o, b};
+                BatchShape b{0, 256, 8, 32};
+                if (std::sscanf(e, "%d,%d,%d,%d", &b.order, &b.blocks, &b.gf, &b.gp) >= 2) choice = b;
             }
-            auto run = [&](int order, int blocks, float* ms) -> int {
-                const std::vector<QfTile> tl = batch_tiles(jobs, cost, order);
+            auto run = [&](const BatchShape& b, float* ms) -> int {
+                const std::vector<QfTile> tl = batch_tiles(jobs, cost, b);
                 if (dtiles.upload(tl.data(), tl.size() * sizeof(QfTile), st)) return -3;
                 CFDWM_HIP(hipEventRecord(e0, st));
                 CFDWM_HIP(launch_qf_persist((const QfJob*)djobs.get(), (const QfTile*)dtiles.get(), (int)tl.size(),
-                                            blocks, st));
+                                            b.blocks, nlv, st));
                 CFDWM_HIP(hipEventRecord(e1, st));
                 CFDWM_HIP(hipEventSynchronize(e1));
                 CFDWM_HIP(hipEventElapsedTime(ms, e0, e1));
@@ -981,26 +1098,31 @@ int refill_batched(Grid** grids, int n, int64_t budget_bytes, bool release, Batc
                 return 0;
             };
             float ms = 0;
-            if (choice.first < 0) {
-                // autotune: every candidate computes the same values; keep the fastest
-                const int cand[4][2] = {{0, 256}, {1, 256}, {0, 1024}, {1, 1024}};
+            if (choice.order < 0) {
+                // autotune on this launch: every candidate computes the same
+                // values; keep the fastest
                 float best = 1e30f;
-                for (auto& c : cand) {
+                for (const BatchShape& c : batch_candidates()) {
                     float t = 0;
-                    if (int rc = run(c[0], c[1], &t)) { cleanup(); return rc; }
-                    if (t < best) { best = t; choice = {c[0], c[1]}; }
+                    if (int rc = run(c, &t)) return rc;
+                    if (t < best) { best = t; choice = c; }
                 }
                 ms = best;
                 S.tuned = 1;
                 std::lock_guard<std::mutex> lk(tune_mutex());
                 tune_cache()[key] = choice;
             } else {
-                if (int rc = run(choice.first, choice.second, &ms)) { cleanup(); return rc; }
+                if (int rc = run(choice, &ms)) return rc;
             }
             S.forms_ms += ms;
-            S.order = choice.first;
-            S.blocks = choice.second;
-        }
+            S.order = choice.order;
+            S.blocks = choice.blocks;
+            S.group_forms = choice.gf;
+            S.group_ptiles = choice.gp;
+            return 0;
+        };
+        if (int rc = launch_list(jobs3, cost3, 3)) { cleanup(); return rc; }
+        if (int rc = launch_list(jobs1, cost1, 1)) { cleanup(); return rc; }
         for (size_t k = 0; k < grp.size(); ++k) {
             const int rc = joined[k] ? grp[k]->batch_end() : grp[k]->refill();
             if (rc) { cleanup(); return rc; }
@@ -1066,10 +1188,12 @@ int Grid::refill() {
     rc = prec == kFP32 ? pack<float>() : pack<double>();
     if (rc) return rc;
     CFDWM_HIP(hipStreamSynchronize(Engine::get().stream()));
+    filled = true;
     return 0;
 }
 
 int Grid::force_update(double t, int* n) {
+    if (!filled) { set_error("the s-grid has not been filled yet (created with defer_fill)"); return -1; }
     int k = 0;
     // CheckTime / update (lsgrid_spline.cpp:610-617, 847-931)
     while (t > t0 + d.dt) {
@@ -1116,6 +1240,7 @@ int64_t Grid::export_field(int which, double* out) {
 // hexahedra between neighbouring nodes (quadrilaterals or lines when the grid
 // is flat in x or y), and the time as field data.
 int Grid::write_vtu(const char* path, bool last, const char* time_label) {
+    if (!filled) { set_error("the s-grid has not been filled yet (created with defer_fill)"); return -1; }
     const int nx = g.nx, ny = g.ny, nl = g.nl;
     const int64_t nc = (int64_t)nx * ny, nv = nc * nl;
     const int tt = last ? (tstep + 2) % 4 : (tstep + 1) % 4;
@@ -1190,6 +1315,7 @@ constexpr int64_t kQuerySortMin = 65536;
 template <typename R, typename TQ>
 int Grid::run_query(int64_t n, const TQ* t, double tconst, const TQ* x, const TQ* y, const TQ* z,
                     TQ* eta, TQ* u, TQ* v, TQ* w, const double* ramp) {
+    if (!filled) { set_error("the s-grid has not been filled yet (created with defer_fill)"); return -1; }
     QueryJob<R, TQ> J;
     J.n = n; J.g = g; J.tstep = tstep; J.t0 = t0; J.dt = d.dt;
     J.tan_a = d.tan_a; J.tan_b = d.tan_b; J.tanab = std::pow(std::tan(d.tan_a), d.tan_b);

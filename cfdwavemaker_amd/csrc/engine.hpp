@@ -162,7 +162,7 @@ hipError_t launch_query_sort(int64_t n, const GridGeom& g, const TQ* x, const TQ
 hipError_t launch_qf_zgen(const QfJob& J, hipStream_t st);
 hipError_t launch_qf_forms(const QfJob& J, hipStream_t st);
 hipError_t launch_qf_build(const QfBuild& B, hipStream_t st);
-hipError_t launch_qf_persist(const QfJob* jobs, const QfTile* tiles, int ntiles, int blocks, hipStream_t st);
+hipError_t launch_qf_persist(const QfJob* jobs, const QfTile* tiles, int ntiles, int blocks, int nlv, hipStream_t st);
 // eta range of the columns of a fill (minus swl) -> eb[2]; then per filtered
 // pair record the levels where its profile exponentials exceed e^{-T}
 template <typename R>
@@ -272,15 +272,27 @@ public:
     // (cos-type) and w (sin-type) forms
     struct QfTables {
         int npad = 0, nform = 0, rank = 0;
-        double basis_err = 0;
+        int nlv = 3, nbarr = 4;              // bf16 products per term, table arrays per form
+        double basis_err = 0, basis_tol = 0;
         std::vector<double> mu;              // skeleton wavenumbers of the depth basis
-        DevBuf b;                            // [nform][4][npad x npad] tiled
+        // [nform][nbarr][npad x npad] tiled; with one product per term (nlv 1)
+        // it holds the 3 R velocity forms (hi parts) and beta the eta2 form
+        // (hi and lo parts: the elevation keeps three products per term)
+        DevBuf b, beta;
         DevBuf ftype;                        // [nform] int8
         DevBuf kend;                         // per 64-wide tile of i
         std::vector<int32_t> kend_h;
     };
-    const QfTables* qform_tables();          // FP32 seas only
+    // 1 and *out set: tables built (or cached); 0: the forms do not apply to
+    // this sea (FP64, or no depth basis within the tolerance at rank <= 32:
+    // the fills then run the pair loop); < 0: device error
+    int qform_tables(const QfTables** out);
     void release_qform();
+    // bf16 products per term of the forms (3: hi/lo split, 1: hi only) and the
+    // depth-basis tolerance (0: 1e-6 for 3 products, 1e-4 for 1 product)
+    int qf_products = 3;
+    double qf_tol = 0;
+    bool qf_fallback = false;                // the last qform_tables call found the forms not applicable
     // vertical_theory (Irregular.h:83): 1 = Smit et al. s-coordinate
     // kinematics for u, v, w, p at points (elevation stays Sharma-Dean)
     int vertical_theory = 0;
@@ -321,7 +333,8 @@ class Grid {
 public:
     Grid(Sea* sea, const GridDesc& d);
     ~Grid();
-    int init();                              // allocate + initialize_kinematics
+    int init(bool fill = true);              // allocate (+ initialize_kinematics)
+    bool filled = false;                     // a fill has run (queries need the packed slots)
     int refill();                            // initialize_kinematics again (tstep back to 0)
     int force_update(double t, int* n);      // wave_force_update
     int64_t export_field(int which, double* out);
@@ -342,7 +355,14 @@ public:
     int64_t points_per_fill() const { return 4LL * g.nx * g.ny * g.nl; }
     // quadratic forms of the last fill (measurement)
     bool qf_used = false;
-    int qf_nform = 0, qf_rank = 0;
+    // why the last fill did or did not use the forms (QF_USED when it did)
+    enum QfSkip { QF_USED = 0, QF_FP64 = 1, QF_OFF = 2, QF_NO_PAIRS = 3, QF_RAMPS = 4, QF_DEPTH = 5, QF_BASIS = 6 };
+    int qf_skip = QF_FP64;
+    // an FP32 fill that takes the pair loop for a reason other than FP64 or an
+    // explicit switch runs about 14x slower: say so once per grid on stderr
+    void warn_pair_loop(const char* why);
+    bool qf_warned_ = false;
+    int qf_nform = 0, qf_rank = 0, qf_nlv = 3;
     double qf_flop = 0, qf_basis_err = 0;
     // shader clock of the forms launches of later fills (diagnostic)
     void qform_clock_enable(bool on) { qf_clock_on_ = on; }
@@ -358,7 +378,11 @@ private:
     bool qform_ = true;                       // FP32: eta2 pair sum as an MFMA quadratic form
     DevBuf zbuf_;                             // its Z operands (one chunk of points)
     DevBuf qout_;                             // form values [nform][points of the fill]
-    QfJob qj_{};                              // prepared forms launch (first chunk)
+    QfJob qj_{};                              // prepared forms launch (first chunk): all forms, or
+                                              // the eta2 form alone when qj2_ holds the velocity forms
+    QfJob qj2_{};                             // velocity forms at one product per term (split fills)
+    bool qf_split_ = false;
+    double qf_fpp1_ = 0;                      // executed flops per point and velocity form (split fills)
     int64_t qf_chunk_ = 0, qf_P_ = 0;
     bool qf_clock_on_ = false, qf_clock_valid_ = false;
     DevBuf qf_clk_;                           // 2 counters of the clock measurement
@@ -375,6 +399,8 @@ public:
     int batch_begin();
     int batch_end();
     const QfJob& batch_job() const { return qj_; }
+    bool batch_split() const { return qf_split_; }
+    const QfJob& batch_job2() const { return qj2_; }
     void release_workspace();
 private:
     // fields: 0 ETA(4) 1 ETAdx(2) 2 ETAdy(2) 3 ETAdt(4) 4..6 U/V/W(4) 7..9 dt(4) 10..18 dx/dy/ds(2)
@@ -401,6 +427,8 @@ private:
 struct BatchStats {
     int groups = 0, batched = 0, single = 0, tiles = 0, blocks = 0, order = 0, tuned = 0;
     double forms_ms = 0, mfma_flop = 0;
+    int group_forms = 0, group_ptiles = 0;   // tile grouping of the tuned launch (order 1)
+    double tables_ms = 0;                    // form tables built during the call (host + device)
 };
 int refill_batched(Grid** grids, int n, int64_t budget_bytes, bool release, BatchStats* stats);
 

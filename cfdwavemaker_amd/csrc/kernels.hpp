@@ -98,7 +98,10 @@ struct QfJob {
     const PhaseRec* ph = nullptr;
     uint16_t *zc_hi = nullptr, *zc_lo = nullptr, *zs_hi = nullptr, *zs_lo = nullptr;   // ppad x npad
     int64_t zn = 0;                       // elements between the four Z arrays (zc_lo = zc_hi + zn, ...)
-    const uint16_t* b = nullptr;          // [nform][B1 hi, B1 lo, B2 hi, B2 lo][npad x npad]
+    const uint16_t* b = nullptr;          // [nform][nbarr][npad x npad]: B1 hi, B1 lo, B2 hi, B2 lo (nbarr 4)
+                                          // or B1 hi, B2 hi (nbarr 2)
+    int nbarr = 4;
+    int nlv = 3;                          // bf16 products per term: 3 (hi/lo split) or 1 (hi only, nbarr 2 or 4)
     const int8_t* ftype = nullptr;        // per form: 0 cos-type, 1 sin-type
     int nform = 0;
     const int32_t* kend = nullptr;        // per 64-wide tile of i: end of its m range (multiple of 16)
@@ -123,7 +126,10 @@ struct QfBuild {
     const XRec<float>* x = nullptr;
     const double* tau = nullptr;          // [R] skeleton depths
     const double* einv = nullptr;         // [R][R]
-    uint16_t* b = nullptr;                // [nform][4][npad x npad], zeroed by the caller
+    uint16_t* b = nullptr;                // [nform][nbarr][npad x npad], zeroed by the caller
+    int nbarr = 4;                        // 4: hi and lo parts, 2: hi parts only
+    uint16_t* beta = nullptr;             // when set: form 0 (eta2) goes here as [4][npad x npad]
+                                          // and form f > 0 to b at index f - 1
 };
 
 // Smit et al. s-coordinate point evaluation (smit.hip).  eta1/eta2 are the

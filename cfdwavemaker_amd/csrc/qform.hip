@@ -1,6 +1,6 @@
 // Filtered second-order elevation eta2 as a quadratic form on the matrix cores.
 //
-// The pair sum of Irregular::eta2 (src/Irregular.cpp:1125-1192, pair-cache
+// The pair sum of Irregular::eta2 (src/Irregular.cpp:223-293, pair-cache
 // semantics of lsGridSpline's column update) is, per point p,
 //
 //   eta2_p = sum_i r_i [ c_pi sum_{m>i} Bcc_im c_pm + s_pi sum_{m>i} Bss_im s_pm ]
@@ -9,9 +9,11 @@
 // of component i at the point's time, Bcc = xP + xM and Bss = xM - xP built
 // from the filtered pair coefficients (the real form of Re(Z^T B+ Z + Z^H B- Z)
 // with Z_i = e^{i theta_i}).  The inner sums are GEMMs (points x components
-// times components x components) on v_mfma_f32_32x32x16_bf16; every operand is
-// split into bf16 hi + lo parts and the three products hi*hi + hi*lo + lo*hi
-// are accumulated in FP32, which keeps about 16 significant bits per product.
+// times components x components) on v_mfma_f32_32x32x16_bf16.  In the
+// three-product mode every operand is split into bf16 hi + lo parts and the
+// products hi*hi + hi*lo + lo*hi are accumulated in FP32, which keeps about 16
+// significant bits per product; the one-product mode takes hi*hi alone (about
+// 8 bits per product, a third of the matrix work and half the staged bytes).
 // The outer sum over i is a fused epilogue (accumulator times c or s, summed
 // in the lane, two exchanges at the end); the column kernel folds the result into
 // its FP64 total.  The same kernel evaluates the z-independent forms of the
@@ -143,29 +145,42 @@ __device__ __forceinline__ gu16p uni_ptr(const uint16_t* p) {
 
 // Operands move global -> LDS directly (global_load_lds_dwordx4, no
 // registers) through four LDS stages, three steps ahead of the products.
-// Step j of an i tile:
-//   1. s_waitcnt vmcnt(8) lgkmcnt(0) + s_barrier: the requests of step j + 1
-//      (issued during step j - 2) have landed for all waves, and all waves'
-//      LDS reads of step j - 1 are done;
-//   2. the 24 products of step j (operand set X) are issued interleaved with
-//      the 16 LDS reads of step j + 1's operands (set Y, stage (j + 1) % 4)
-//      and the 8 requests of step j + 3 (stage (j + 3) % 4 = (j - 1) % 4,
-//      last read during step j - 2, before step j - 1's barrier), so the
-//      matrix pipe stays busy while they issue.
+// NLV is the number of bf16 products per term: 3 (hi*hi + lo*hi + hi*lo of
+// table and Z operands, about 16 significant bits) or 1 (hi*hi, about 8
+// significant bits; only the hi arrays are staged).  A stage holds NA = 4
+// (NLV 3) or 2 (NLV 1) operand arrays for the block's 128 points and 128 i,
+// 8 NA KiB.  Step j of an i tile:
+//   1. s_waitcnt vmcnt(2 NA) lgkmcnt(0) + s_barrier: the requests of step
+//      j + 1 (issued during step j - 2) have landed for all waves, and all
+//      waves' LDS reads of step j - 1 are done;
+//   2. the 8 NLV products of step j (operand set X) are issued interleaved
+//      with the 4 NA LDS reads of step j + 1's operands (set Y, stage
+//      (j + 1) % 4) and the 2 NA requests of step j + 3 (stage (j + 3) % 4 =
+//      (j - 1) % 4, last read during step j - 2, before step j - 1's
+//      barrier), so the matrix pipe stays busy while they issue.
 // The barrier is an asm statement with a memory clobber after an explicit
 // s_waitcnt: __syncthreads() would also wait for the requests of steps
 // j + 2 and j + 3.  Every wave runs every step of the block's k range: table
 // entries below the pair triangle (m <= i) and past a row tile's filtered end
 // are zero, so the extra products add exact zeros, and the step body has no
 // branch.
+template <int NLV>
 __device__ __forceinline__ void qf_tile_il(const QfJob& J, int ptile, int form) {
-    constexpr int NB = 4;
-    __shared__ __attribute__((aligned(16))) char lds[NB][32768];
+    static_assert(NLV == 1 || NLV == 3, "one or three bf16 products per term");
+    constexpr int NB = 4;                         // stages
+    constexpr int NA = NLV == 1 ? 2 : 4;          // operand arrays per side in a stage
+    constexpr int SB = 8192 * NA;                 // stage bytes
+    constexpr int NF = 4 * NA;                    // operand fragments per set
+    constexpr int NG = 2 * NA;                    // request chunks per step = issue groups
+    constexpr int PG = 8 * NLV / NG;              // products per issue group (3 or 2)
+    __shared__ __attribute__((aligned(16))) char lds[NB][SB];
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const int wp = w & 1, wi = w >> 1;
     const int nkb = J.npad >> 4;
     const size_t nn = (size_t)J.npad * J.npad;
-    const uint16_t* __restrict__ Bf = J.b + (size_t)form * 4 * nn;
+    // tables: 4 arrays per form (B1 hi, B1 lo, B2 hi, B2 lo) or 2 (B1 hi, B2 hi)
+    const uint16_t* __restrict__ Bf = J.b + (size_t)form * J.nbarr * nn;
+    const int bhs = J.nbarr == 4 ? 2 : 1;         // arrays between B1 hi and B2 hi
     const bool sint = J.ftype[form] != 0;
     const int64_t PB = (int64_t)ptile * 128;
     const int64_t rbB = PB >> 5;
@@ -180,22 +195,22 @@ __device__ __forceinline__ void qf_tile_il(const QfJob& J, int ptile, int form) 
 #define QF4_SYNC(VM)                                                                 \
     __builtin_amdgcn_s_waitcnt(((VM) & 15) | (7 << 4) | (((VM) >> 4) << 14));        \
     __asm__ volatile("s_barrier" ::: "memory")
-    // chunk r (< 8) of a stage: A (array r, row block rbB + wv) for r < 4,
-    // B (array r - 4, i row block ib0 + wv) for r >= 4; gp[r] is its address
+    // chunk r (< 2 NA) of a stage: A (array r, row block rbB + wv) for r < NA,
+    // B (array r - NA, i row block ib0 + wv) for r >= NA; gp[r] is its address
     // at k step 0 of the current i tile (wave-uniform)
-    const uint16_t* gp[8];
+    const uint16_t* gp[NG];
     auto dma = [&](int r, int kb, int st) {
         __builtin_amdgcn_global_load_lds((glb_vp)(uni_ptr(gp[r] + kb * 512) + lo8),
-                                         (lds_vp)(lds[0] + st * 32768 + (r * 4 + wv) * 1024), 16, 0, 0);
+                                         (lds_vp)(lds[0] + st * SB + (r * 4 + wv) * 1024), 16, 0, 0);
     };
-    // operand fragment f of a stage: f < 8 A (a2 = f / 4, array f % 4), else
-    // B (b = (f - 8) / 4, array f % 4)
+    // operand fragment f of a stage: f < 2 NA A (a2 = f / NA, array f % NA),
+    // else B (b = (f - 2 NA) / NA, array f % NA)
     auto frag = [&](int st, int f) {
-        const char* L = lds[0] + st * 32768;
-        return f < 8 ? lds8(L, ((f & 3) * 4 + wp * 2 + (f >> 2)) * 1024 + loff)
-                     : lds8(L, 16384 + ((f & 3) * 4 + wi * 2 + ((f - 8) >> 2)) * 1024 + loff);
+        const char* L = lds[0] + st * SB;
+        return f < 2 * NA ? lds8(L, ((f % NA) * 4 + wp * 2 + f / NA) * 1024 + loff)
+                          : lds8(L, NA * 4096 + ((f % NA) * 4 + wi * 2 + (f - 2 * NA) / NA) * 1024 + loff);
     };
-    bf16x8 opA[16], opB[16];
+    bf16x8 opA[NF], opB[NF];
     for (int it = 0; it < J.npad; it += 128) {
         const int t0 = it >> 6;
         const int kb_lo = it >> 4;
@@ -209,50 +224,57 @@ __device__ __forceinline__ void qf_tile_il(const QfJob& J, int ptile, int form) 
         for (int q = 0; q < 8; ++q)
 #pragma unroll
             for (int r = 0; r < 16; ++r) acc[q][r] = 0.f;
-        // product s (< 24) of a step: level s / 8 (hi x hi, hi x lo, lo x hi),
-        // accumulator s % 8; an accumulator's products are 8 issues apart
-        auto mma = [&](const bf16x8 (&op)[16], int s) {
+        // product s (< 8 NLV) of a step: level s / 8 (hi x hi, lo x hi, hi x
+        // lo of table x Z), accumulator s % 8; an accumulator's products are
+        // 8 issues apart
+        auto mma = [&](const bf16x8 (&op)[NF], int s) {
             const int lv = s >> 3, q = s & 7, cs = q & 1, a2 = q >> 2, b = (q >> 1) & 1;
-            const int fa = a2 * 4 + 2 * cs + (lv == 2 ? 1 : 0);
-            const int fb = 8 + b * 4 + 2 * cs + (lv == 1 ? 1 : 0);
+            const int fa = NLV == 1 ? a2 * NA + cs : a2 * 4 + 2 * cs + (lv == 2 ? 1 : 0);
+            const int fb = NLV == 1 ? 2 * NA + b * NA + cs : 8 + b * 4 + 2 * cs + (lv == 1 ? 1 : 0);
             acc[q] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(op[fb], op[fa], acc[q], 0, 0, 0);
         };
         // one step: products of operand set x, reads of step j + 1 into set y
-        // from stage s1, requests of step kbd into stage s3; 8 chunks of 3
+        // from stage s1, requests of step kbd into stage s3; NG chunks of PG
         // products, 2 reads and 1 request in a fixed issue order
-        auto step = [&](const bf16x8 (&x)[16], bf16x8 (&y)[16], int s1, int s3, int kbd) {
+        auto step = [&](const bf16x8 (&x)[NF], bf16x8 (&y)[NF], int s1, int s3, int kbd) {
 #pragma unroll
-            for (int g = 0; g < 8; ++g) {
-                mma(x, 3 * g);
+            for (int g = 0; g < NG; ++g) {
+                mma(x, PG * g);
                 y[2 * g] = frag(s1, 2 * g);
-                mma(x, 3 * g + 1);
+                mma(x, PG * g + 1);
                 y[2 * g + 1] = frag(s1, 2 * g + 1);
                 dma(g, kbd, s3);
-                mma(x, 3 * g + 2);
+                if (PG > 2) mma(x, PG * g + 2);
                 __builtin_amdgcn_sched_barrier(0);
             }
         };
+        // global array of chunk r: A arrays c_hi, c_lo, s_hi, s_lo (NLV 3) or
+        // c_hi, s_hi (NLV 1); B arrays likewise
 #pragma unroll
-        for (int r = 0; r < 8; ++r)
-            gp[r] = r < 4 ? J.zc_hi + (int64_t)r * J.zn + (rbB + wv) * nkb * 512
-                          : Bf + (size_t)(r - 4) * nn + (int64_t)(ib0 + wv) * nkb * 512;
+        for (int r = 0; r < NG; ++r) {
+            const int a = r % NA;
+            const int ga = NLV == 1 ? 2 * a : a;                       // Z array index
+            const int gb = NLV == 1 ? a * bhs : a;                     // table array index
+            gp[r] = r < NA ? J.zc_hi + (int64_t)ga * J.zn + (rbB + wv) * nkb * 512
+                           : Bf + (size_t)gb * nn + (int64_t)(ib0 + wv) * nkb * 512;
+        }
         // requests past the last step re-read step kb_hi - 1 (in bounds, unused)
         const int kl = kb_hi - 1;
 #pragma unroll
         for (int d = 0; d < 3; ++d)
 #pragma unroll
-            for (int r = 0; r < 8; ++r) dma(r, min(kb_lo + d, kl), d);
-        QF4_SYNC(16);
+            for (int r = 0; r < NG; ++r) dma(r, min(kb_lo + d, kl), d);
+        QF4_SYNC(2 * NG);
 #pragma unroll
-        for (int f = 0; f < 16; ++f) opA[f] = frag(0, f);
+        for (int f = 0; f < NF; ++f) opA[f] = frag(0, f);
         int s3 = 3, s1 = 1;                           // stages of steps j + 3 and j + 1
         for (int kb = kb_lo; kb < kb_hi; kb += 2) {
-            QF4_SYNC(8);
+            QF4_SYNC(NG);
             step(opA, opB, s1, s3, min(kb + 3, kl));
             if (kb + 1 >= kb_hi) break;               // block-uniform
             s3 = (s3 + 1) & 3;
             s1 = (s1 + 1) & 3;
-            QF4_SYNC(8);
+            QF4_SYNC(NG);
             step(opB, opA, s1, s3, min(kb + 4, kl));
             s3 = (s3 + 1) & 3;
             s1 = (s1 + 1) & 3;
@@ -306,6 +328,7 @@ __device__ __forceinline__ void qf_tile_il(const QfJob& J, int ptile, int form) 
 }
 
 // Single-job launch: block -> (point tile, form) through the swizzle.
+template <int NLV>
 __global__ __launch_bounds__(256) void k_qf_forms_il(QfJob J, int npt) {
     const int id = blockIdx.x;
     const int per = kQfGF * kQfGP;
@@ -316,11 +339,11 @@ __global__ __launch_bounds__(256) void k_qf_forms_il(QfJob J, int npt) {
     const int form = fg * kQfGF + in / kQfGP;
     if (ptile >= npt || form >= J.nform) return;      // whole block, before any barrier
     if (!J.clk) {
-        qf_tile_il(J, ptile, form);
+        qf_tile_il<NLV>(J, ptile, form);
         return;
     }
     const unsigned long long c0 = clock64(), w0 = wall_clock64();
-    qf_tile_il(J, ptile, form);
+    qf_tile_il<NLV>(J, ptile, form);
     if (threadIdx.x == 0) {
         atomicAdd(J.clk, clock64() - c0);
         atomicAdd(J.clk + 1, wall_clock64() - w0);
@@ -330,13 +353,14 @@ __global__ __launch_bounds__(256) void k_qf_forms_il(QfJob J, int npt) {
 // Persistent launch over the tiles of many jobs (sea states): block b takes
 // tiles b, b + gridDim.x, ... of the ordered tile list; every block exits
 // after its last tile.
+template <int NLV>
 __global__ __launch_bounds__(256) void k_qf_forms_persist(const QfJob* __restrict__ jobs,
                                                           const QfTile* __restrict__ tiles, int ntiles) {
     for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
         const QfTile tl = tiles[t];
         const QfJob J = jobs[tl.job];
         __syncthreads();                              // the previous tile's LDS readers are done
-        qf_tile_il(J, tl.ptile, tl.form);
+        qf_tile_il<NLV>(J, tl.ptile, tl.form);
     }
 }
 
@@ -363,11 +387,14 @@ __global__ __launch_bounds__(256) void k_qf_build(QfBuild B) {
     const size_t nn = (size_t)B.npad * B.npad;
     const size_t o = ((size_t)(i >> 5) * nkb + (m >> 4)) * 512 + ((m >> 3) & 1) * 256 + (i & 31) * 8 + (m & 7);
     auto put = [&](int f, int k, double v) {
-        uint16_t* d = B.b + ((size_t)f * 4 + 2 * k) * nn;
+        const bool sep = B.beta && f == 0;            // the eta2 form in its own hi/lo table
+        const bool lo = sep || B.nbarr == 4;
+        const int fi = B.beta ? f - 1 : f;
+        uint16_t* d = sep ? B.beta + (size_t)(2 * k) * nn : B.b + ((size_t)fi * B.nbarr + (lo ? 2 * k : k)) * nn;
         const float x = (float)v;
         const uint32_t hi = bf16_bits(x);
         d[o] = (uint16_t)hi;
-        d[o + nn] = (uint16_t)bf16_bits(x - bf16_val(hi));
+        if (lo) d[o + nn] = (uint16_t)bf16_bits(x - bf16_val(hi));
     };
     put(0, 0, (double)xr.xP + (double)xr.xM);
     put(0, 1, (double)xr.xM - (double)xr.xP);
@@ -394,12 +421,14 @@ __global__ __launch_bounds__(256) void k_qf_build(QfBuild B) {
 }  // namespace
 
 hipError_t launch_qf_build(const QfBuild& B, hipStream_t st) {
-    if (B.n <= 0 || B.R < 0 || B.R > 32 || (B.npad & 127) || B.npad < B.n) return hipErrorInvalidValue;
+    if (B.n <= 0 || B.R < 0 || B.R > 32 || (B.npad & 127) || B.npad < B.n || (B.nbarr != 2 && B.nbarr != 4))
+        return hipErrorInvalidValue;
     hipLaunchKernelGGL(k_qf_build, dim3((unsigned)((B.npad + 255) / 256), (unsigned)B.n), dim3(256), 0, st, B);
     return hipGetLastError();
 }
 
 static bool qf_shape_ok(const QfJob& J) {
+    if ((J.nlv != 1 && J.nlv != 3) || (J.nbarr != 2 && J.nbarr != 4) || (J.nlv == 3 && J.nbarr != 4)) return false;
     return !((J.ppad & 127) || (J.npad & 127) || J.ppad < J.P || J.npad < J.n);
 }
 
@@ -411,10 +440,13 @@ hipError_t launch_qf_zgen(const QfJob& J, hipStream_t st) {
     return hipGetLastError();
 }
 
-hipError_t launch_qf_persist(const QfJob* jobs, const QfTile* tiles, int ntiles, int blocks, hipStream_t st) {
+hipError_t launch_qf_persist(const QfJob* jobs, const QfTile* tiles, int ntiles, int blocks, int nlv, hipStream_t st) {
     if (ntiles <= 0) return hipSuccess;
-    if (blocks <= 0) return hipErrorInvalidValue;
-    hipLaunchKernelGGL(k_qf_forms_persist, dim3((unsigned)blocks), dim3(256), 0, st, jobs, tiles, ntiles);
+    if (blocks <= 0 || (nlv != 1 && nlv != 3)) return hipErrorInvalidValue;
+    if (nlv == 1)
+        hipLaunchKernelGGL(k_qf_forms_persist<1>, dim3((unsigned)blocks), dim3(256), 0, st, jobs, tiles, ntiles);
+    else
+        hipLaunchKernelGGL(k_qf_forms_persist<3>, dim3((unsigned)blocks), dim3(256), 0, st, jobs, tiles, ntiles);
     return hipGetLastError();
 }
 
@@ -424,7 +456,10 @@ hipError_t launch_qf_forms(const QfJob& J, hipStream_t st) {
     const int npt = (int)(J.ppad / 128);
     const int nfg = (J.nform + kQfGF - 1) / kQfGF, npg = (npt + kQfGP - 1) / kQfGP;
     const unsigned nblk = (unsigned)((int64_t)nfg * npg * kQfGF * kQfGP);
-    hipLaunchKernelGGL(k_qf_forms_il, dim3(nblk), dim3(256), 0, st, J, npt);
+    if (J.nlv == 1)
+        hipLaunchKernelGGL(k_qf_forms_il<1>, dim3(nblk), dim3(256), 0, st, J, npt);
+    else
+        hipLaunchKernelGGL(k_qf_forms_il<3>, dim3(nblk), dim3(256), 0, st, J, npt);
     return hipGetLastError();
 }
 

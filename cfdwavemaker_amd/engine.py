@@ -22,7 +22,9 @@ KIND = {k: i for i, k in enumerate(KINDS)}
 KIND.update({"u_smit": 23, "v_smit": 24, "w_smit": 25, "dp_smit": 26})
 GRID_FIELDS = ["ETA", "ETAdx", "ETAdy", "ETAdt", "UX", "UY", "UZ", "Udt", "Vdt", "Wdt",
                "Udx", "Udy", "Uds", "Vdx", "Vdy", "Vds", "Wdx", "Wdy", "Wds", "IGNORE"]
-FP64, FP32 = 0, 1
+# precision of a sea: FP64; FP32 (FP64 phase reduction, three-product bf16
+# quadratic forms); BF16 (FP32 whose quadratic forms take one bf16 product)
+FP64, FP32, BF16 = 0, 1, 2
 
 
 def _f64(a):
@@ -102,14 +104,22 @@ class Sea:
         return out
 
     def grid(self, domain, nx, ny, nl, t0, dt, tan_a, tan_b, water_depth, swl=0.0,
-             domain_ignore=(1e12, -1e12, 1e12, -1e12), ignore_at_init=0) -> "Grid":
+             domain_ignore=(1e12, -1e12, 1e12, -1e12), ignore_at_init=0, defer_fill=False) -> "Grid":
         return Grid(self, domain, nx, ny, nl, t0, dt, tan_a, tan_b, water_depth, swl, domain_ignore,
-                    ignore_at_init)
+                    ignore_at_init, defer_fill)
+
+    def clone(self, precision: int) -> "Sea":
+        """The same sea at another precision (host coefficient tables copied)."""
+        h = C.c_void_p()
+        check(self.L.cfdwm_sea_clone(C.byref(h), self.h, int(precision)), "cfdwm_sea_clone")
+        other = Sea.__new__(Sea)
+        other.L, other.precision, other.h, other.n, other.depth = self.L, precision, h, self.n, self.depth
+        return other
 
 
 class Grid:
     def __init__(self, sea: Sea, domain, nx, ny, nl, t0, dt, tan_a, tan_b, water_depth, swl=0.0,
-                 domain_ignore=(1e12, -1e12, 1e12, -1e12), ignore_at_init=0):
+                 domain_ignore=(1e12, -1e12, 1e12, -1e12), ignore_at_init=0, defer_fill=False):
         self.sea, self.L = sea, sea.L
         d = GridDescC()
         for q in range(4):
@@ -118,6 +128,7 @@ class Grid:
         d.nx, d.ny, d.nl = int(nx), int(ny), int(nl)
         d.t0, d.dt, d.tan_a, d.tan_b = float(t0), float(dt), float(tan_a), float(tan_b)
         d.water_depth, d.swl, d.ignore_at_init = float(water_depth), float(swl), int(ignore_at_init)
+        d.defer_fill = int(bool(defer_fill))
         h = C.c_void_p()
         check(self.L.cfdwm_grid_create(C.byref(h), sea.h, C.byref(d)), "cfdwm_grid_create")
         self.h = h
@@ -179,7 +190,19 @@ class Grid:
         used = self.L.cfdwm_grid_qform_info(self.h, C.byref(nf), C.byref(rk), C.byref(fl), C.byref(er))
         if used != 1:
             return None
-        return dict(nform=nf.value, rank=rk.value, mfma_flop=fl.value, basis_err=er.value)
+        pr, tol, why = C.c_int(), C.c_double(), C.c_int()
+        self.L.cfdwm_grid_qform_mode(self.h, C.byref(pr), C.byref(tol), C.byref(why))
+        return dict(nform=nf.value, rank=rk.value, mfma_flop=fl.value, basis_err=er.value,
+                    products=pr.value, basis_tol=tol.value)
+
+    QFORM_REASONS = ("used", "fp64", "switched off", "no second-order pairs", "frequency ramps differ",
+                     "grid deeper than the depth basis", "no depth basis within the tolerance")
+
+    def qform_reason(self):
+        """Why the last fill did (\"used\") or did not use the MFMA forms."""
+        pr, tol, why = C.c_int(), C.c_double(), C.c_int()
+        self.L.cfdwm_grid_qform_mode(self.h, C.byref(pr), C.byref(tol), C.byref(why))
+        return self.QFORM_REASONS[why.value]
 
     def qform_clock(self, enable):
         """Mean shader clock (MHz) of the last fill's form launches, or None

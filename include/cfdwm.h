@@ -129,7 +129,9 @@ typedef struct {
     int order;        /* 2: second order (Irregular::order default), 0: linear */
     int normalize;    /* Irregular::normalize_data: A = ampl A / sum(A), phases 0 */
     int bw_auto;      /* Irregular::bandwidth_estimator */
-    int precision;    /* 0 FP64, 1 FP32 (FP64 phase reduction) */
+    int precision;    /* 0 FP64, 1 FP32 (FP64 phase reduction), 2 FP32 whose MFMA
+                         quadratic forms take one bf16 product per term instead
+                         of three (elevation error ~3e-4 Hs at N=1024) */
     int with_direct;  /* also build the unfiltered table of u2/v2/w2/dp2 */
 } cfdwm_sea_params;
 
@@ -140,6 +142,10 @@ CFDWM_EXPORT int cfdwm_sea_create(cfdwm_sea** out, int n, const double* omega, c
                                   const double* ramptime, const double* starttime,
                                   const double* duration, const cfdwm_sea_params* params);
 CFDWM_EXPORT int cfdwm_sea_destroy(cfdwm_sea* sea);
+/* A second device copy of a sea at another precision (0, 1, 2 as in
+ * cfdwm_sea_params), sharing nothing with src afterwards; the host
+ * coefficient tables are copied, not recomputed. */
+CFDWM_EXPORT int cfdwm_sea_clone(cfdwm_sea** out, const cfdwm_sea* src, int precision);
 /* vertical_theory of the sea (Irregular.h:83, waveinput.dat key
  * "vertical_theory smit", src/CFDwavemaker.cpp:571-585): 1 routes the u, v, w
  * and p of every point entry (wave_VeloX..wave_DynPres, wave_EvalPoints,
@@ -169,6 +175,9 @@ typedef struct {
     double t0, dt, tan_a, tan_b, water_depth, swl;
     double domain_ignore[4];     /* columns inside are skipped by updates */
     int ignore_at_init;          /* initialize_kinematics_with_ignore */
+    int defer_fill;              /* 1: allocate only; the first cfdwm_grid_refill or
+                                    cfdwm_grids_refill_batched fills (queries, exports
+                                    and updates before that fail with -1) */
 } cfdwm_grid_desc;
 
 /* lsGridSpline: allocate and fill (initialize_kinematics). */
@@ -207,6 +216,15 @@ CFDWM_EXPORT int cfdwm_grid_query_device_f32(cfdwm_grid* grid, int64_t n, double
  * the pair loop. */
 CFDWM_EXPORT int cfdwm_grid_qform_info(cfdwm_grid* grid, int* nform, int* rank, double* mfma_flop,
                                        double* basis_err);
+/* Product mode of the last fill's quadratic forms (no reference
+ * counterpart): bf16 products per term (3: hi/lo split, 1: hi only), the
+ * depth-basis tolerance, and why the forms were or were not used: 0 used,
+ * 1 FP64 sea, 2 switched off (CFDWM_QFORM=0), 3 no second-order pairs,
+ * 4 frequency ramps differ between components at a slot time, 5 grid depth
+ * beyond the depth basis, 6 no depth basis within the tolerance (the last
+ * three print a one-time warning: the pair loop is much slower).  Returns 1
+ * when that fill used the forms, else 0 (products and tolerance 0). */
+CFDWM_EXPORT int cfdwm_grid_qform_mode(cfdwm_grid* grid, int* products, double* basis_tol, int* reason);
 /* Shader clock of the MFMA form launches (no reference counterpart: a
  * measurement aid).  Writes the mean shader clock in MHz over the blocks of
  * the last fill's form launches, if that fill was measured (returns 0; else -1
@@ -225,6 +243,8 @@ CFDWM_EXPORT int cfdwm_grid_qform_clock(cfdwm_grid* grid, int enable, double* mh
 typedef struct {
     int groups, batched, single, tiles, blocks, order, tuned;
     double forms_ms, mfma_flop;
+    int group_forms, group_ptiles;   /* tile grouping of the tuned launch (order 1) */
+    double tables_ms;                /* form tables built during the call */
 } cfdwm_batch_stats;
 /* Write the s-grid as an ASCII VTK XML UnstructuredGrid (.vtu) for QA in
  * ParaView (lsGridSpline::export_vtu, src/lsgrid_spline.cpp:1222-1403): the

@@ -49,3 +49,58 @@ def test_batched_fp32_vs_fp64_per_state():
             err = scaled_err(g32.export(f), g64.export(f))
             print(f"N={len(sea.omega)} {f} fp32 batched vs fp64 {err:.2e}", flush=True)
             assert err <= 1e-5, (f, err)
+
+
+@pytest.mark.parametrize("prec_name", ["fp32", "bf16"])
+def test_batched_full_grids_vs_oracle(oracle, prec_name):
+    """Ten sea states (long- and short-crested, N 162-455, depths 57-194 m) on
+    their full 64 x 64 x 8 s-grids, deferred at creation and filled by one
+    batched call, against the FP64 oracle's fill of the four corner columns
+    (all levels and time levels).  Bounds: FP32 2e-5 of the field scale; BF16
+    the elevation gate 1e-3 Hs and 2e-3 of the field scale for velocities."""
+    from cfdwavemaker_amd import engine as eng
+    from cfdwavemaker_amd import surrogate as S
+    from cfdwavemaker_amd.seastate import config_sea
+    states = [2, 3, 6, 14, 24, 31, 47, 59, 60, 62]
+    prec = eng.FP32 if prec_name == "fp32" else eng.BF16
+    keep = []
+    for st in states:
+        sea = config_sea(4, st)
+        lam = 2 * np.pi / S.peak_wavenumber(sea)
+        s = eng.Sea(sea.as_dict(), precision=prec, with_direct=False)
+        g = s.grid((0.0, 20 * lam, 0.0, 20 * lam), 64, 64, 8, 0.0, 0.5, 0.7, 1.5, sea.depth, defer_fill=True)
+        keep.append((sea, lam, s, g))
+    stats = eng.refill_batched([k[3] for k in keep], budget_gb=16.0)
+    assert stats["batched"] == len(states), stats
+    idx = np.array([0, 63])
+    for sea, lam, _, g in keep:
+        og = oracle.sea(sea.as_dict()).grid((0.0, 20 * lam, 0.0, 20 * lam), 2, 2, 8, 0.0, 0.5, 0.7, 1.5, sea.depth)
+        for f in ("ETA", "UX", "UY", "UZ"):
+            ref = og.export(f)
+            got = g.export(f)
+            got = (got.reshape(4, 64, 64)[:, idx][:, :, idx] if f == "ETA"
+                   else got.reshape(4, 64, 64, 8)[:, idx][:, :, idx]).ravel()
+            if f == "ETA":
+                assert np.max(np.abs(got - ref)) <= 1e-3 * sea.hs
+            if f == "UY" and sea.ndir == 1:
+                assert np.max(np.abs(got)) == 0.0
+                continue
+            err = scaled_err(got, ref)
+            print(f"{prec_name} N={len(sea.omega)} ndir {sea.ndir} {f} err {err:.2e}", flush=True)
+            assert err <= (2e-5 if prec_name == "fp32" else 2e-3), (len(sea.omega), f, err)
+
+
+def test_deferred_grid_refuses_queries():
+    """A grid created with defer_fill holds no packed slots: queries and
+    updates fail with an error instead of reading unfilled memory."""
+    from cfdwavemaker_amd import engine as eng
+    from cfdwavemaker_amd.seastate import config_sea
+    sea = config_sea(4, 62)
+    s = eng.Sea(sea.as_dict(), precision=eng.FP32, with_direct=False)
+    g = s.grid((0.0, 100.0, 0.0, 100.0), 8, 8, 4, 0.0, 0.5, 0.7, 1.5, sea.depth, defer_fill=True)
+    with pytest.raises(RuntimeError):
+        g.query([0.1], [10.0], [10.0], [-5.0])
+    with pytest.raises(RuntimeError):
+        g.force_update(2.0)
+    g.refill()
+    assert np.all(np.isfinite(g.query([0.1], [10.0], [10.0], [-5.0])))

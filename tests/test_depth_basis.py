@@ -47,3 +47,30 @@ def test_basis_interpolates_at_skeleton_points(eng):
 def test_rank_grows_with_tolerance(eng):
     r = [eng.depth_basis(1.4, 200.0, tol)["rank"] for tol in (1e-4, 1e-6, 1e-8)]
     assert r[0] < r[1] < r[2] <= 32
+
+
+def test_unreachable_tolerance_reports_failure(eng):
+    """A tolerance that no rank <= 32 reaches is reported as a failure (negative
+    rank from cfdwm_depth_basis), with the measured error above the tolerance:
+    the s-grid fills then take the pair loop instead of the forms."""
+    b = eng.depth_basis(1.4, 200.0, 1e-9)
+    assert not b["ok"]
+    assert b["err"] > 1e-9
+
+
+@pytest.mark.gpu
+def test_tight_basis_tolerance_falls_back_to_pair_loop(eng, oracle, monkeypatch):
+    """An FP32 fill whose depth basis cannot meet CFDWM_QF_TOL runs the pair
+    loop (no forms) and stays within the FP32 bound against the oracle."""
+    from cfdwavemaker_amd.seastate import config_sea
+    from tests.conftest import scaled_err
+    monkeypatch.setenv("CFDWM_QF_TOL", "1e-9")
+    s = config_sea(3).as_dict()
+    dom = (0.0, 1000.0 * 2 / 255, 0.0, 1000.0 * 2 / 255)
+    ref = oracle.sea(s).grid(dom, 3, 3, 16, 0.0, 0.5, 0.7, 1.5, 100.0)
+    grid = eng.Sea(s, precision=eng.FP32, with_direct=False).grid(dom, 3, 3, 16, 0.0, 0.5, 0.7, 1.5, 100.0)
+    assert grid.qform_info() is None
+    assert grid.qform_reason() == "no depth basis within the tolerance"
+    for f in ("ETA", "UX", "UY", "UZ"):
+        err = scaled_err(grid.export(f), ref.export(f))
+        assert err <= 1e-5, (f, err)
