@@ -524,18 +524,18 @@ def main():
     qx = (torch.rand(nq, generator=g, dtype=torch.float32) * 1000.0).to(dev)
     qy = (torch.rand(nq, generator=g, dtype=torch.float32) * 1000.0).to(dev)
     qz = torch.zeros(nq, dtype=torch.float32, device=dev)
-    out = [torch.empty(nq, dtype=torch.float32, device=dev) for _ in range(4)]
+    # results {eta, u, v, w} per query (the layout of wave_Kinematics)
+    out4 = torch.empty(4 * nq, dtype=torch.float32, device=dev)
     torch.cuda.synchronize()
     tq = 0.25
 
     def query():
-        grid.query_device_f32(nq, tq, qx.data_ptr(), qy.data_ptr(), qz.data_ptr(),
-                              *[o.data_ptr() for o in out])
+        grid.kinematics_device_f32(nq, tq, qx.data_ptr(), qy.data_ptr(), qz.data_ptr(), out4.data_ptr())
 
     query()
     engine.sync()
     u = torch.rand(nq, generator=g, dtype=torch.float32).to(dev)
-    qz.copy_(-DEPTH + u * (out[0] + DEPTH))
+    qz.copy_(-DEPTH + u * (out4[0::4] + DEPTH))
     torch.cuda.synchronize()
 
     def step():

@@ -37,7 +37,7 @@ BATCH_API = [
     "cfdwm_mlp_loss_grad_device", "cfdwm_mlp_train_device", "cfdwm_surrogate_linear", "cfdwm_surrogate_add",
     "cfdwm_grids_refill_batched", "cfdwm_grid_release_workspace", "wave_PrefetchPoints", "wave_PrefetchClear",
     "wave_PrefetchStats", "cfdwm_eval_fields", "cfdwm_grid_write_vtu", "cfdwm_sea_set_vertical_theory",
-    "cfdwm_grid_qform_clock", "cfdwm_grid_qform_mode", "cfdwm_sea_clone",
+    "cfdwm_grid_qform_clock", "cfdwm_grid_qform_mode", "cfdwm_sea_clone", "cfdwm_grid_kinematics_device_f32",
 ]
 
 
@@ -137,6 +137,8 @@ def load(path: str | None = None) -> C.CDLL:
     L.cfdwm_grid_query.argtypes = [C.c_void_p, _i64] + [_dp] * 5
     L.cfdwm_grid_query_device_f32.restype = C.c_int
     L.cfdwm_grid_query_device_f32.argtypes = [C.c_void_p, _i64, _d] + [C.c_void_p] * 7
+    L.cfdwm_grid_kinematics_device_f32.restype = C.c_int
+    L.cfdwm_grid_kinematics_device_f32.argtypes = [C.c_void_p, _i64, _d] + [C.c_void_p] * 4
     L.cfdwm_sync.restype = C.c_int
     L.cfdwm_last_error.restype = C.c_int
     L.cfdwm_last_error.argtypes = [C.c_char_p, C.c_int]

@@ -1006,6 +1006,12 @@ int cfdwm_grid_query_device_f32(cfdwm_grid* g, int64_t n, double t, const float*
     if (!g || n < 0 || !x || !y || !z || !eta || !u || !v || !w) { set_error("bad arguments"); return -1; }
     return g->g->query_device_f32(n, t, x, y, z, eta, u, v, w);
 }
+int cfdwm_grid_kinematics_device_f32(cfdwm_grid* g, int64_t n, double t, const float* x, const float* y,
+                                     const float* z, float* out4) {
+    if (!g || n < 0 || !x || !y || !z || !out4) { set_error("bad arguments"); return -1; }
+    if ((reinterpret_cast<uintptr_t>(out4) & 15) != 0) { set_error("out4 must be 16-byte aligned"); return -1; }
+    return g->g->query_device_f32(n, t, x, y, z, nullptr, nullptr, nullptr, nullptr, out4);
+}
 int cfdwm_grid_qform_info(cfdwm_grid* grid, int* nform, int* rank, double* mfma_flop, double* basis_err) {
     if (!grid) return -1;
     const Grid& g = *grid->g;

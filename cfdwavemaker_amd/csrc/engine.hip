@@ -1312,9 +1312,27 @@ int Grid::write_vtu(const char* path, bool last, const char* time_label) {
 // Query batches from this size on are evaluated in cell order.
 constexpr int64_t kQuerySortMin = 65536;
 
+// Device bytes of the cell-ordering buffers of an n-query batch (keys and
+// records, input and sorted halves, and the radix sort's scratch, each part
+// 256-byte aligned); 0 below the ordering threshold.
+template <typename TQ>
+int query_ws_bytes(int64_t n, int ncol, size_t* bytes) {
+    *bytes = 0;
+    if (n < kQuerySortMin || n > INT32_MAX) return 0;
+    int bits = 1;
+    while (((int64_t)1 << bits) < ncol) ++bits;
+    size_t tmp = 0;
+    CFDWM_HIP(query_sort_tmp_bytes<TQ>(n, bits, &tmp));
+    auto up = [](size_t b) { return (b + 255) / 256 * 256; };
+    *bytes = up(2 * n * sizeof(uint32_t)) + up(2 * n * sizeof(QRec<TQ>)) + up(std::max<size_t>(tmp, 256));
+    return 0;
+}
+template int query_ws_bytes<float>(int64_t, int, size_t*);
+template int query_ws_bytes<double>(int64_t, int, size_t*);
+
 template <typename R, typename TQ>
 int Grid::run_query(int64_t n, const TQ* t, double tconst, const TQ* x, const TQ* y, const TQ* z,
-                    TQ* eta, TQ* u, TQ* v, TQ* w, const double* ramp) {
+                    TQ* eta, TQ* u, TQ* v, TQ* w, const double* ramp, TQ* aos, void* ws, size_t ws_bytes) {
     if (!filled) { set_error("the s-grid has not been filled yet (created with defer_fill)"); return -1; }
     QueryJob<R, TQ> J;
     J.n = n; J.g = g; J.tstep = tstep; J.t0 = t0; J.dt = d.dt;
@@ -1325,6 +1343,7 @@ int Grid::run_query(int64_t n, const TQ* t, double tconst, const TQ* x, const TQ
     J.t = t; J.tconst = tconst; J.x = x; J.y = y; J.z = z;
     J.ramp_scale = ramp;
     J.o_eta = eta; J.o_u = u; J.o_v = v; J.o_w = w;
+    J.o_aos = aos;
     hipStream_t st = Engine::get().stream();
     const int64_t ncol = (int64_t)g.nx * g.ny;
     if (n >= kQuerySortMin && n <= INT32_MAX) {
@@ -1333,12 +1352,31 @@ int Grid::run_query(int64_t n, const TQ* t, double tconst, const TQ* x, const TQ
         while (((int64_t)1 << bits) < ncol) ++bits;
         size_t tmp = 0;
         CFDWM_HIP(query_sort_tmp_bytes<TQ>(n, bits, &tmp));
-        if (qs_key_.alloc(2 * n * sizeof(uint32_t)) || qs_rec_.alloc(2 * n * sizeof(QRec<TQ>)) ||
-            qs_tmp_.alloc(std::max<size_t>(tmp, 256)))
-            return -3;
-        QRec<TQ>* rec = (QRec<TQ>*)qs_rec_.get();
-        PROF("k_query_sort", (launch_query_sort<TQ>(n, g, x, y, z, bits, (uint32_t*)qs_key_.get(), rec,
-                                                    qs_tmp_.get(), qs_tmp_.size(), st)));
+        uint32_t* keys = nullptr;
+        QRec<TQ>* rec = nullptr;
+        void* tbuf = nullptr;
+        size_t tlen = 0;
+        if (ws) {
+            // caller's workspace (the allocation-free entry): keys, records, sort scratch
+            size_t need = 0;
+            if (int e = query_ws_bytes<TQ>(n, (int)ncol, &need)) return e;
+            if (need > ws_bytes) { set_error("query workspace too small"); return -1; }
+            auto up = [](size_t b) { return (b + 255) / 256 * 256; };
+            char* b = (char*)ws;
+            keys = (uint32_t*)b;
+            rec = (QRec<TQ>*)(b + up(2 * n * sizeof(uint32_t)));
+            tbuf = b + up(2 * n * sizeof(uint32_t)) + up(2 * n * sizeof(QRec<TQ>));
+            tlen = std::max<size_t>(tmp, 256);
+        } else {
+            if (qs_key_.alloc(2 * n * sizeof(uint32_t)) || qs_rec_.alloc(2 * n * sizeof(QRec<TQ>)) ||
+                qs_tmp_.alloc(std::max<size_t>(tmp, 256)))
+                return -3;
+            keys = (uint32_t*)qs_key_.get();
+            rec = (QRec<TQ>*)qs_rec_.get();
+            tbuf = qs_tmp_.get();
+            tlen = qs_tmp_.size();
+        }
+        PROF("k_query_sort", (launch_query_sort<TQ>(n, g, x, y, z, bits, keys, rec, tbuf, tlen, st)));
         J.srec = rec + n;
     }
     PROF("k_query", (launch_query<R, TQ>(J, st)));
@@ -1358,22 +1396,21 @@ int Grid::query(int64_t n, const double* t, const double* x, const double* y, co
     CFDWM_HIP(hipMemcpyAsync(b + n, x, n * 8, hipMemcpyHostToDevice, st));
     CFDWM_HIP(hipMemcpyAsync(b + 2 * n, y, n * 8, hipMemcpyHostToDevice, st));
     CFDWM_HIP(hipMemcpyAsync(b + 3 * n, z, n * 8, hipMemcpyHostToDevice, st));
-    int rc = prec == kFP32 ? run_query<float, double>(n, b, 0.0, b + n, b + 2 * n, b + 3 * n, o, o + n, o + 2 * n, o + 3 * n, nullptr)
-                           : run_query<double, double>(n, b, 0.0, b + n, b + 2 * n, b + 3 * n, o, o + n, o + 2 * n, o + 3 * n, nullptr);
+    // {eta, u, v, w} per point, the layout of out4
+    int rc = prec == kFP32
+        ? run_query<float, double>(n, b, 0.0, b + n, b + 2 * n, b + 3 * n, nullptr, nullptr, nullptr, nullptr, nullptr, o)
+        : run_query<double, double>(n, b, 0.0, b + n, b + 2 * n, b + 3 * n, nullptr, nullptr, nullptr, nullptr, nullptr, o);
     if (rc) return rc;
-    std::vector<double> tmp(4 * n);
-    CFDWM_HIP(hipMemcpyAsync(tmp.data(), o, 4 * n * 8, hipMemcpyDeviceToHost, st));
+    CFDWM_HIP(hipMemcpyAsync(out4, o, 4 * n * 8, hipMemcpyDeviceToHost, st));
     CFDWM_HIP(hipStreamSynchronize(st));
-    for (int64_t p = 0; p < n; ++p)
-        for (int c = 0; c < 4; ++c) out4[4 * p + c] = tmp[c * n + p];
     return 0;
 }
 
 int Grid::query_device_f32(int64_t n, double t, const float* x, const float* y, const float* z,
-                           float* eta, float* u, float* v, float* w) {
+                           float* eta, float* u, float* v, float* w, float* out4, void* ws, size_t ws_bytes) {
     std::lock_guard<std::mutex> lk(sea->mu);          // the grid's query-ordering buffers
-    if (prec == kFP32) return run_query<float, float>(n, nullptr, t, x, y, z, eta, u, v, w, nullptr);
-    return run_query<double, float>(n, nullptr, t, x, y, z, eta, u, v, w, nullptr);
+    if (prec == kFP32) return run_query<float, float>(n, nullptr, t, x, y, z, eta, u, v, w, nullptr, out4, ws, ws_bytes);
+    return run_query<double, float>(n, nullptr, t, x, y, z, eta, u, v, w, nullptr, out4, ws, ws_bytes);
 }
 
 int Grid::query_device_f64(int64_t n, const double* t, const double* x, const double* y, const double* z,

@@ -126,6 +126,7 @@ struct QueryJob {
     double tconst = 0;
     const double* ramp_scale = nullptr;
     TQ *o_eta = nullptr, *o_u = nullptr, *o_v = nullptr, *o_w = nullptr;
+    TQ* o_aos = nullptr;                  // when set: {eta, u, v, w} of query p at o_aos[4 p .. 4 p + 3]
     // when set: thread p evaluates the query srec[p] (cell order) instead of
     // (x, y, z)[p] and writes its outputs at the query's own index
     const QRec<TQ>* srec = nullptr;
@@ -342,8 +343,12 @@ public:
     int write_vtu(const char* path, bool last, const char* time_label);
     int query(int64_t n, const double* t, const double* x, const double* y, const double* z,
               double* out4, const int* ramp_flags, const double* ramp_se);
+    // SoA outputs (eta, u, v, w) or, with out4, {eta, u, v, w} per query;
+    // ws (ws_bytes): caller workspace for the cell-ordering buffers instead of
+    // the grid's own (query_ws_bytes gives the size)
     int query_device_f32(int64_t n, double t, const float* x, const float* y, const float* z,
-                         float* eta, float* u, float* v, float* w);
+                         float* eta, float* u, float* v, float* w, float* out4 = nullptr,
+                         void* ws = nullptr, size_t ws_bytes = 0);
     int query_device_f64(int64_t n, const double* t, const double* x, const double* y, const double* z,
                          double* eta, double* u, double* v, double* w);
     Sea* sea;
@@ -416,7 +421,8 @@ private:
     int gradients();
     template <typename R, typename TQ>
     int run_query(int64_t n, const TQ* t, double tconst, const TQ* x, const TQ* y, const TQ* z,
-                  TQ* eta, TQ* u, TQ* v, TQ* w, const double* ramp);
+                  TQ* eta, TQ* u, TQ* v, TQ* w, const double* ramp, TQ* aos = nullptr,
+                  void* ws = nullptr, size_t ws_bytes = 0);
     int64_t field_count(int which) const;
 };
 
@@ -431,5 +437,9 @@ struct BatchStats {
     double tables_ms = 0;                    // form tables built during the call (host + device)
 };
 int refill_batched(Grid** grids, int n, int64_t budget_bytes, bool release, BatchStats* stats);
+// Device bytes of the cell-ordering buffers of an n-query batch on a grid of
+// ncol columns (0 below the ordering threshold); 0 or a negative code.
+template <typename TQ>
+int query_ws_bytes(int64_t n, int ncol, size_t* bytes);
 
 }  // namespace cfdwm

@@ -834,16 +834,31 @@ __global__ __launch_bounds__(256) void k_query(QueryJob<R, TQ> J) {
     l1 = l1 < 0 ? 0 : (l1 > nl - 1 ? nl - 1 : l1);
     const R ds = (R)g.ds;
     const double rmp = J.ramp_scale ? (double)J.ramp_scale[p] : 1.0;
-    J.o_eta[p] = (TQ)((double)eta * rmp);
+    const TQ oeta = (TQ)((double)eta * rmp);
     R VC[3], VCt[3], VD[3], VDt[3];
     slot_velo(VC, VCt, J.pk_C, ac, nl, l0, xd, yd, sd0, dx, dy, ds);
     slot_velo(VD, VDt, J.pk_D, ac, nl, l1, xd, yd, sd1, dx, dy, ds);
-    TQ* outs[3] = {J.o_u, J.o_v, J.o_w};
+    TQ ov[3];
 #pragma unroll
     for (int f = 0; f < 3; ++f) {
         const R a = dt * VCt[f] - (VD[f] - VC[f]), b = -dt * VDt[f] + (VD[f] - VC[f]);
-        outs[f][p] = (TQ)((double)herm(VC[f], VD[f], a, b, tdR) * rmp);
+        ov[f] = (TQ)((double)herm(VC[f], VD[f], a, b, tdR) * rmp);
     }
+    if (J.o_aos) {
+        // {eta, u, v, w} of the query in one vector store (16 B FP32, 2 x 16 B FP64)
+        if constexpr (sizeof(TQ) == 4) {
+            *reinterpret_cast<float4*>(J.o_aos + 4 * p) = make_float4(oeta, ov[0], ov[1], ov[2]);
+        } else {
+            double2* o = reinterpret_cast<double2*>(J.o_aos + 4 * p);
+            o[0] = make_double2(oeta, ov[0]);
+            o[1] = make_double2(ov[1], ov[2]);
+        }
+        return;
+    }
+    J.o_eta[p] = oeta;
+    J.o_u[p] = ov[0];
+    J.o_v[p] = ov[1];
+    J.o_w[p] = ov[2];
 }
 
 // ===================================================== device point batches

@@ -211,6 +211,12 @@ class Grid:
         r = self.L.cfdwm_grid_qform_clock(self.h, 1 if enable else 0, C.byref(mhz))
         return mhz.value if r == 0 else None
 
+    def kinematics_device_f32(self, n, t, x_ptr, y_ptr, z_ptr, out4_ptr):
+        """Device pointers; out4 receives {eta, u, v, w} per query (the layout
+        of wave_Kinematics), enqueued on the library stream."""
+        check(self.L.cfdwm_grid_kinematics_device_f32(self.h, int(n), float(t), x_ptr, y_ptr, z_ptr, out4_ptr),
+              "cfdwm_grid_kinematics_device_f32")
+
     def query_device_f32(self, n, t, x_ptr, y_ptr, z_ptr, eta_ptr, u_ptr, v_ptr, w_ptr):
         """Device pointers (e.g. torch tensors' data_ptr()), enqueued on the library stream."""
         check(self.L.cfdwm_grid_query_device_f32(self.h, int(n), float(t), x_ptr, y_ptr, z_ptr,

@@ -209,6 +209,12 @@ CFDWM_EXPORT int cfdwm_grid_query(cfdwm_grid* grid, int64_t n, const double* t, 
 CFDWM_EXPORT int cfdwm_grid_query_device_f32(cfdwm_grid* grid, int64_t n, double t, const float* x,
                                              const float* y, const float* z, float* eta, float* u,
                                              float* v, float* w);
+/* As cfdwm_grid_query_device_f32, with the results in the layout of
+ * wave_Kinematics (src/CFDwavemaker.cpp:1623-1667): out4[4 i .. 4 i + 3] =
+ * {eta, u, v, w} of query i (out4 16-byte aligned).  One 16-byte store per
+ * query instead of four scattered 4-byte stores. */
+CFDWM_EXPORT int cfdwm_grid_kinematics_device_f32(cfdwm_grid* grid, int64_t n, double t, const float* x,
+                                                  const float* y, const float* z, float* out4);
 /* Second-order quadratic forms of the last FP32 fill (no reference
  * counterpart: measurement hook).  Returns 1 when the fill ran the pair sums
  * as MFMA quadratic forms (then nform, the depth-basis rank, the executed MFMA
