@@ -137,6 +137,8 @@ def load(path: str | None = None) -> C.CDLL:
     L.cfdwm_grid_query.argtypes = [C.c_void_p, _i64] + [_dp] * 5
     L.cfdwm_grid_query_device_f32.restype = C.c_int
     L.cfdwm_grid_query_device_f32.argtypes = [C.c_void_p, _i64, _d] + [C.c_void_p] * 7
+    L.cfdwm_grid_query_device_f32_noalloc.restype = C.c_int
+    L.cfdwm_grid_query_device_f32_noalloc.argtypes = [C.c_void_p, _i64, _d] + [C.c_void_p] * 7
     L.cfdwm_grid_kinematics_device_f32.restype = C.c_int
     L.cfdwm_grid_kinematics_device_f32.argtypes = [C.c_void_p, _i64, _d] + [C.c_void_p] * 4
     L.cfdwm_sync.restype = C.c_int

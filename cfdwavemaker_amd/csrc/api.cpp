@@ -1006,6 +1006,11 @@ int cfdwm_grid_query_device_f32(cfdwm_grid* g, int64_t n, double t, const float*
     if (!g || n < 0 || !x || !y || !z || !eta || !u || !v || !w) { set_error("bad arguments"); return -1; }
     return g->g->query_device_f32(n, t, x, y, z, eta, u, v, w);
 }
+int cfdwm_grid_query_device_f32_noalloc(cfdwm_grid* g, int64_t n, double t, const float* x, const float* y,
+                                        const float* z, float* eta, float* u, float* v, float* w) {
+    if (!g || n < 0 || !x || !y || !z || !eta || !u || !v || !w) { set_error("bad arguments"); return -1; }
+    return query_device_f32_noalloc(*g->g, n, t, x, y, z, eta, u, v, w);
+}
 int cfdwm_grid_kinematics_device_f32(cfdwm_grid* g, int64_t n, double t, const float* x, const float* y,
                                      const float* z, float* out4) {
     if (!g || n < 0 || !x || !y || !z || !out4) { set_error("bad arguments"); return -1; }

@@ -441,5 +441,9 @@ int refill_batched(Grid** grids, int n, int64_t budget_bytes, bool release, Batc
 // ncol columns (0 below the ordering threshold); 0 or a negative code.
 template <typename TQ>
 int query_ws_bytes(int64_t n, int ncol, size_t* bytes);
+// Grid::query_device_f32 with the ordering buffers in a fixed device array
+// (noalloc.hip): allocates no device memory; too large a batch gives -1.
+int query_device_f32_noalloc(Grid& grid, int64_t n, double t, const float* x, const float* y, const float* z,
+                             float* eta, float* u, float* v, float* w);
 
 }  // namespace cfdwm

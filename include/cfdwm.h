@@ -209,6 +209,19 @@ CFDWM_EXPORT int cfdwm_grid_query(cfdwm_grid* grid, int64_t n, const double* t, 
 CFDWM_EXPORT int cfdwm_grid_query_device_f32(cfdwm_grid* grid, int64_t n, double t, const float* x,
                                              const float* y, const float* z, float* eta, float* u,
                                              float* v, float* w);
+/* Allocation-free variant of cfdwm_grid_query_device_f32 (no counterpart in
+ * the reference): for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * Same arguments, kernels and results; the cell-ordering buffers of batches of
+ * 65,536 queries or more live in a fixed 64 MiB device array per GPU (held
+ * until the process ends) instead of the grid's own buffers, so the call makes
+ * no device allocation.  A batch whose ordering buffers exceed the array
+ * (about a million queries) returns -1 with the maximum in the message.  All
+ * calls share the GPU's array and are serialised on the library stream; work
+ * on other streams that overlaps one of them must not use it. */
+CFDWM_EXPORT int cfdwm_grid_query_device_f32_noalloc(cfdwm_grid* grid, int64_t n, double t, const float* x,
+                                                     const float* y, const float* z, float* eta, float* u,
+                                                     float* v, float* w);
 /* As cfdwm_grid_query_device_f32, with the results in the layout of
  * wave_Kinematics (src/CFDwavemaker.cpp:1623-1667): out4[4 i .. 4 i + 3] =
  * {eta, u, v, w} of query i (out4 16-byte aligned).  One 16-byte store per
