@@ -34,7 +34,7 @@ BATCH_API = [
     "cfdwm_prof_enable", "cfdwm_prof_read", "cfdwm_grid_qform_info", "cfdwm_depth_basis",
     "cfdwm_mlp_create", "cfdwm_mlp_destroy", "cfdwm_mlp_nparams", "cfdwm_mlp_get_params",
     "cfdwm_mlp_set_params", "cfdwm_mlp_set_norm", "cfdwm_mlp_get_norm", "cfdwm_mlp_forward_device",
-    "cfdwm_mlp_loss_grad_device", "cfdwm_mlp_train_device", "cfdwm_surrogate_linear", "cfdwm_surrogate_add",
+    "cfdwm_mlp_loss_grad_device", "cfdwm_mlp_train_device", "cfdwm_surrogate_linear", "cfdwm_surrogate_features", "cfdwm_surrogate_add",
     "cfdwm_grids_refill_batched", "cfdwm_grid_release_workspace", "wave_PrefetchPoints", "wave_PrefetchClear",
     "wave_PrefetchStats", "cfdwm_eval_fields", "cfdwm_grid_write_vtu", "cfdwm_sea_set_vertical_theory",
     "cfdwm_grid_qform_clock", "cfdwm_grid_qform_mode", "cfdwm_sea_clone", "cfdwm_grid_kinematics_device_f32",
@@ -180,6 +180,11 @@ def load(path: str | None = None) -> C.CDLL:
     L.cfdwm_surrogate_linear.restype = C.c_int
     L.cfdwm_surrogate_linear.argtypes = [C.c_void_p, _i64, C.c_void_p, C.c_void_p, _d, _d, C.c_int, _dp, C.c_int,
                                          _d, C.c_void_p, C.c_void_p]
+    L.cfdwm_surrogate_features.restype = C.c_int
+    L.cfdwm_surrogate_features.argtypes = [C.c_void_p, _i64, C.c_void_p, C.c_void_p, _d, _d, C.c_int, _dp,
+                                           C.c_int, _d, C.c_int, C.c_void_p, C.c_void_p]
+    L.cfdwm_surrogate_set_column.restype = C.c_int
+    L.cfdwm_surrogate_set_column.argtypes = [_i64, C.c_void_p, C.c_void_p, C.c_int, C.c_int]
     L.cfdwm_surrogate_add.restype = C.c_int
     L.cfdwm_surrogate_add.argtypes = [_i64, C.c_void_p, C.c_void_p, C.c_void_p]
     L.cfdwm_grid_write_vtu.restype = C.c_int

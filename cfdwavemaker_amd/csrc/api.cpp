@@ -1096,6 +1096,26 @@ int cfdwm_surrogate_linear(cfdwm_sea* sea, int64_t n, const double* x, const dou
     J.n = n; J.x = x; J.y = y; J.t = t; J.h = h; J.kp = kp;
     J.nlev = nlev; J.fstride = fstride;
     J.nx = 3 + 3 * ((nlev + fstride - 1) / fstride) + 2;
+    J.xstride = J.nx;
+    for (int l = 0; l < nlev; ++l) J.sigma[l] = sigma[l];
+    J.X = X; J.Lin = Lin;
+    return surrogate_linear(*sea->s, J);
+}
+int cfdwm_surrogate_features(cfdwm_sea* sea, int64_t n, const double* x, const double* y, double t, double h,
+                             int nlev, const double* sigma, int fstride, double kp, int xstride, float* X,
+                             float* Lin) {
+    const int nx = 3 + 3 * ((std::max(nlev, 1) + std::max(fstride, 1) - 1) / std::max(fstride, 1)) + 2;
+    if (!sea || n < 0 || nlev < 1 || nlev > 32 || fstride < 1 || !sigma || xstride < nx + 3 ||
+        (n && (!x || !y || !X || !Lin))) {
+        set_error("bad arguments");
+        return -1;
+    }
+    SurrJob J;
+    J.n = n; J.x = x; J.y = y; J.t = t; J.h = h; J.kp = kp;
+    J.nlev = nlev; J.fstride = fstride;
+    J.nx = nx;
+    J.xstride = xstride;
+    J.quad = 1;
     for (int l = 0; l < nlev; ++l) J.sigma[l] = sigma[l];
     J.X = X; J.Lin = Lin;
     return surrogate_linear(*sea->s, J);
@@ -1104,6 +1124,11 @@ int cfdwm_surrogate_add(int64_t n, const float* a, const float* b, float* out) {
     if (n < 0 || (n && (!a || !b || !out))) { set_error("bad arguments"); return -1; }
     if (int e = Engine::get().init()) return e;
     return surrogate_add(n, a, b, out);
+}
+int cfdwm_surrogate_set_column(int64_t n, const double* src, float* X, int stride, int col) {
+    if (n < 0 || stride < 1 || col < 0 || col >= stride || (n && (!src || !X))) { set_error("bad arguments"); return -1; }
+    if (int e = Engine::get().init()) return e;
+    return surrogate_set_column(n, src, X, stride, col);
 }
 int cfdwm_sync(void) {
     if (Engine::get().init()) return -2;

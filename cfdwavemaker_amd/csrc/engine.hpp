@@ -177,8 +177,10 @@ struct SurrJob {
     const double *x = nullptr, *y = nullptr;   // device column coordinates
     double t = 0, h = 0, kp = 0;
     int nlev = 0, fstride = 2, nx = 0;         // nx = 3 + 3 ceil(nlev / fstride) + 2
+    int xstride = 0;                           // row stride of X (>= nx, >= nx + 3 with quad)
+    int quad = 0;                              // also X[nx .. nx + 2] = quadrature of eta1, its x and y slopes
     double sigma[32] = {};
-    float* X = nullptr;                        // [n][nx]
+    float* X = nullptr;                        // [n][xstride]
     float* Lin = nullptr;                      // [n][1 + 4 nlev]
 };
 
@@ -251,6 +253,7 @@ enum Precision { kFP64 = 0, kFP32 = 1 };
 class Sea;
 int surrogate_linear(Sea& s, const SurrJob& J);
 int surrogate_add(int64_t n, const float* a, const float* b, float* o);
+int surrogate_set_column(int64_t n, const double* src, float* X, int stride, int col);
 
 class Sea {
 public:

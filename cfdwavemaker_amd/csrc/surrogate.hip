@@ -56,6 +56,7 @@ __global__ __launch_bounds__(256) void k_surr_linear(SeaDev<R> S, SurrJob J) {
     if (q >= J.n) return;
     const double dx = J.x[q] - S.x0, dy = J.y[q] - S.y0, t = J.t;
     double e1 = 0, sx = 0, sy = 0, gx = 0, gy = 0, gz = 0, gp = 0;
+    double eh = 0, hx = 0, hy = 0;                 // quadrature (Hilbert pair) of eta1 and its slopes
     for (int i = 0; i < S.n; ++i) {
         const PhaseRec ph = S.ph[i];
         const CompRec<R>& c = S.cr[i];
@@ -71,8 +72,14 @@ __global__ __launch_bounds__(256) void k_surr_linear(SeaDev<R> S, SurrJob J) {
         gy += (double)(c.gyz * cr);
         gz += (double)(c.gzz * sr);
         gp += (double)(c.gpz * cr);
+        eh += (double)(c.A * sr);
+        hx += (double)(ak * (R)ph.cth * cr);
+        hy += (double)(ak * (R)ph.sth * cr);
     }
-    float* X = J.X + q * J.nx;
+    float* X = J.X + q * J.xstride;
+    if (J.quad) {
+        X[J.nx] = (float)eh; X[J.nx + 1] = (float)hx; X[J.nx + 2] = (float)hy;
+    }
     float* L = J.Lin + q * (1 + 4 * J.nlev);
     X[0] = (float)e1; X[1] = (float)sx; X[2] = (float)sy;
     L[0] = (float)e1;
@@ -140,6 +147,13 @@ __global__ __launch_bounds__(256) void k_surr_linear(SeaDev<R> S, SurrJob J) {
     X[J.nx - 1] = (float)J.kp;
 }
 
+// X[q][col] = src[q] (row stride `stride`): a column of FP64 values into an FP32 matrix
+__global__ void k_surr_set_column(int64_t n, const double* __restrict__ src, float* __restrict__ X, int stride,
+                                  int col) {
+    const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q < n) X[q * stride + col] = (float)src[q];
+}
+
 // out = lin + pred elementwise (the surrogate's total fields)
 __global__ void k_surr_add(int64_t n, const float* __restrict__ a, const float* __restrict__ b, float* __restrict__ o) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -156,6 +170,14 @@ int surrogate_linear(Sea& s, const SurrJob& J) {
         hipLaunchKernelGGL((k_surr_linear<float>), dim3(nb), dim3(256), 0, st, s.dev<float>(), J);
     else
         hipLaunchKernelGGL((k_surr_linear<double>), dim3(nb), dim3(256), 0, st, s.dev<double>(), J);
+    CFDWM_HIP(hipGetLastError());
+    return 0;
+}
+
+int surrogate_set_column(int64_t n, const double* src, float* X, int stride, int col) {
+    if (n <= 0) return 0;
+    hipLaunchKernelGGL(k_surr_set_column, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, Engine::get().stream(),
+                       n, src, X, stride, col);
     CFDWM_HIP(hipGetLastError());
     return 0;
 }

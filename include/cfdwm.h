@@ -320,6 +320,17 @@ CFDWM_EXPORT int cfdwm_mlp_train_device(cfdwm_mlp* mlp, int64_t n, const float* 
 CFDWM_EXPORT int cfdwm_surrogate_linear(cfdwm_sea* sea, int64_t n, const double* x, const double* y, double t,
                                         double h, int nlev, const double* sigma, int fstride, double kp,
                                         float* X, float* Lin);
+/* As cfdwm_surrogate_linear, with X rows of xstride floats (xstride >=
+ * nx + 3, nx = 3 + 3 ceil(nlev / fstride) + 2): the nx features above, then
+ * the quadrature (Hilbert pair) of eta1, sum A r sin(theta), and its x and y
+ * slopes; columns past nx + 3 are left to the caller. */
+CFDWM_EXPORT int cfdwm_surrogate_features(cfdwm_sea* sea, int64_t n, const double* x, const double* y, double t,
+                                          double h, int nlev, const double* sigma, int fstride, double kp,
+                                          int xstride, float* X, float* Lin);
+/* X[i * stride + col] = (float)src[i] for i < n (device pointers; enqueued on
+ * the library stream): an FP64 column (the exact eta2) into the surrogate's
+ * feature or output matrix. */
+CFDWM_EXPORT int cfdwm_surrogate_set_column(int64_t n, const double* src, float* X, int stride, int col);
 /* out[i] = a[i] + b[i] over n device floats (assembly of the surrogate fields). */
 CFDWM_EXPORT int cfdwm_surrogate_add(int64_t n, const float* a, const float* b, float* out);
 CFDWM_EXPORT int cfdwm_sync(void);

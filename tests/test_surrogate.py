@@ -1,7 +1,8 @@
-"""Learned second-order surrogate: linear-field feature kernel against the
-oracle, the committed network's gain over linear theory on held-out sea
-states, and the inference pipeline (features -> network -> assembly ->
-residual check)."""
+"""Learned second-order surrogate: the feature kernel against the oracle, the
+committed network's gain over linear theory on held-out sea states, and the
+hybrid inference pipeline (features -> exact eta2 form -> network ->
+assembly) against the exact kernels on the N=1024 short-crested sea, with
+the elevation gate |eta error| <= 1e-3 Hs asserted."""
 import os
 
 import numpy as np
@@ -16,7 +17,10 @@ WEIGHTS = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__)
 def test_weights_file_shapes():
     z = np.load(WEIGHTS, allow_pickle=False)
     nin, width, nh, nout = (int(v) for v in z["dims"])
-    assert (nin, nout) == (29, 65) and nh == 3 and width == 256
+    cols = z["feature_cols"]
+    assert str(z["mode"]) == "hybrid"
+    assert nin == cols.size and nout == 65 and nh == 3 and width == 256
+    assert cols.min() >= 0 and cols.max() < 33 and np.unique(cols).size == cols.size
     n = 0
     d = nin
     for l in range(nh + 1):
@@ -29,6 +33,11 @@ def test_weights_file_shapes():
 
 @pytest.mark.gpu
 def test_linear_features_vs_oracle(oracle):
+    """The 29 local features and the linear level fields against the oracle
+    (Airy members and their Taylor extension above the still-water level), the
+    quadrature features against the oracle's eta1 of the same sea with every
+    phase shifted by -pi/2 (cos(theta - pi/2) = sin theta) and its central
+    differences, and the eta2 column against the oracle's filtered eta2."""
     import torch
     from cfdwavemaker_amd import engine, surrogate as S
     g = golden("points.npz")
@@ -40,9 +49,9 @@ def test_linear_features_vs_oracle(oracle):
     n = 64
     x, y, t = rng.uniform(0, 200, n), rng.uniform(-50, 50, n), 3.7
     kp = S.peak_wavenumber(s)
-    X, Lin = S.linear_fields(es, torch.tensor(x, device="cuda"), torch.tensor(y, device="cuda"), t, h, kp)
+    X, Lin, e2 = S.linear_fields(es, torch.tensor(x, device="cuda"), torch.tensor(y, device="cuda"), t, h, kp)
     engine.sync()
-    X, Lin = X.cpu().numpy().astype(float), Lin.cpu().numpy().astype(float)
+    X, Lin, e2 = X.cpu().numpy().astype(float), Lin.cpu().numpy().astype(float), e2.cpu().numpy()
     e1 = osea.eval("eta1", t, x, y)
     sc = np.abs(e1).max()
     assert np.abs(X[:, 0] - e1).max() <= 1e-6 * sc
@@ -68,43 +77,65 @@ def test_linear_features_vs_oracle(oracle):
         if c < 3:
             feat = X[:, 3 + c:27:3]
             assert np.allclose(feat, got[:, ::2])
+    # quadrature: eta1 of the sea with phases - pi/2
+    q = dict(s)
+    q["phase"] = np.asarray(s["phase"], float) - np.pi / 2
+    oq = oracle.sea(q)
+    eh = oq.eval("eta1", t, x, y)
+    hx = (oq.eval("eta1", t, x + d, y) - oq.eval("eta1", t, x - d, y)) / (2 * d)
+    hy = (oq.eval("eta1", t, x, y + d) - oq.eval("eta1", t, x, y - d)) / (2 * d)
+    assert np.abs(X[:, S.COL_QUAD] - eh).max() <= 1e-6 * np.abs(eh).max()
+    hs = max(np.abs(hx).max(), np.abs(hy).max())
+    assert np.abs(X[:, S.COL_QUAD + 1] - hx).max() <= 1e-5 * hs
+    assert np.abs(X[:, S.COL_QUAD + 2] - hy).max() <= 1e-5 * hs
+    ref2 = osea.eval("eta2", t, x, y)
+    assert np.abs(e2 - ref2).max() <= 1e-9 * np.abs(ref2).max()
+    assert np.allclose(X[:, S.COL_ETA2], e2, rtol=1e-6, atol=1e-7 * np.abs(e2).max())
 
 
 @pytest.mark.gpu
 def test_network_gain_over_linear_theory():
     """Held-out sea states (never in the training set): the network's
-    second-order prediction beats the method without it (linear theory,
-    correction zero) for eta2 and the velocities."""
+    second-order horizontal and vertical velocity beats the method without it
+    (linear theory, correction zero) by at least 20% in RMS (measured 30% and
+    48% on these states).  The pressure output is reported, not asserted: on
+    these states it is worse than no correction (relative RMS error 9.7)."""
     import torch
     from cfdwavemaker_amd import surrogate as S
-    from cfdwavemaker_amd.mlp import Mlp
-    m, _ = Mlp.load_file(WEIGHTS)
+    sur = S.Surrogate.load()
     X, Y = S.build_dataset(range(20000, 20003), ncol=24)
-    P = m.forward(torch.tensor(X, device="cuda")).cpu().numpy()
-    for name, sl in (("eta2", slice(0, 1)), ("u2", slice(1, None, 4)), ("w2", slice(3, None, 4))):
+    P = sur.mlp.forward(torch.tensor(np.ascontiguousarray(X[:, sur.cols]), device="cuda")).cpu().numpy()
+    for name, sl in (("u2", slice(1, None, 4)), ("w2", slice(3, None, 4)), ("p2", slice(4, None, 4))):
         e = np.sqrt(np.mean((P[:, sl] - Y[:, sl]) ** 2))
         e0 = np.sqrt(np.mean(Y[:, sl] ** 2))            # error without the network
         print(f"{name}: rel rms error with network {e / e0:.3f} (without: 1.000)", flush=True)
-        assert e < 0.8 * e0, (name, e / e0)
-    m.close()
+        if name != "p2":
+            assert e < 0.8 * e0, (name, e / e0)
 
 
 @pytest.mark.gpu
-def test_inference_pipeline_and_residual_check():
+def test_hybrid_surrogate_meets_elevation_gate():
+    """64 x 64 columns over the N=1024 short-crested sea's 1 km x 1 km: the
+    surrogate elevation (eta1 + the exact eta2 form) is within 1e-3 Hs of the
+    exact kernels on every sampled column (the gate; no fallback), and the
+    predicted second-order velocities beat linear theory there too."""
     import torch
     from cfdwavemaker_amd import engine, surrogate as S
     from cfdwavemaker_amd.seastate import config_sea
     sea = config_sea(5)
     d = sea.as_dict()
-    es = engine.Sea(d, precision=engine.FP32, with_direct=False)
+    es = engine.Sea(d, precision=engine.FP32, with_direct=True)
     sur = S.Surrogate.load()
-    n = 32
-    g = np.linspace(0, 300, n)
+    assert sur.mode == "hybrid"
+    n = 64
+    g = np.linspace(0, 1000, n)
     x, y = (a.ravel() for a in np.meshgrid(g, g, indexing="ij"))
     out, Lin, Yp = sur.fields(es, sea, torch.tensor(x, device="cuda"), torch.tensor(y, device="cuda"), 0.25)
     engine.sync()
     out, Lin, Yp = (a.cpu().numpy() for a in (out, Lin, Yp))
     assert np.all(np.isfinite(out)) and np.allclose(out, Lin + Yp, atol=1e-6 * np.abs(out).max())
-    err, ok = sur.residual_check(es, x, y, 0.25, out[:, 0], np.arange(0, n * n, 7), 1e-3 * sea.hs)
-    print(f"surrogate eta max err on sample {err:.4f} m (gate {1e-3 * sea.hs} m): {'pass' if ok else 'fallback'}")
-    assert np.isfinite(err) and ok == (err <= 1e-3 * sea.hs)
+    res = sur.residual_check(es, x, y, 0.25, out[:, 0], Yp, np.arange(0, n * n, 13), 1e-3 * sea.hs, sea.depth)
+    print(f"surrogate residual check: {res}", flush=True)
+    assert res["pass"] and res["eta_max_err_m"] <= 1e-3 * sea.hs
+    for f in ("u2", "w2"):
+        assert res[f]["rel_rms_err"] < 0.8, (f, res[f])
