@@ -44,7 +44,8 @@ BATCH_API = [
 class BatchStatsC(C.Structure):
     _fields_ = [(n, C.c_int) for n in ("groups", "batched", "single", "tiles", "blocks", "order", "tuned")] + \
                [("forms_ms", C.c_double), ("mfma_flop", C.c_double)] + \
-               [("group_forms", C.c_int), ("group_ptiles", C.c_int), ("tables_ms", C.c_double)]
+               [("group_forms", C.c_int), ("group_ptiles", C.c_int), ("tables_ms", C.c_double),
+                ("setup_ms", C.c_double)]
 
 
 class SeaParams(C.Structure):

@@ -1072,6 +1072,7 @@ int cfdwm_grids_refill_batched(cfdwm_grid** grids, int n, double budget_gb, int 
         out->forms_ms = S.forms_ms; out->mfma_flop = S.mfma_flop;
         out->group_forms = S.group_forms; out->group_ptiles = S.group_ptiles;
         out->tables_ms = S.tables_ms;
+        out->setup_ms = S.setup_ms;
     }
     return 0;
 }

@@ -287,10 +287,11 @@ double freq_ramp_host(double t, double rt, double ts, double du) {
 // (coefficients cP = c_q(kP), cM = c_q(kM)):  u: B1 = cuP cP + cuM cM,
 // B2 = cuM cM - cuP cP;  v likewise with cv;  w (sin-type): B1 = cwP cP + cwM cM,
 // B2 = cwP cP - cwM cM.
-int Sea::qform_tables(const QfTables** out) {
+int Sea::qform_meta(const QfTables** out) {
     *out = nullptr;
     if (qf_) { *out = qf_.get(); return 1; }
     if (prec != kFP32) return 0;
+    if (qf_fallback) return 0;
     const PairTable& P = tab.filt;
     const int N = tab.comp.n;
     const double h = tab.in.depth;
@@ -321,35 +322,82 @@ int Sea::qform_tables(const QfTables** out) {
     T->npad = npad; T->nform = nform; T->rank = R; T->mu = DB.mu; T->basis_err = DB.err;
     T->basis_tol = tol; T->nlv = nlv; T->nbarr = nbarr;
     T->kend_h = kend;
-    hipStream_t st = Engine::get().stream();
-    DevBuf tau, einv;
-    // one product per term: the velocity forms (hi parts) in b, the eta2 form
-    // (hi and lo parts, three products) in beta
+    for (size_t t = 0; t < kend.size(); ++t) T->cost += std::max(0, kend[t] - (int)(64 * t));
+    // one product per term: the velocity forms (hi parts) in one buffer, the
+    // eta2 form (hi and lo parts, three products) in another
     const bool split = nlv == 1;
-    const size_t bytes = (size_t)(split ? nform - 1 : nform) * nbarr * nn * sizeof(uint16_t);
-    const size_t ebytes = split ? 4 * nn * sizeof(uint16_t) : 0;
-    if (split && T->beta.alloc(ebytes)) return -3;
-    if (split) CFDWM_HIP(hipMemsetAsync(T->beta.get(), 0, ebytes, st));
-    if (T->b.alloc(bytes) || tau.upload(DB.tau.data(), R * sizeof(double), st) ||
-        einv.upload(DB.einv.data(), (size_t)R * R * sizeof(double), st) ||
+    T->table_bytes = (size_t)(split ? nform - 1 : nform) * nbarr * nn * sizeof(uint16_t);
+    T->eta_bytes = split ? 4 * nn * sizeof(uint16_t) : 0;
+    hipStream_t st = Engine::get().stream();
+    if (T->tau.upload(DB.tau.data(), R * sizeof(double), st) ||
+        T->einv.upload(DB.einv.data(), (size_t)R * R * sizeof(double), st) ||
         T->ftype.upload(ftype.data(), ftype.size(), st) ||
         T->kend.upload(kend.data(), kend.size() * sizeof(int32_t), st))
         return -3;
-    CFDWM_HIP(hipMemsetAsync(T->b.get(), 0, bytes, st));
-    QfBuild Bq;
-    Bq.n = N; Bq.npad = npad; Bq.R = R; Bq.nform = nform; Bq.nbarr = nbarr;
-    Bq.frow = d32_.frow; Bq.fmhi = d32_.fmhi; Bq.uvw = d32_.fuvw; Bq.x = d32_.fx;
-    Bq.tau = (const double*)tau.get(); Bq.einv = (const double*)einv.get();
-    Bq.b = (uint16_t*)T->b.get();
-    Bq.beta = split ? (uint16_t*)T->beta.get() : nullptr;
-    if (N > 0) CFDWM_HIP(launch_qf_build(Bq, st));
-    CFDWM_HIP(hipStreamSynchronize(st));
+    CFDWM_HIP(hipStreamSynchronize(st));       // host staging vectors go out of scope
     qf_ = std::move(T);
     *out = qf_.get();
     return 1;
 }
 
-void Sea::release_qform() { qf_.reset(); }
+// Zero the table memory and launch the table build into it (library stream).
+int Sea::qform_build(QfTables& T, uint16_t* b, uint16_t* beta) {
+    hipStream_t st = Engine::get().stream();
+    const bool split = T.nlv == 1;
+    if (split) CFDWM_HIP(hipMemsetAsync(beta, 0, T.eta_bytes, st));
+    CFDWM_HIP(hipMemsetAsync(b, 0, T.table_bytes, st));
+    QfBuild Bq;
+    Bq.n = tab.comp.n; Bq.npad = T.npad; Bq.R = T.rank; Bq.nform = T.nform; Bq.nbarr = T.nbarr;
+    Bq.frow = d32_.frow; Bq.fmhi = d32_.fmhi; Bq.uvw = d32_.fuvw; Bq.x = d32_.fx;
+    Bq.tau = (const double*)T.tau.get(); Bq.einv = (const double*)T.einv.get();
+    Bq.b = b;
+    Bq.beta = split ? beta : nullptr;
+    if (Bq.n > 0) PROF("k_qf_build", launch_qf_build(Bq, st));
+    T.bp = b;
+    T.betap = split ? beta : nullptr;
+    return 0;
+}
+
+int Sea::qform_tables(const QfTables** out) {
+    const QfTables* M = nullptr;
+    const int rc = qform_meta(&M);
+    *out = nullptr;
+    if (rc != 1) return rc;
+    QfTables& T = *qf_;
+    if (!T.bp) {
+        if (T.b.alloc(T.table_bytes) || (T.eta_bytes && T.beta.alloc(T.eta_bytes))) return -3;
+        if (int e = qform_build(T, (uint16_t*)T.b.get(), (uint16_t*)T.beta.get())) return e;
+        CFDWM_HIP(hipStreamSynchronize(Engine::get().stream()));
+    }
+    *out = &T;
+    return 1;
+}
+
+int Sea::qform_tables_into(void* mem, const QfTables** out) {
+    const QfTables* M = nullptr;
+    const int rc = qform_meta(&M);
+    *out = nullptr;
+    if (rc != 1) return rc;
+    QfTables& T = *qf_;
+    if (!T.bp) {
+        char* m = (char*)mem;
+        const size_t tb = (T.table_bytes + 255) / 256 * 256;
+        if (int e = qform_build(T, (uint16_t*)m, (uint16_t*)(m + tb))) return e;
+        qf_ext_ = true;
+    }
+    *out = &T;
+    return 1;
+}
+
+// Drop the tables (owned or in caller memory); the sizes and basis stay.
+void Sea::release_qform() {
+    if (!qf_) return;
+    qf_->b.release();
+    qf_->beta.release();
+    qf_->bp = nullptr;
+    qf_->betap = nullptr;
+    qf_ext_ = false;
+}
 
 const Sea::SmitDev* Sea::smit_tables() {
     if (smit_) return smit_.get();
@@ -412,7 +460,7 @@ int points_eta2_form(Sea& s, int64_t n, const double* dx, const double* dy, cons
     Q.zc_hi = z; Q.zc_lo = z + zn; Q.zs_hi = z + 2 * zn; Q.zs_lo = z + 3 * zn;
     Q.zn = (int64_t)zn;
     // eta2 only, with three products per term in every mode
-    Q.b = (const uint16_t*)(T->nlv == 1 ? T->beta.get() : T->b.get());
+    Q.b = T->nlv == 1 ? T->betap : T->bp;
     Q.nbarr = 4; Q.nlv = 3;
     Q.ftype = (const int8_t*)T->ftype.get();
     Q.nform = 1;
@@ -739,7 +787,13 @@ int Grid::qform_prepare(int nslot, const double* slot_t, int64_t ncol, const int
     qf_chunk_ = std::min(ppad_all, cap);
     qf_P_ = P;
     const size_t zn = (size_t)qf_chunk_ * npad;
-    if (zbuf_.alloc(zn * 4 * sizeof(uint16_t))) return -3;
+    uint16_t* z = nullptr;
+    if (zext_ && zext_bytes_ >= zn * 4 * sizeof(uint16_t)) {
+        z = zext_;                            // a batched refill's arena
+    } else {
+        if (zbuf_.alloc(zn * 4 * sizeof(uint16_t))) return -3;
+        z = (uint16_t*)zbuf_.get();
+    }
     if (qout_.alloc((size_t)T->nform * ppad_all * sizeof(float))) return -3;
     QfJob& Q = qj_;
     Q = QfJob{};
@@ -748,7 +802,6 @@ int Grid::qform_prepare(int nslot, const double* slot_t, int64_t ncol, const int
     for (int s = 0; s < nslot; ++s) Q.slot_t[s] = slot_t[s];
     Q.x0 = sea->tab.in.x0; Q.y0 = sea->tab.in.y0;
     Q.ph = sea->dev<float>().ph;
-    uint16_t* z = (uint16_t*)zbuf_.get();
     Q.zc_hi = z; Q.zc_lo = z + zn; Q.zs_hi = z + 2 * zn; Q.zs_lo = z + 3 * zn;
     Q.zn = (int64_t)zn;
     Q.ftype = (const int8_t*)T->ftype.get();
@@ -760,16 +813,16 @@ int Grid::qform_prepare(int nslot, const double* slot_t, int64_t ncol, const int
     Q.ppad = (Q.P + 127) / 128 * 128;
     qf_split_ = T->nlv == 1;
     if (!qf_split_) {                             // every form, three products per term
-        Q.b = (const uint16_t*)T->b.get();
+        Q.b = T->bp;
         Q.nbarr = 4; Q.nlv = 3;
         Q.nform = T->nform;
     } else {                                      // eta2 at three products; velocity forms at one
-        Q.b = (const uint16_t*)T->beta.get();
+        Q.b = T->betap;
         Q.nbarr = 4; Q.nlv = 3;
         Q.nform = 1;
         QfJob& V = qj2_;
         V = Q;
-        V.b = (const uint16_t*)T->b.get();
+        V.b = T->bp;
         V.nbarr = T->nbarr; V.nlv = 1;
         V.ftype = Q.ftype + 1;
         V.nform = T->nform - 1;
@@ -931,6 +984,16 @@ int Grid::batch_end() {
     return 0;
 }
 
+size_t Grid::refill_z_bytes() const {
+    const Sea::QfTables* T = nullptr;
+    if (sea->qform_meta(&T) != 1) return 0;
+    const int64_t ncol = d.ignore_at_init ? nact_ : (int64_t)g.nx * g.ny;
+    const int64_t ppad = (4 * ncol + 127) / 128 * 128;
+    const int64_t cap = std::max<int64_t>(128, ((int64_t)1 << 31) / ((int64_t)T->npad * 8) / 128 * 128);
+    if (ppad > cap) return 0;
+    return (size_t)ppad * T->npad * 4 * sizeof(uint16_t);
+}
+
 void Grid::release_workspace() {
     zbuf_.release();
     qs_key_.release();
@@ -988,6 +1051,13 @@ const std::vector<BatchShape>& batch_candidates() {
     return c;
 }
 
+// Device arena of the batched refill's form tables and Z operands (one
+// group at a time); freed at the end of a call with release, else kept.
+DevBuf& batch_arena() {
+    static DevBuf a;
+    return a;
+}
+
 struct TuneKey {
     int nlv, jobs_log2, tiles_log2;
     bool operator<(const TuneKey& o) const {
@@ -1019,23 +1089,83 @@ int refill_batched(Grid** grids, int n, int64_t budget_bytes, bool release, Batc
     CFDWM_HIP(hipEventCreate(&e0));
     CFDWM_HIP(hipEventCreate(&e1));
     auto cleanup = [&]() { (void)hipEventDestroy(e0); (void)hipEventDestroy(e1); };
+    auto al = [](size_t b) { return (b + 255) / 256 * 256; };
+    // device bytes of a grid's share of a group: its sea's tables (unless
+    // shared with an earlier grid or resident) and its Z operands, or an
+    // estimate for grids whose forms do not apply
+    auto share = [&](Grid* g, size_t* tbytes, size_t* zbytes) -> int {
+        *tbytes = *zbytes = 0;
+        const Sea::QfTables* T = nullptr;
+        const int rc = g->sea->qform_meta(&T);
+        if (rc < 0) return rc;
+        if (rc == 1) {
+            if (!T->bp) *tbytes = al(T->table_bytes) + al(T->eta_bytes);
+            *zbytes = al(g->refill_z_bytes());
+        }
+        return 0;
+    };
+    hipEvent_t tb0 = nullptr, tb1 = nullptr;
+    CFDWM_HIP(hipEventCreate(&tb0));
+    CFDWM_HIP(hipEventCreate(&tb1));
+    auto cleanup_all = [&]() {
+        cleanup();
+        (void)hipEventDestroy(tb0);
+        (void)hipEventDestroy(tb1);
+        // on an error no sea or grid may keep pointers into the arena
+        for (int k = 0; k < n; ++k) {
+            if (grids[k]->sea->qform_external()) grids[k]->sea->release_qform();
+            grids[k]->set_z_external(nullptr, 0);
+        }
+    };
     while (i < n) {
-        // group of grids whose tables and operands fit the budget (estimate:
-        // 61 forms of bf16 hi/lo tables + Z operands + form values)
+        // group of grids whose tables and Z operands fit the budget
         std::vector<Grid*> grp;
+        std::vector<size_t> tbytes, zbytes;
         int64_t bytes = 0;
+        const auto h0 = std::chrono::steady_clock::now();
         while (i < n) {
             Grid* g = grids[i];
-            const int64_t npad = (g->sea->tab.comp.n + 127) / 128 * 128;
-            const int64_t pts = 4LL * g->g.nx * g->g.ny;
-            const int64_t tb = g->sea->qf_products == 1 ? 4 : 8;    // table bytes per entry and form
-            const int64_t est = 61 * tb * npad * npad + pts * npad * 8 + 61 * pts * 4;
+            size_t tb = 0, zb = 0;
+            if (g->sea->prec == kFP32)
+                if (int rc = share(g, &tb, &zb)) { cleanup_all(); return rc; }
+            // a sea shared by grids of this group is built once
+            for (Grid* o : grp)
+                if (o->sea == g->sea) tb = 0;
+            const int64_t est = (int64_t)(tb + zb);
             if (!grp.empty() && bytes + est > budget_bytes) break;
             grp.push_back(g);
+            tbytes.push_back(tb);
+            zbytes.push_back(zb);
             bytes += est;
             ++i;
         }
         ++S.groups;
+        // release != 0: the group's tables and Z operands live in one arena,
+        // reused by the next group (no allocation per sea or grid); else each
+        // sea keeps its own tables resident for later fills
+        CFDWM_HIP(hipEventRecord(tb0, st));
+        if (release) {
+            if (bytes > 0 && batch_arena().alloc((size_t)bytes)) { cleanup_all(); return -3; }
+            char* a = (char*)batch_arena().get();
+            size_t off = 0;
+            for (size_t k = 0; k < grp.size(); ++k) {
+                if (tbytes[k]) {
+                    const Sea::QfTables* T = nullptr;
+                    if (grp[k]->sea->qform_tables_into(a + off, &T) < 0) { cleanup_all(); return -3; }
+                    off += tbytes[k];
+                }
+                if (zbytes[k]) {
+                    grp[k]->set_z_external((uint16_t*)(a + off), zbytes[k]);
+                    off += zbytes[k];
+                }
+            }
+        } else {
+            for (Grid* g : grp) {
+                const Sea::QfTables* T = nullptr;
+                if (g->sea->prec == kFP32 && g->sea->qform_tables(&T) < 0) { cleanup_all(); return -3; }
+            }
+        }
+        CFDWM_HIP(hipEventRecord(tb1, st));
         // per product mode (3: hi/lo split, 1: hi only) the jobs of the group
         // and their costs; a grid whose velocity forms take one product adds
         // its eta2 form to the three-product list and its velocity forms to
@@ -1044,29 +1174,28 @@ int refill_batched(Grid** grids, int n, int64_t budget_bytes, bool release, Batc
         std::vector<double> cost3, cost1;
         std::vector<char> joined(grp.size(), 0);
         for (size_t k = 0; k < grp.size(); ++k) {
-            if (grp[k]->sea->prec == kFP32) {
-                // the sea's form tables (built here unless still resident)
-                const auto c0 = std::chrono::steady_clock::now();
-                const Sea::QfTables* T = nullptr;
-                if (grp[k]->sea->qform_tables(&T) < 0) { cleanup(); return -3; }
-                S.tables_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - c0).count();
-            }
             const int rc = grp[k]->batch_begin();
-            if (rc < 0) { cleanup(); return rc; }
+            if (rc < 0) { cleanup_all(); return rc; }
             if (rc == 1) {
                 joined[k] = 1;
-                double c = 0;
                 const Sea::QfTables* T = nullptr;
-                (void)grp[k]->sea->qform_tables(&T);  // cached by batch_begin
-                for (size_t t = 0; t < T->kend_h.size(); ++t) c += std::max(0, T->kend_h[t] - (int)(64 * t));
+                (void)grp[k]->sea->qform_meta(&T);
                 jobs3.push_back(grp[k]->batch_job());
-                cost3.push_back(c);
+                cost3.push_back(T->cost);
                 if (grp[k]->batch_split()) {
                     jobs1.push_back(grp[k]->batch_job2());
-                    cost1.push_back(c);
+                    cost1.push_back(T->cost);
                 }
                 S.mfma_flop += grp[k]->qf_flop;
             }
+        }
+        {
+            // host time of the group's set-up plus the device time of its table builds
+            float ms = 0;
+            CFDWM_HIP(hipEventSynchronize(tb1));
+            CFDWM_HIP(hipEventElapsedTime(&ms, tb0, tb1));
+            S.tables_ms += ms;
+            S.setup_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - h0).count();
         }
         // one persistent launch per product mode, its shape from the tuner
         auto launch_list = [&](const std::vector<QfJob>& jobs, const std::vector<double>& cost, int nlv) -> int {
@@ -1121,18 +1250,21 @@ int refill_batched(Grid** grids, int n, int64_t budget_bytes, bool release, Batc
             S.group_ptiles = choice.gp;
             return 0;
         };
-        if (int rc = launch_list(jobs3, cost3, 3)) { cleanup(); return rc; }
-        if (int rc = launch_list(jobs1, cost1, 1)) { cleanup(); return rc; }
+        if (int rc = launch_list(jobs3, cost3, 3)) { cleanup_all(); return rc; }
+        if (int rc = launch_list(jobs1, cost1, 1)) { cleanup_all(); return rc; }
         for (size_t k = 0; k < grp.size(); ++k) {
             const int rc = joined[k] ? grp[k]->batch_end() : grp[k]->refill();
-            if (rc) { cleanup(); return rc; }
+            if (rc) { cleanup_all(); return rc; }
             if (joined[k]) ++S.batched; else ++S.single;
         }
-        if (hipStreamSynchronize(st) != hipSuccess) { cleanup(); return -3; }
-        if (release)
-            for (Grid* g : grp) { g->sea->release_qform(); g->release_workspace(); }
+        if (hipStreamSynchronize(st) != hipSuccess) { cleanup_all(); return -3; }
+        for (Grid* g : grp) {
+            if (release) { g->sea->release_qform(); g->release_workspace(); }
+            g->set_z_external(nullptr, 0);
+        }
     }
-    cleanup();
+    if (release) batch_arena().release();
+    cleanup_all();
     if (stats) *stats = S;
     return 0;
 }

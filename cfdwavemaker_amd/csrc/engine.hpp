@@ -279,18 +279,32 @@ public:
         int nlv = 3, nbarr = 4;              // bf16 products per term, table arrays per form
         double basis_err = 0, basis_tol = 0;
         std::vector<double> mu;              // skeleton wavenumbers of the depth basis
-        // [nform][nbarr][npad x npad] tiled; with one product per term (nlv 1)
-        // it holds the 3 R velocity forms (hi parts) and beta the eta2 form
-        // (hi and lo parts: the elevation keeps three products per term)
+        // the tables: [nform][nbarr][npad x npad] tiled; with one product per
+        // term (nlv 1) bp holds the 3 R velocity forms (hi parts) and betap the
+        // eta2 form (hi and lo parts: the elevation keeps three products per
+        // term).  They live in b / beta, or in caller memory (a batched
+        // refill's arena); null while not built.
+        const uint16_t* bp = nullptr;
+        const uint16_t* betap = nullptr;
         DevBuf b, beta;
+        size_t table_bytes = 0, eta_bytes = 0;   // device bytes of bp and betap
+        // kept while the tables come and go: sizes, basis, tile ends
         DevBuf ftype;                        // [nform] int8
         DevBuf kend;                         // per 64-wide tile of i
+        DevBuf tau, einv;                    // depth-basis skeleton depths and interpolation matrix
         std::vector<int32_t> kend_h;
+        double cost = 0;                     // sum over i tiles of their m range (tile ordering)
     };
     // 1 and *out set: tables built (or cached); 0: the forms do not apply to
     // this sea (FP64, or no depth basis within the tolerance at rank <= 32:
     // the fills then run the pair loop); < 0: device error
     int qform_tables(const QfTables** out);
+    // The sizes and basis of the tables without building them (1, 0 or < 0 as above).
+    int qform_meta(const QfTables** out);
+    // Build the tables into caller device memory (table_bytes + eta_bytes,
+    // 256-byte aligned parts) on the library stream, without waiting; they
+    // stay valid until release_qform or the memory's reuse.
+    int qform_tables_into(void* mem, const QfTables** out);
     void release_qform();
     // bf16 products per term of the forms (3: hi/lo split, 1: hi only) and the
     // depth-basis tolerance (0: 1e-6 for 3 products, 1e-4 for 1 product)
@@ -317,6 +331,11 @@ private:
     std::map<std::pair<bool, int>, std::unique_ptr<DevBuf>> seg_cache_;
     std::map<int, std::unique_ptr<DevBuf>> cseg_cache_;
     std::unique_ptr<QfTables> qf_;
+    int qform_build(QfTables& T, uint16_t* b, uint16_t* beta);
+    bool qf_ext_ = false;                    // the tables live in caller memory
+public:
+    bool qform_external() const { return qf_ext_; }
+private:
     std::unique_ptr<SmitDev> smit_;
     SeaDev<double> d64_{};
     SeaDev<float> d32_{};
@@ -385,6 +404,13 @@ private:
     bool prune_ = true;
     bool qform_ = true;                       // FP32: eta2 pair sum as an MFMA quadratic form
     DevBuf zbuf_;                             // its Z operands (one chunk of points)
+    uint16_t* zext_ = nullptr;                // or caller memory (a batched refill's arena)
+    size_t zext_bytes_ = 0;
+public:
+    // Z operand bytes of a full refill whose forms fit one launch (0: they do not)
+    size_t refill_z_bytes() const;
+    void set_z_external(uint16_t* p, size_t bytes) { zext_ = p; zext_bytes_ = bytes; }
+private:
     DevBuf qout_;                             // form values [nform][points of the fill]
     QfJob qj_{};                              // prepared forms launch (first chunk): all forms, or
                                               // the eta2 form alone when qj2_ holds the velocity forms
@@ -437,7 +463,8 @@ struct BatchStats {
     int groups = 0, batched = 0, single = 0, tiles = 0, blocks = 0, order = 0, tuned = 0;
     double forms_ms = 0, mfma_flop = 0;
     int group_forms = 0, group_ptiles = 0;   // tile grouping of the tuned launch (order 1)
-    double tables_ms = 0;                    // form tables built during the call (host + device)
+    double tables_ms = 0;                    // device time of the form-table builds of the call
+    double setup_ms = 0;                     // host time of the groups' set-up (incl. waiting for the builds)
 };
 int refill_batched(Grid** grids, int n, int64_t budget_bytes, bool release, BatchStats* stats);
 // Device bytes of the cell-ordering buffers of an n-query batch on a grid of

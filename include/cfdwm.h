@@ -263,7 +263,8 @@ typedef struct {
     int groups, batched, single, tiles, blocks, order, tuned;
     double forms_ms, mfma_flop;
     int group_forms, group_ptiles;   /* tile grouping of the tuned launch (order 1) */
-    double tables_ms;                /* form tables built during the call */
+    double tables_ms;                /* device time of the form-table builds of the call */
+    double setup_ms;                 /* host time of the groups' set-up */
 } cfdwm_batch_stats;
 /* Write the s-grid as an ASCII VTK XML UnstructuredGrid (.vtu) for QA in
  * ParaView (lsGridSpline::export_vtu, src/lsgrid_spline.cpp:1222-1403): the
