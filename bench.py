@@ -47,21 +47,19 @@ DOMAIN = (0.0, 1000.0, 0.0, 1000.0)
 T0, DT, TAN_A, TAN_B, DEPTH, HS = 0.0, 0.5, 0.7, 1.5, 100.0, 8.0
 
 
-def cpu_worker(ncols: int, out_path: str) -> None:
-    """Times the reference's OpenMP s-grid fill (initialize_kinematics) on an
-    ncols x ncols subset of the same grid (same spacing), or the oracle's
-    column body when the compiled reference is not present."""
+def _grid_fill_seconds(ncols: int):
+    """One timed s-grid fill (initialize_kinematics) of an ncols x ncols subset
+    of the N=1024 grid (same spacing) by the compiled reference's OpenMP code,
+    or by the oracle's column body where the reference is not present."""
     from cfdwavemaker_amd.seastate import config_sea
     s = config_sea(3).as_dict()
     dx = (DOMAIN[1] - DOMAIN[0]) / (NX - 1)
     dom = (0.0, dx * (ncols - 1), 0.0, dx * (ncols - 1))
-    res = {}
     try:
         from oracle import RefHarness
         R = RefHarness()
         R.setup(s)
-        sec = R.grid_init(dom, ncols, ncols, NL, T0, DT, TAN_A, TAN_B, DEPTH)
-        res = dict(kind="reference", seconds=sec, points=4 * ncols * ncols * NL)
+        return "reference", R.grid_init(dom, ncols, ncols, NL, T0, DT, TAN_A, TAN_B, DEPTH)
     except FileNotFoundError:
         from oracle import Oracle
         O = Oracle().sea(s)
@@ -70,7 +68,52 @@ def cpu_worker(ncols: int, out_path: str) -> None:
         for t in (T0 - DT, T0, T0 + DT, T0 + 2 * DT):
             sec, *_ = O.column_fill(xs.ravel(), ys.ravel(), t, NL, TAN_A, TAN_B, DEPTH)
             tot += sec
-        res = dict(kind="port", seconds=tot, points=4 * ncols * ncols * NL)
+        return "port", tot
+
+
+def _direct_seconds(ncolumns: int):
+    """One timed direct-mode evaluation by the compiled reference (Irregular
+    members in an OpenMP parallel loop over points): eta at ncolumns (x, t)
+    columns of the N=512 long-crested workload and u, v, w, dp at its 64 z
+    per column (the points of the GPU direct leg's first columns)."""
+    import time as _t
+    from cfdwavemaker_amd.seastate import config_sea
+    from oracle import RefHarness
+    d = config_sea(2).as_dict()
+    R = RefHarness()
+    R.setup(d)
+    nz = 64
+    xs = np.linspace(0.0, 500.0, 256)[:ncolumns]
+    ts = np.arange(ncolumns, dtype=float) % 16
+    ys = np.zeros(ncolumns)
+    t0 = _t.perf_counter()
+    eta = R.eval("eta", ts, xs, ys)
+    frac = np.linspace(0.0, 1.0, nz)
+    z = (-30.0 + (eta[:, None] + 30.0) * frac[None, :]).ravel()
+    rep = lambda a: np.repeat(a, nz)
+    for kind in ("u", "v", "w", "dp"):
+        R.eval(kind, rep(ts), rep(xs), rep(ys), z)
+    return "reference", _t.perf_counter() - t0, ncolumns * nz
+
+
+def cpu_worker(spec: str, out_path: str) -> None:
+    """Timing protocol of one CPU baseline: one warm-up run, then five runs
+    at each of two sample sizes; per size the median.  spec: "grid:a,b"
+    (ncols x ncols s-grid subsets) or "direct:a,b" (columns of 64 points)."""
+    mode, sizes = spec.split(":")
+    sizes = [int(v) for v in sizes.split(",")]
+    res = {"mode": mode, "runs": {}}
+    if mode == "grid":
+        kind, _ = _grid_fill_seconds(sizes[0])                  # warm-up
+        for n in sizes:
+            secs = [_grid_fill_seconds(n)[1] for _ in range(5)]
+            res["runs"][str(n)] = {"points": 4 * n * n * NL, "seconds": secs}
+    else:
+        kind, _, _ = _direct_seconds(sizes[0])                 # warm-up
+        for n in sizes:
+            runs = [_direct_seconds(n) for _ in range(5)]
+            res["runs"][str(n)] = {"points": runs[0][2], "seconds": [r[1] for r in runs]}
+    res["kind"] = kind
     with open(out_path, "w") as f:
         json.dump(res, f)
 
@@ -89,18 +132,31 @@ def host_cpu() -> str:
     return f"{model}, {os.cpu_count()} logical CPUs"
 
 
-def cpu_baseline(ncols: int, threads: int):
+def cpu_baseline(spec: str, threads: int, what: str):
+    """Run one CPU baseline (cpu_worker protocol) in a child process with
+    `threads` OpenMP threads; returns the cpu_baseline object: the larger
+    sample's median rate, both medians and the linearity of the two."""
     import tempfile
     env = dict(os.environ, OMP_NUM_THREADS=str(threads), OMP_PROC_BIND="close", OMP_PLACES="cores")
     with tempfile.TemporaryDirectory() as td:
         out = os.path.join(td, "cpu.json")
-        r = subprocess.run([sys.executable, os.path.abspath(__file__), "--cpu-worker", str(ncols), out],
+        r = subprocess.run([sys.executable, os.path.abspath(__file__), "--cpu-worker", spec, out],
                            cwd=td, env=env, stdout=subprocess.DEVNULL, stderr=subprocess.PIPE, text=True,
                            timeout=900)
         if r.returncode != 0:
             raise RuntimeError(r.stderr[-2000:])
         with open(out) as f:
-            return json.load(f)
+            c = json.load(f)
+    rates = {}
+    for n, run in c["runs"].items():
+        med = float(np.median(run["seconds"]))
+        rates[n] = {"points": run["points"], "median_s": med, "runs_s": run["seconds"],
+                    "points_per_s": run["points"] / med}
+    keys = list(rates)
+    lin = rates[keys[-1]]["points_per_s"] / rates[keys[0]]["points_per_s"]
+    return {"value": rates[keys[-1]]["points_per_s"], "unit": "points/s", "cores": threads, "kind": c["kind"],
+            "host": host_cpu(), "sample": what, "protocol": "1 warm-up, median of 5 runs at each of 2 sizes",
+            "sizes": rates, "linearity": lin}
 
 
 # PMC calibration of the committed profiles: a 256 MiB device copy reads
@@ -380,6 +436,39 @@ def abi_worker(out_path: str) -> None:
     res["bitwise_equal"] = bool(all(np.array_equal(h, d.cpu().numpy()) for h, d in zip(host, D)))
     res["workload"] = (f"wave_Initialize on the {NX}x{NY}x{NL} s-grid of the N=1024 sea (FP32), then "
                        f"{n} points: eta at z = 0 and u, v, w at z, FP64 arrays")
+    # per-point calls as a CFD host makes them: 16 OpenMP threads, each
+    # calling wave_SurfElev and wave_VeloX/Y/Z per point; the first time value
+    # queries every point on its own (and remembers them), the next one is
+    # served from one replayed batch; then the same with the cache off
+    H = C.CDLL(os.path.join(ROOT, "cfdwavemaker_amd", "libcfdwm_hostdrv.so"))
+    I64 = C.c_int64
+    H.cfdwm_host_sweep.argtypes = [C.c_int, I64, P, P, P, C.c_double, P, P]
+    L.wave_PrefetchAuto.argtypes = [C.c_int]
+    L.wave_PrefetchAutoStats.argtypes = [C.POINTER(I64)] * 4
+    npp, threads = 20_000, 16
+    px, py, pz = x[:npp].copy(), y[:npp].copy(), z[:npp].copy()
+
+    def sweep(t):
+        out = np.zeros(4 * npp)
+        sec = C.c_double()
+        if H.cfdwm_host_sweep(threads, npp, ptr(px), ptr(py), ptr(pz), t, ptr(out), C.byref(sec)) != 0:
+            raise RuntimeError("host sweep failed")
+        return out, sec.value
+    L.wave_PrefetchAuto(1)
+    _, s_first = sweep(T0 + 0.25)
+    A, s_cached = sweep(T0 + 0.30)
+    st = [I64() for _ in range(4)]
+    L.wave_PrefetchAutoStats(*[C.byref(v) for v in st])
+    L.wave_PrefetchAuto(0)
+    B, s_unc = sweep(T0 + 0.30)
+    calls = 4 * npp
+    res["per_point_threads"] = {
+        "workload": f"{npp} points x (wave_SurfElev + wave_VeloX/Y/Z) from {threads} host threads per time value",
+        "us_per_call_first_time_value": s_first / calls * 1e6,
+        "us_per_call_replayed": s_cached / calls * 1e6,
+        "us_per_call_uncached": s_unc / calls * 1e6,
+        "calls_per_s_replayed": calls / s_cached, "bitwise_equal_to_uncached": bool(np.array_equal(A, B)),
+        "cache": dict(zip(("hits", "misses", "replays", "replayed_points"), (v.value for v in st)))}
     with open(out_path, "w") as f:
         json.dump(res, f)
 
@@ -490,7 +579,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--precision", default="bf16", choices=["bf16", "fp32", "fp64"])
     ap.add_argument("--queries", type=int, default=10_000_000)
-    ap.add_argument("--cpu-cols", type=int, default=12)
+    ap.add_argument("--cpu-cols", type=int, default=8)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-worker", nargs=2, default=None)
     ap.add_argument("--abi-worker", default=None)
@@ -510,7 +599,7 @@ def main():
         print(rocprof_summary(*a.rocprof_summary), end="")
         return
     if a.cpu_worker:
-        cpu_worker(int(a.cpu_worker[0]), a.cpu_worker[1])
+        cpu_worker(a.cpu_worker[0], a.cpu_worker[1])
         return
     if a.abi_worker:
         abi_worker(a.abi_worker)
@@ -671,15 +760,21 @@ def main():
     cpu = None
     if not a.no_cpu:
         threads = min(16, os.cpu_count() or 1)
-        print(f"[bench] cpu baseline ({a.cpu_cols}x{a.cpu_cols} columns, {threads} threads)", file=sys.stderr, flush=True)
+        print(f"[bench] cpu baselines ({threads} threads)", file=sys.stderr, flush=True)
         try:
-            c = cpu_baseline(a.cpu_cols, threads)
-            cpu = {"value": c["points"] / c["seconds"], "unit": "points/s", "cores": threads, "kind": c["kind"],
-                   "host": host_cpu(),
-                   "sample": f"s-grid fill of {a.cpu_cols}x{a.cpu_cols} columns x {NL} levels x 4 time levels "
-                             f"({c['points']} points, same N=1024 sea and spacing), {c['seconds']:.2f} s"}
+            cpu = cpu_baseline(f"grid:{a.cpu_cols // 2},{a.cpu_cols}", threads,
+                               f"s-grid fills of {a.cpu_cols // 2}^2 and {a.cpu_cols}^2 columns x {NL} levels x 4 "
+                               f"time levels of the same N=1024 sea and spacing (value: the larger)")
         except Exception as e:  # reported, not fatal
             cpu = {"value": None, "error": str(e)[-300:]}
+        if dirl is not None:
+            try:
+                dirl["cpu_baseline"] = cpu_baseline("direct:4,8", threads,
+                                                    "eta at 4 and 8 (x, t) columns and u, v, w, dp at their 64 z "
+                                                    "each, N=512 long-crested (value: the larger)")
+                dirl["fp32_over_cpu"] = dirl["fp32"]["points_per_s"] / dirl["cpu_baseline"]["value"]
+            except Exception as e:  # reported, not fatal
+                dirl["cpu_baseline"] = {"value": None, "error": str(e)[-300:]}
 
     line = {
         "metric": "2nd-order kinematics points/s (N=1024), exact GPU",

@@ -5,6 +5,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <array>
 #include <atomic>
 #include <unordered_map>
 #include <mutex>
@@ -415,11 +416,6 @@ void direct_point(double x, double y, double z, double t, double* eta, double* u
     if (eta) *eta = G.swl + e1 + (G.order == 2 ? e2 : 0.0);
 }
 
-double grid_point(double x, double y, double z, double t, int comp) {
-    double o4[4];
-    if (G.cgrid->g->query(1, &t, &x, &y, &z, o4, nullptr, nullptr)) gpu_fatal("s-grid query");
-    return o4[comp];
-}
 
 // ------------------------------------------------- prefetched point values
 // A CFD host knows the points it will ask for in a time step: evaluating
@@ -476,6 +472,109 @@ bool cached_eta(double x, double y, double t, double& out) {
     return true;
 }
 
+// ------------------------------------------ per-point calls on the s-grid
+// A CFD host that calls wave_SurfElev / wave_VeloX .. per cell asks for the
+// same points every time step.  On the s-grid each per-point call is one
+// spline query; the calls of one time value and grid state form a generation
+// whose points are remembered (the cache keys).  The first call at a new
+// time (or after wave_force_update advanced the grid) evaluates every point
+// of the previous generation at the new time in one batched query, and later
+// calls for those points are served from the cache under a shared lock, from
+// any number of host threads.  A point not in the cache is queried on its
+// own, as without the cache, and joins it.  The spline query evaluates every
+// point identically whatever the batch, so served values equal the uncached
+// ones bit for bit.  wave_PrefetchAuto(0) switches this off.
+struct K3 {
+    uint64_t a, b, c;
+    bool operator==(const K3& o) const { return a == o.a && b == o.b && c == o.c; }
+};
+struct K3Hash {
+    size_t operator()(const K3& k) const noexcept {
+        uint64_t h = k.a * 0x9E3779B97F4A7C15ull;
+        h ^= (k.b + 0x632BE59BD9B4E019ull + (h << 6) + (h >> 2));
+        h ^= (k.c * 0xBF58476D1CE4E5B9ull + (h << 6) + (h >> 2));
+        return (size_t)h;
+    }
+};
+struct AutoCache {
+    std::shared_mutex mu;
+    std::atomic<bool> on{true};
+    bool valid = false;                   // entries belong to (t_bits, version)
+    uint64_t t_bits = 0, version = 0;
+    std::unordered_map<K3, std::array<double, 4>, K3Hash> m;   // (x, y, z of the query) -> {eta, u, v, w} x ramp
+    std::atomic<int64_t> hits{0}, misses{0}, replays{0}, replayed{0};
+};
+AutoCache AC;
+std::atomic<uint64_t> g_grid_version{0};
+constexpr size_t kAutoCacheMax = size_t(1) << 23;
+
+// {eta, u, v, w} x ramp of the s-grid query at (x, y, zq, t), through the cache.
+void grid_kin(double x, double y, double zq, double t, double* o4) {
+    auto single = [&]() {
+        if (G.cgrid->g->query(1, &t, &x, &y, &zq, o4, nullptr, nullptr)) gpu_fatal("s-grid query");
+        const double r = G.ramp(t, x, y);
+        for (int q = 0; q < 4; ++q) o4[q] *= r;
+    };
+    if (!AC.on.load(std::memory_order_relaxed)) { single(); return; }
+    const K3 key{bits(x), bits(y), bits(zq)};
+    const uint64_t tb = bits(t), ver = g_grid_version.load();
+    {
+        std::shared_lock<std::shared_mutex> lk(AC.mu);
+        if (AC.valid && AC.t_bits == tb && AC.version == ver) {
+            auto it = AC.m.find(key);
+            if (it != AC.m.end()) {
+                for (int q = 0; q < 4; ++q) o4[q] = it->second[q];
+                ++AC.hits;
+                return;
+            }
+        }
+    }
+    std::unique_lock<std::shared_mutex> lk(AC.mu);
+    if (!(AC.valid && AC.t_bits == tb && AC.version == ver)) {
+        // a new generation: the previous generation's points at time t, one batch
+        const size_t n = AC.m.size() <= kAutoCacheMax ? AC.m.size() : 0;
+        std::vector<double> px(n), py(n), pz(n), pt(n, t), out(4 * n);
+        size_t k = 0;
+        for (const auto& e : AC.m) {
+            std::memcpy(&px[k], &e.first.a, 8);
+            std::memcpy(&py[k], &e.first.b, 8);
+            std::memcpy(&pz[k], &e.first.c, 8);
+            ++k;
+        }
+        AC.m.clear();
+        if (n) {
+            if (G.cgrid->g->query((int64_t)n, pt.data(), px.data(), py.data(), pz.data(), out.data(), nullptr, nullptr))
+                gpu_fatal("s-grid query");
+            AC.m.reserve(n);
+            for (size_t q = 0; q < n; ++q) {
+                const double r = G.ramp(t, px[q], py[q]);
+                AC.m[K3{bits(px[q]), bits(py[q]), bits(pz[q])}] = {out[4 * q] * r, out[4 * q + 1] * r,
+                                                                    out[4 * q + 2] * r, out[4 * q + 3] * r};
+            }
+            ++AC.replays;
+            AC.replayed += (int64_t)n;
+        }
+        AC.t_bits = tb;
+        AC.version = ver;
+        AC.valid = true;
+        auto it = AC.m.find(key);
+        if (it != AC.m.end()) {
+            for (int q = 0; q < 4; ++q) o4[q] = it->second[q];
+            ++AC.hits;
+            return;
+        }
+    }
+    single();
+    AC.m[key] = {o4[0], o4[1], o4[2], o4[3]};
+    ++AC.misses;
+}
+
+void auto_cache_reset() {
+    std::unique_lock<std::shared_mutex> lk(AC.mu);
+    AC.m.clear();
+    AC.valid = false;
+}
+
 double velo(double x, double y, double z, double t, int comp) {
     if (!G.initialized) return 0.0;
     {
@@ -483,12 +582,16 @@ double velo(double x, double y, double z, double t, int comp) {
         if (cached_vel(x, y, z, t, c)) return comp == 1 ? c.u : (comp == 2 ? c.v : c.w);
     }
     z = std::max(-G.depth, z);                       // CFDwavemaker.cpp:1547
+    if (G.wavetype == 5) {
+        double o4[4];
+        grid_kin(x, y, z, t, o4);
+        return o4[comp];
+    }
     if (G.wavetype == 1) {
         double r = 0;
         direct_point(x, y, z, t, nullptr, comp == 1 ? &r : nullptr, comp == 2 ? &r : nullptr, comp == 3 ? &r : nullptr, nullptr);
         return G.ramp(t, x, y) * r;
     }
-    if (G.wavetype == 5) return grid_point(x, y, z, t, comp) * G.ramp(t, x, y);
     return 0.0;
 }
 
@@ -534,6 +637,7 @@ int wave_Initialize(void) {
         std::exit(0);
     }
     if (G.csea || G.cgrid) wave_Cleanup();
+    auto_cache_reset();
     G = Global();
     parse(res);
     setup_after_parse();
@@ -543,6 +647,7 @@ int wave_Initialize(void) {
 
 int wave_Cleanup(void) {
     wave_PrefetchClear();
+    auto_cache_reset();
     if (G.cgrid) { cfdwm_grid_destroy(G.cgrid); G.cgrid = nullptr; }
     if (G.csea) { cfdwm_sea_destroy(G.csea); G.csea = nullptr; }
     return 0;
@@ -583,7 +688,11 @@ double wave_SurfElev(double x, double y, double t) {
         direct_point(x, y, 0.0, t, &eta, nullptr, nullptr, nullptr, nullptr);
         return G.ramp(t, x, y) * eta;
     }
-    if (G.wavetype == 5) return grid_point(x, y, 0.0, t, 0) * G.ramp(t, x, y);
+    if (G.wavetype == 5) {
+        double o4[4];
+        grid_kin(x, y, 0.0, t, o4);
+        return o4[0];
+    }
     return 0.0;
 }
 
@@ -609,10 +718,7 @@ double* wave_Kinematics(double x, double y, double z, double t) {
         }
     }
     if (G.wavetype == 5) {
-        double o4[4];
-        if (G.cgrid->g->query(1, &t, &x, &y, &z, o4, nullptr, nullptr)) gpu_fatal("s-grid query");
-        const double r = G.ramp(t, x, y);
-        for (int q = 0; q < 4; ++q) res[q] = o4[q] * r;
+        grid_kin(x, y, z, t, res);
     } else if (G.wavetype == 1) {
         const double zc = std::max(-G.depth, z);
         direct_point(x, y, zc, t, &res[0], &res[1], &res[2], &res[3], nullptr);
@@ -628,7 +734,10 @@ void wave_force_update(double t) {
     if (!G.initialized || G.wavetype != 5) return;
     int n = 0;
     if (cfdwm_grid_force_update(G.cgrid, t, &n)) gpu_fatal("s-grid update");
-    if (n > 0) wave_PrefetchClear();                  // kept values belong to the previous grid state
+    if (n > 0) {
+        wave_PrefetchClear();                         // kept values belong to the previous grid state
+        ++g_grid_version;                             // the per-point cache starts a new generation
+    }
 }
 
 void wave_set_mpi_comm(int) {}
@@ -817,6 +926,19 @@ int wave_PrefetchClear(void) {
     PC.vel.clear();
     PC.eta.clear();
     PC.has_vel = PC.has_p = PC.has_eta = false;
+    return 0;
+}
+int wave_PrefetchAuto(int enable) {
+    // the remembered points stay: kept values are only served for the time
+    // value and grid state they were computed at
+    const bool was = AC.on.exchange(enable != 0);
+    return was ? 1 : 0;
+}
+int wave_PrefetchAutoStats(int64_t* hits, int64_t* misses, int64_t* replays, int64_t* replayed_points) {
+    if (hits) *hits = AC.hits.load();
+    if (misses) *misses = AC.misses.load();
+    if (replays) *replays = AC.replays.load();
+    if (replayed_points) *replayed_points = AC.replayed.load();
     return 0;
 }
 int wave_PrefetchStats(int64_t* hits, int64_t* misses) {

@@ -105,13 +105,28 @@ CFDWM_EXPORT int wave_EvalPoints(int64_t n, const double* x, const double* y, co
  * evaluate the n points (outputs selected by CFDWM_OUT_* flags) in one batch
  * and keep the results; later wave_SurfElev / wave_VeloX/Y/Z / wave_DynPres /
  * wave_Kinematics calls whose arguments are bitwise equal to a prefetched
- * point return the kept value (identical to the uncached result) without a
- * GPU round trip.  A new prefetch replaces the previous one.  Call from one
- * thread, outside concurrent per-point calls (like wave_force_update). */
+ * point return the kept value without a GPU round trip.  Wave type 5: the
+ * kept value is the uncached result bit for bit.  Wave type 1: the batch sums
+ * the same terms in another order (its split over pair segments, and for FP32
+ * batches of 2048 points or more the eta2 quadratic form), so kept values
+ * agree with uncached ones to rounding level.  A new prefetch replaces the
+ * previous one.  Call from one thread, outside concurrent per-point calls
+ * (like wave_force_update). */
 CFDWM_EXPORT int wave_PrefetchPoints(int64_t n, const double* x, const double* y, const double* z,
                                      const double* t, uint32_t flags);
 CFDWM_EXPORT int wave_PrefetchClear(void);
 CFDWM_EXPORT int wave_PrefetchStats(int64_t* hits, int64_t* misses);
+/* Per-point calls on the s-grid (wave type 5; no reference counterpart, no
+ * host change needed): the points asked for at one time value and grid state
+ * are remembered, and the first per-point call at a new time (or after
+ * wave_force_update advanced the grid) evaluates all of them at that time in
+ * one batched query; later calls for those points are served from the kept
+ * values under a shared lock, concurrently from any number of host threads,
+ * bit-equal to the uncached values.  On by default; enable 0 switches it off
+ * (returns the previous setting). */
+CFDWM_EXPORT int wave_PrefetchAuto(int enable);
+CFDWM_EXPORT int wave_PrefetchAutoStats(int64_t* hits, int64_t* misses, int64_t* replays,
+                                        int64_t* replayed_points);
 /* As wave_EvalPoints, all pointers in device memory (the call synchronises
  * the library's stream before returning).  The points never leave the
  * device: the z clamp, the batch, the elevation sum and the ramps run as
