@@ -75,6 +75,9 @@ struct Global {
     double lsg_t0 = 0., lsg_dt = 0.1, lsg_tan_a = 0., lsg_tan_b = 0.;
     double lsg_ignore[4] = {1.E12, -1.E12, 1.E12, -1.E12};
     bool lsg_ignore_domain = false, lsg_ignore_at_init = false, lsg_init_only = false;
+    // [lsgrid2vtk] (lsgrid_spline.h:58-61 defaults)
+    bool vtk_on = false, vtk_last_done = false;
+    std::string vtk_dir = "./", vtk_prefix = "kinematics", vtk_label = "TimeValue";
     RampCfg ramp;
     int precision = kFP64;
     cfdwm_sea* csea = nullptr;
@@ -314,6 +317,34 @@ void parse(const std::string& res) {
             else input_error("LS grid may currently only be used with irregular second order wave theory or swd");
             G.has_grid = true;
         }
+        if (lineA == "[lsgrid2vtk]") {                // optional (src/CFDwavemaker.cpp:1205-1244)
+            std::cout << "--------------------" << std::endl << "VTK output settings:" << std::endl
+                      << "--------------------" << std::endl;
+            if (G.wavetype != 5) {
+                std::cout << "InputError: VTK output only works in combination with LSgrid at the moment." << std::endl;
+                std::exit(-1);
+            }
+            G.vtk_on = true;
+            while (!f.eof()) {
+                std::getline(f, lineA);
+                trim(lineA);
+                std::istringstream b(lineA);
+                std::string key;
+                if (starts(lineA, "storage_path")) {
+                    b >> key >> G.vtk_dir;
+                    std::cout << "Directory for storage of vtk files: " << G.vtk_dir << std::endl;
+                }
+                if (starts(lineA, "filename")) {
+                    b >> key >> G.vtk_prefix;
+                    std::cout << "filename prefix: " << G.vtk_prefix << std::endl;
+                }
+                if (starts(lineA, "vtk_timelabel")) {
+                    b >> key >> G.vtk_label;
+                    std::cout << "time label to use in vtk specified as: " << G.vtk_label << std::endl;
+                }
+                if (starts(lineA, "[")) { skip = true; break; }
+            }
+        }
     }
 }
 
@@ -345,6 +376,15 @@ int env_precision() {
 }
 
 // post-processing of process_inputdata (src/CFDwavemaker.cpp:1334-1437)
+// The end-of-run .vtu of the slot after the base slot (the reference writes it
+// from its grid's destructor): once, at wave_Cleanup or at process exit.
+void vtk_final_dump() {
+    if (G.vtk_on && !G.vtk_last_done && G.cgrid) {
+        G.vtk_last_done = true;
+        (void)G.cgrid->g->write_vtk_dump(true);
+    }
+}
+
 int setup_after_parse() {
     if (G.wavetype == 5 && G.order != 2)
         input_error("LSgrid interpolation uses strictly second order theory. Set kinematics_surf_extrap to 2 under [wave input irregular], or remove [lsgrid].");
@@ -353,6 +393,7 @@ int setup_after_parse() {
     SeaInput& S = G.sea;
     S.depth = G.depth; S.mtheta = G.mtheta; S.tofmax = G.tofmax; S.x0 = G.x_pos; S.y0 = G.y_pos;
     S.swl = G.swl; S.g = G.gravity; S.rho = G.rho; S.order = G.order;
+    S.nfreq = G.nfreq;
     if (G.bw_auto) {                                                    // bandwidth_estimator
         if (G.ndir > 1) input_error("Automatic calculation of bandwidth parameter only supported for 2D wave spectra");
         double m[3];
@@ -391,8 +432,24 @@ int setup_after_parse() {
         d.tan_a = G.lsg_tan_a; d.tan_b = G.lsg_tan_b; d.water_depth = G.depth; d.swl = G.swl;
         std::memcpy(d.domain_ignore, G.lsg_ignore, sizeof(d.domain_ignore));
         d.ignore_at_init = G.lsg_ignore_at_init ? 1 : 0;
+        d.defer_fill = 1;                             // the dump settings first, then the fill
         rc = cfdwm_grid_create(&G.cgrid, G.csea, &d);
         if (rc) gpu_fatal("s-grid initialisation");
+        if (G.vtk_on) {
+            Grid& gr = *G.cgrid->g;
+            gr.vtk.on = true;
+            gr.vtk.dir = G.vtk_dir;
+            gr.vtk.prefix = G.vtk_prefix;
+            gr.vtk.label = G.vtk_label;
+            // the reference writes the following slot once more when its grid
+            // is destroyed at process exit (lsgrid_spline.h:25-28)
+            static bool registered = false;
+            if (!registered) {
+                std::atexit([] { vtk_final_dump(); });
+                registered = true;
+            }
+        }
+        if (cfdwm_grid_refill(G.cgrid)) gpu_fatal("s-grid initialisation");
     }
     return 0;
 }
@@ -646,6 +703,7 @@ int wave_Initialize(void) {
 }
 
 int wave_Cleanup(void) {
+    vtk_final_dump();
     wave_PrefetchClear();
     auto_cache_reset();
     if (G.cgrid) { cfdwm_grid_destroy(G.cgrid); G.cgrid = nullptr; }

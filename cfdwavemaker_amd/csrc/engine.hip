@@ -4,7 +4,11 @@
 #include "depth_basis.hpp"
 
 #include <algorithm>
+#include <cerrno>
 #include <chrono>
+#include <iostream>
+#include <sys/stat.h>
+#include <sys/types.h>
 #include <cmath>
 #include <cstring>
 #include <cstdio>
@@ -1321,7 +1325,7 @@ int Grid::refill() {
     if (rc) return rc;
     CFDWM_HIP(hipStreamSynchronize(Engine::get().stream()));
     filled = true;
-    return 0;
+    return write_vtk_dump(false);                 // initialize_kinematics (lsgrid_spline.cpp:746-748)
 }
 
 int Grid::force_update(double t, int* n) {
@@ -1340,6 +1344,10 @@ int Grid::force_update(double t, int* n) {
         rc = prec == kFP32 ? pack<float>() : pack<double>();
         if (rc) return rc;
         ++k;
+        if (vtk.on) {                             // update (lsgrid_spline.cpp:924-926)
+            CFDWM_HIP(hipStreamSynchronize(Engine::get().stream()));
+            if ((rc = write_vtk_dump(false))) return rc;
+        }
     }
     if (k) CFDWM_HIP(hipStreamSynchronize(Engine::get().stream()));
     if (n) *n = k;
@@ -1367,10 +1375,14 @@ int64_t Grid::export_field(int which, double* out) {
 }
 
 // ASCII VTK XML UnstructuredGrid of the slot that the query uses as its base
-// (last: the following slot), at time t0 (last: t0 + dt): velocity vectors at
-// the nodes, node positions from the sigma map of the slot's elevation,
-// hexahedra between neighbouring nodes (quadrilaterals or lines when the grid
-// is flat in x or y), and the time as field data.
+// (last: the following slot), at time t0 (last: t0 + dt), in the layout and
+// number formats of the reference's lsGridSpline::export_vtu
+// (lsgrid_spline.cpp:1223-1403), so that tools reading its files read these:
+// velocity vectors at the nodes, node positions from the sigma map of the
+// slot's elevation, hexahedra between neighbouring nodes (quadrilaterals for
+// grids one node wide in x, y or the vertical; no cells for grids one node
+// wide in two directions, whose cell count is still given as the product of
+// max(n - 1, 1)), and the time as field data.
 int Grid::write_vtu(const char* path, bool last, const char* time_label) {
     if (!filled) { set_error("the s-grid has not been filled yet (created with defer_fill)"); return -1; }
     const int nx = g.nx, ny = g.ny, nl = g.nl;
@@ -1383,61 +1395,100 @@ int Grid::write_vtu(const char* path, bool last, const char* time_label) {
     FILE* fp = std::fopen(path, "w");
     if (!fp) { set_error(std::string("cannot open ") + path); return -1; }
     const char* lab = time_label && time_label[0] ? time_label : "TimeValue";
-    // cells: hexahedra (VTK type 12), or quadrilaterals (9) / lines (3) for flat grids
-    const int ex = nx > 1 ? nx - 1 : 0, ey = ny > 1 ? ny - 1 : 0, el = nl > 1 ? nl - 1 : 0;
-    int ctype = 0, cn = 0;
-    int64_t ncell = 0;
-    if (ex && ey && el) { ctype = 12; cn = 8; ncell = (int64_t)ex * ey * el; }
-    else if ((ex || ey) && el) { ctype = 9; cn = 4; ncell = (int64_t)(ex ? ex : ey) * el; }
-    else if (el) { ctype = 3; cn = 2; ncell = el; }
-    std::fprintf(fp, "<?xml version=\"1.0\"?>\n<VTKFile type=\"UnstructuredGrid\" version=\"1.0\" "
-                     "byte_order=\"LittleEndian\" header_type=\"UInt64\">\n <UnstructuredGrid>\n");
-    std::fprintf(fp, "  <FieldData>\n   <DataArray type=\"Float64\" Name=\"%s\" NumberOfTuples=\"1\" "
-                     "format=\"ascii\">%.6f</DataArray>\n  </FieldData>\n", lab, tv);
-    std::fprintf(fp, "  <Piece NumberOfPoints=\"%lld\" NumberOfCells=\"%lld\">\n", (long long)nv, (long long)ncell);
-    std::fprintf(fp, "   <PointData Vectors=\"Velocity\">\n    <DataArray type=\"Float64\" NumberOfComponents=\"3\" "
-                     "Name=\"Velocity\" format=\"ascii\">\n");
+    std::fputs("<?xml version=\"1.0\"?>\n<VTKFile type=\"UnstructuredGrid\" version=\"1.0\" "
+               "byte_order=\"LittleEndian\" header_type=\"UInt64\">\n", fp);
+    std::fputs("\t <UnstructuredGrid>\n\t\t <FieldData> \n", fp);
+    std::fprintf(fp, "\t\t\t <DataArray type = \"Float64\" Name = \"%s\" NumberOfTuples = \"1\" format = \"ascii\" "
+                     "RangeMin = \"%.3f\" RangeMax = \"%.3f\"> \n", lab, tv, tv);
+    std::fprintf(fp, "\t\t\t %.3f \n\t\t\t </DataArray > \n\t\t </FieldData> \n", tv);
+    const long long ncells = (long long)std::max(nx - 1, 1) * std::max(ny - 1, 1) * std::max(nl - 1, 1);
+    std::fprintf(fp, "\t\t <Piece NumberOfPoints=\"%lld\" NumberOfCells=\"%lld\">\n", (long long)nv, ncells);
+    std::fputs("\t\t\t <PointData Scalars=\"scalars\">\n", fp);
+    std::fputs("\t\t\t\t <DataArray type=\"Float64\" NumberOfComponents=\"3\" Name=\"Velocity\" format=\"ascii\">\n",
+               fp);
     const int64_t so = (int64_t)tt * nv;
-    for (int64_t q = 0; q < nv; ++q)
-        std::fprintf(fp, "%.9g %.9g %.9g\n", U[so + q], V[so + q], W[so + q]);
-    std::fprintf(fp, "    </DataArray>\n   </PointData>\n   <Points>\n    <DataArray type=\"Float64\" "
-                     "NumberOfComponents=\"3\" format=\"ascii\">\n");
-    for (int i = 0; i < nx; ++i)
+    for (int64_t q = 0; q < nv; ++q) std::fprintf(fp, "%g %g %g\n", U[so + q], V[so + q], W[so + q]);
+    std::fputs("\t\t\t\t </DataArray>\n\t\t\t </PointData>\n\t\t\t <Points>\n", fp);
+    std::fputs("\t\t\t\t <DataArray type=\"Float64\" NumberOfComponents=\"3\" format=\"ascii\">\n", fp);
+    for (int i = 0; i < nx; ++i) {
+        const double xp = g.x0 + g.dx * i;
         for (int j = 0; j < ny; ++j) {
-            const double e = eta[(int64_t)tt * nc + (int64_t)i * ny + j] - g.swl;
+            const double yp = g.y0 + g.dy * j;
+            const double e = eta[(int64_t)tt * nc + (int64_t)i * ny + j];
             for (int m = 0; m < nl; ++m) {
-                const double z = e + sigma_[m] * (g.h + e);          // s2z (lsgrid_spline.cpp:85-87)
-                std::fprintf(fp, "%.9g %.9g %.9g\n", g.x0 + g.dx * i, g.y0 + g.dy * j, z);
+                const double z = (e - g.swl) + sigma_[m] * (g.h + (e - g.swl));      // s2z (lsgrid_spline.cpp:96-98)
+                std::fprintf(fp, "%12.4f %12.4f %12.4f\n", xp, yp, z);
             }
         }
-    std::fprintf(fp, "    </DataArray>\n   </Points>\n   <Cells>\n    <DataArray type=\"Int64\" "
-                     "Name=\"connectivity\" format=\"ascii\">\n");
-    auto node = [&](int i, int j, int m) -> long long { return ((long long)i * ny + j) * nl + m; };
-    if (ctype == 12) {
-        for (int i = 0; i < ex; ++i)
-            for (int j = 0; j < ey; ++j)
-                for (int m = 0; m < el; ++m)
-                    std::fprintf(fp, "%lld %lld %lld %lld %lld %lld %lld %lld\n", node(i, j, m), node(i + 1, j, m),
-                                 node(i + 1, j + 1, m), node(i, j + 1, m), node(i, j, m + 1), node(i + 1, j, m + 1),
-                                 node(i + 1, j + 1, m + 1), node(i, j + 1, m + 1));
-    } else if (ctype == 9) {
-        const int nh = ex ? ex : ey;
-        for (int a = 0; a < nh; ++a)
-            for (int m = 0; m < el; ++m) {
-                const long long p0 = ex ? node(a, 0, m) : node(0, a, m), p1 = ex ? node(a + 1, 0, m) : node(0, a + 1, m);
-                std::fprintf(fp, "%lld %lld %lld %lld\n", p0, p1, p1 + 1, p0 + 1);
-            }
-    } else if (ctype == 3) {
-        for (int m = 0; m < el; ++m) std::fprintf(fp, "%lld %lld\n", node(0, 0, m), node(0, 0, m + 1));
     }
-    std::fprintf(fp, "    </DataArray>\n    <DataArray type=\"Int64\" Name=\"offsets\" format=\"ascii\">\n");
-    for (int64_t c = 1; c <= ncell; ++c) std::fprintf(fp, "%lld\n", (long long)(c * cn));
-    std::fprintf(fp, "    </DataArray>\n    <DataArray type=\"UInt8\" Name=\"types\" format=\"ascii\">\n");
-    for (int64_t c = 0; c < ncell; ++c) std::fprintf(fp, "%d\n", ctype);
-    std::fprintf(fp, "    </DataArray>\n   </Cells>\n  </Piece>\n </UnstructuredGrid>\n</VTKFile>\n");
+    std::fputs("\t\t\t\t </DataArray>\n\t\t\t </Points>\n\t\t\t <Cells>\n", fp);
+    std::fputs("\t\t\t\t <DataArray type=\"Int64\" Name=\"connectivity\" format=\"ascii\">\n", fp);
+    auto tail = [&](long long nce, int per, int type) {
+        std::fputs("\t\t\t\t </DataArray>\n\t\t\t\t <DataArray type=\"Int64\" Name=\"offsets\" format=\"ascii\">\n",
+                   fp);
+        for (long long c = 1; c <= nce; ++c) std::fprintf(fp, "%lld \n", c * per);
+        std::fputs("\t\t\t\t </DataArray>\n\t\t\t\t <DataArray type=\"UInt8\" Name=\"types\" format=\"ascii\">\n", fp);
+        for (long long c = 1; c <= nce; ++c) std::fprintf(fp, "%d \n", type);
+    };
+    if (nx > 1 && ny > 1 && nl > 1) {
+        for (int i = 0; i < nx - 1; ++i)
+            for (int j = 0; j < ny - 1; ++j)
+                for (int m = 0; m < nl - 1; ++m) {
+                    const long long a = (long long)nl * ny * i + (long long)nl * j + m;
+                    const long long b = a + (long long)nl * ny, c = b + nl, e = a + nl;
+                    std::fprintf(fp, "%lld %lld %lld %lld %lld %lld %lld %lld\n", a, b, c, e, a + 1, b + 1, c + 1, e + 1);
+                }
+        tail((long long)(nx - 1) * (ny - 1) * (nl - 1), 8, 12);
+    } else if (nx > 1 && ny == 1 && nl > 1) {
+        for (int i = 0; i < nx - 1; ++i)
+            for (int m = 0; m < nl - 1; ++m) {
+                const long long a = (long long)nl * i + m, b = (long long)nl * (i + 1) + m;
+                std::fprintf(fp, "%lld %lld %lld %lld\n", a, b, b + 1, a + 1);
+            }
+        tail((long long)(nx - 1) * (nl - 1), 4, 9);
+    } else if (nx == 1 && ny > 1 && nl > 1) {
+        for (int j = 0; j < ny - 1; ++j)
+            for (int m = 0; m < nl - 1; ++m) {
+                const long long a = (long long)nl * j + m, b = (long long)nl * (j + 1) + m;
+                std::fprintf(fp, "%lld %lld %lld %lld\n", a, b, b + 1, a + 1);
+            }
+        tail((long long)(ny - 1) * (nl - 1), 4, 9);
+    } else if (nx > 1 && ny > 1 && nl == 1) {
+        for (int i = 0; i < nx - 1; ++i)
+            for (int j = 0; j < ny - 1; ++j) {
+                const long long a = (long long)ny * i + j, b = (long long)ny * (i + 1) + j;
+                std::fprintf(fp, "%lld %lld %lld %lld\n", a, b, b + 1, a + 1);
+            }
+        tail((long long)(nx - 1) * (ny - 1), 4, 9);
+    }
+    std::fputs("\t\t\t\t </DataArray>\n\t\t\t </Cells>\n\t\t </Piece>\n\t </UnstructuredGrid>\n</VTKFile>\n", fp);
     const bool ok = std::ferror(fp) == 0;
     std::fclose(fp);
     if (!ok) { set_error(std::string("write error on ") + path); return -1; }
+    return 0;
+}
+
+// lsGridSpline::write_vtk (lsgrid_spline.cpp:1200-1221): directory (created if
+// missing) + prefix + the time step as %05d + ".vtu".
+int Grid::write_vtk_dump(bool last) {
+    if (!vtk.on) return 0;
+    if (!vtk.dir.empty()) {
+        struct stat sb;
+        if (stat(vtk.dir.c_str(), &sb) != 0 || !S_ISDIR(sb.st_mode)) {
+            std::cout << "WARNING: Specified directory for storage of VTK files does not exist. Directory will be "
+                         "created at the following path:  " << vtk.dir << std::endl;
+            if (mkdir(vtk.dir.c_str(), 0755) != 0 && errno != EEXIST) {
+                set_error("cannot create " + vtk.dir);
+                return -1;
+            }
+        }
+    }
+    char buf[32];
+    std::snprintf(buf, sizeof(buf), "%05d", tstep);
+    const std::string path = vtk.dir + vtk.prefix + buf + ".vtu";
+    std::cout << path << std::endl;
+    if (int e = write_vtu(path.c_str(), last, vtk.label.c_str())) return e;
+    std::cout << "wrote kinematics to: " << path << std::endl;
     return 0;
 }
 

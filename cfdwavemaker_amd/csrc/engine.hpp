@@ -363,6 +363,13 @@ public:
     int64_t export_field(int which, double* out);
     // VTK XML unstructured grid of one time slot (lsgrid_spline.cpp:1200-1403 role)
     int write_vtu(const char* path, bool last, const char* time_label);
+    // [lsgrid2vtk] dumps (lsgrid_spline.h:58-61): after every fill and update,
+    // and of the following slot at the end (last)
+    struct VtkDump {
+        bool on = false;
+        std::string dir = "./", prefix = "kinematics", label = "TimeValue";
+    } vtk;
+    int write_vtk_dump(bool last);
     int query(int64_t n, const double* t, const double* x, const double* y, const double* z,
               double* out4, const int* ramp_flags, const double* ramp_se);
     // SoA outputs (eta, u, v, w) or, with out4, {eta, u, v, w} per query;

@@ -260,7 +260,7 @@ std::string build_tables(const SeaInput& in, SeaTables& out, bool with_direct) {
 // Irregular::spectral_moments (Irregular.cpp:1336-1369): the loop stops at n-1.
 void spectral_moments(const SeaInput& in, double m[3]) {
     m[0] = m[1] = m[2] = 0.;
-    const int n = (int)in.omega.size();
+    const int n = std::min((int)in.omega.size(), in.nfreq > 0 ? in.nfreq : (int)in.omega.size());
     for (int i = 0; i < n - 1; ++i) {
         m[0] += in.A[i] * in.A[i] / 2.;
         m[1] += in.omega[i] * in.A[i] * in.A[i] / 2.;

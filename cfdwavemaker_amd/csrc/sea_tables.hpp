@@ -23,6 +23,10 @@ struct SeaInput {
     double tofmax = 0.0, x0 = 0.0, y0 = 0.0;
     double bandwidth = 100.0, cutoff = 100000.0;
     int order = 2;
+    // frequency rows of the input file (Irregular::nfreq): with ndir > 0 the
+    // components are nfreq x ndir, and the spectral moments sum only the first
+    // nfreq - 1 of them (Irregular.cpp:1356-1360); 0: the component count
+    int nfreq = 0;
 };
 
 // Per-pair record.  Branch "P" is the sum interaction (phi_i + phi_m), "M"

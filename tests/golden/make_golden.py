@@ -130,7 +130,10 @@ def make_grid(R):
     np.savez_compressed(os.path.join(HERE, "grid.npz"), **out)
 
 
-def run_abi(lib, waveinput_text, prog, extra_files=()):
+def run_abi(lib, waveinput_text, prog, files=False):
+    """Run a call program in a process of its own; with files, also return the
+    text of every file the library wrote into the run directory
+    (spectral_components.dat, .vtu dumps, including those written at exit)."""
     with tempfile.TemporaryDirectory() as wd:
         with open(os.path.join(wd, "waveinput.dat"), "w") as f:
             f.write(waveinput_text)
@@ -140,7 +143,35 @@ def run_abi(lib, waveinput_text, prog, extra_files=()):
         subprocess.run([sys.executable, os.path.join(HERE, "ref_abi.py"), lib, wd, pp, op],
                        check=True, stdout=subprocess.DEVNULL)
         r = np.load(op)
-        return int(r["rc"]), r["out"]
+        if not files:
+            return int(r["rc"]), r["out"]
+        written = {}
+        for root, _, names in os.walk(wd):
+            for nm in names:
+                if nm in ("waveinput.dat", "prog.npz", "out.npz"):
+                    continue
+                p = os.path.join(root, nm)
+                written[os.path.relpath(p, wd)] = open(p).read()
+        return int(r["rc"]), r["out"], written
+
+
+def file_fields(written):
+    """npz entries for the files a run wrote: file__<name> = text."""
+    return {"file__" + k: np.str_(v) for k, v in written.items()}
+
+
+def waveinput_spreading(sea_rows, dirs, depth, extra_general=(), irregular_keys=(), lsgrid=None, tail=""):
+    """waveinput.dat in the reference's ndir > 0 form (src/CFDwavemaker.cpp:
+    729-797): nfreq rows (omega, A, k, phase), then ndir rows (theta, D)."""
+    L = ["@v310", "[wave type]", "irregular", "", "[general input data]", f"depth {depth!r}", "mtheta 0.0"]
+    L += list(extra_general) + ["", "[wave input irregular]"] + list(irregular_keys)
+    L += [f"nfreq {len(sea_rows)}", f"ndir {len(dirs)}"]
+    L += [" ".join(repr(float(v)) for v in r) for r in sea_rows]
+    L += [" ".join(repr(float(v)) for v in d) for d in dirs]
+    L.append("")
+    if lsgrid:
+        L += lsgrid
+    return "\n".join(L) + "\n" + tail
 
 
 def prog_direct(seed, n=60, depth=30.0):
@@ -224,8 +255,96 @@ def make_abi():
                         targetwave=tw[sel], **prog)
 
 
+def make_abi_inputs():
+    """Input-format and output-file coverage of the waveinput.dat loader and
+    the s-grid dumps, each a call program with the reference's answers and the
+    text of the files the reference wrote (spectral_components.dat, .vtu):
+      abi_t008     the reference's tests/t008_3D_irregular_sea/waveinput_linear.dat
+                   (312 components, bandwidth auto, s-grid with ignore_at_init);
+      abi_example  the reference's examples/python/linux/waveinput.dat (ndir 1
+                   spreading form, normalize, amplify, ramps; direct mode);
+      abi_ndir     a 10 x 5 frequency x direction spreading input (direct mode);
+      abi_iramp    individual_ramp rows (direct mode, start times on both sides of 0);
+      abi_vtk      an s-grid with [lsgrid2vtk]: dumps at init and after each
+                   update, the end-of-run dump of the following slot, and a
+                   storage directory that has to be created."""
+    from cfdwavemaker_amd.seastate import wavenumber
+    lib = os.path.join(REF_DIR, "libCFDwavemaker_ref.so")
+    # t008, without its ignore_subdomain / ignore_at_init lines: the reference
+    # allocates the s-grid arrays uninitialised (lsgrid_spline.cpp:35-61) and
+    # ignore_at_init leaves the ignored columns unwritten, so its queries there
+    # (and in every cell of this 5 x 5 grid, which all touch an ignored
+    # column) read whatever the heap held; the ignore paths are pinned by the
+    # harness fixtures (grid.npz), which zero the arrays first
+    text = open(os.path.join(REF_SRC, "tests", "t008_3D_irregular_sea", "waveinput_linear.dat")).read()
+    text = "\n".join(ln for ln in text.splitlines() if not ln.strip().startswith(("ignore_subdomain", "ignore_at_init")))
+    text += "\n"
+    lsg = dict(bounds=(-100.0, 100.0, -100.0, 100.0), t0=0.0, dt=0.5)
+    prog = prog_grid(31, lsg, 40.0, nsteps=2, nq=20)
+    rc, out, wr = run_abi(lib, text, prog, files=True)
+    np.savez_compressed(os.path.join(HERE, "abi_t008.npz"), waveinput=text, rc=rc, out=out, **prog,
+                        **file_fields(wr))
+    # python example input (tank scale: depth 1.2 m)
+    text = open(os.path.join(REF_SRC, "examples", "python", "linux", "waveinput.dat")).read()
+    rng = np.random.default_rng(32)
+    ops, args = [], []
+    for q in range(40):
+        x, y, z, t = rng.uniform(-2, 8), rng.uniform(-1, 1), rng.uniform(-1.3, 0.05), rng.uniform(-0.2, 20.0)
+        for o in (0, 1, 2, 3, 4):
+            ops.append(o)
+            args.append((x, y, z, t))
+    prog = dict(op=np.array(ops, np.int32), args=np.array(args, float))
+    rc, out, wr = run_abi(lib, text, prog, files=True)
+    np.savez_compressed(os.path.join(HERE, "abi_example.npz"), waveinput=text, rc=rc, out=out, **prog,
+                        **file_fields(wr))
+    # synthetic spreading input: 10 frequencies x 5 directions, unsorted rows
+    rng = np.random.default_rng(33)
+    w = rng.permutation(np.linspace(0.45, 1.6, 10))
+    k = wavenumber(w, 30.0)
+    A = 0.4 * np.exp(-((w - 0.8) / 0.3) ** 2)
+    ph = rng.uniform(0, 2 * np.pi, 10)
+    th = np.linspace(-0.9, 0.9, 5)
+    D = np.cos(th) ** 2
+    D /= np.sqrt(np.sum(D * D))
+    text = waveinput_spreading(np.stack([w, A, k, ph], 1), np.stack([th, D], 1), 30.0,
+                               extra_general=["swl 0.1", "gravity 9.81", "rho 1025.0"],
+                               irregular_keys=["bandwidth 0.7"])
+    prog = prog_direct(33, n=40, depth=30.0)
+    rc, out, wr = run_abi(lib, text, prog, files=True)
+    np.savez_compressed(os.path.join(HERE, "abi_ndir.npz"), waveinput=text, rc=rc, out=out, **prog,
+                        **file_fields(wr))
+    # individual ramps
+    sea = make_sea(18, 2.0, 8.0, 25.0, seed=34)
+    rng = np.random.default_rng(34)
+    sea.ramptime = rng.uniform(0.5, 4.0, 18)
+    sea.starttime = rng.uniform(-3.0, 6.0, 18)
+    sea.duration = rng.uniform(8.0, 30.0, 18)
+    with tempfile.NamedTemporaryFile("r", suffix=".dat") as tf:
+        write_waveinput(tf.name, sea, individual_ramp=True)
+        text = open(tf.name).read()
+    prog = prog_direct(34, n=40, depth=25.0)
+    rc, out, wr = run_abi(lib, text, prog, files=True)
+    np.savez_compressed(os.path.join(HERE, "abi_iramp.npz"), waveinput=text, rc=rc, out=out, **prog,
+                        **file_fields(wr))
+    # [lsgrid2vtk]
+    sea = make_sea(8, 3.0, 9.0, 30.0, seed=35, ndir=3)
+    lsg = dict(bounds=(0.0, 100.0, -20.0, 20.0), nx=6, ny=4, nl=5, t0=0.0, dt=0.5, tan_a=0.7, tan_b=1.5)
+    with tempfile.NamedTemporaryFile("r", suffix=".dat") as tf:
+        write_waveinput(tf.name, sea, lsgrid=lsg, extrap=2)
+        text = open(tf.name).read()
+    text += "[lsgrid2vtk]\nstorage_path ./vtkout/\nfilename waves_\nvtk_timelabel Time\n"
+    prog = prog_grid(35, lsg, sea.depth, nsteps=2, nq=10)
+    rc, out, wr = run_abi(lib, text, prog, files=True)
+    np.savez_compressed(os.path.join(HERE, "abi_vtk.npz"), waveinput=text, rc=rc, out=out, **prog,
+                        **file_fields(wr))
+
+
 def main():
     subprocess.run(["make", "-s", "-C", os.path.join(ROOT, "oracle"), "ref"], check=True)
+    if sys.argv[1:] == ["inputs"]:
+        make_abi_inputs()
+        print("input-format fixtures written to", HERE)
+        return
     R = RefHarness()
     make_points(R)
     make_linear(R)
@@ -233,6 +352,7 @@ def main():
     make_smit(R)
     make_grid(R)
     make_abi()
+    make_abi_inputs()
     print("golden fixtures written to", HERE)
 
 

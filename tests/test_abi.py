@@ -36,7 +36,10 @@ def test_library_exports_every_declared_symbol():
     assert not missing, missing
 
 
-def run_program(fixture):
+def run_program(fixture, files=False):
+    """Run a fixture's call program through libcfdwm in a process of its own;
+    with files, also the text of every file the run wrote (including files
+    written at process exit)."""
     g = golden(fixture)
     with tempfile.TemporaryDirectory() as wd:
         with open(os.path.join(wd, "waveinput.dat"), "w") as f:
@@ -48,7 +51,15 @@ def run_program(fixture):
                            capture_output=True, text=True, timeout=600)
         assert r.returncode == 0, r.stderr[-2000:]
         o = np.load(op)
-        return g, int(o["rc"]), o["out"]
+        if not files:
+            return g, int(o["rc"]), o["out"]
+        written = {}
+        for root, _, names in os.walk(wd):
+            for nm in names:
+                if nm not in ("waveinput.dat", "prog.npz", "out.npz"):
+                    p = os.path.join(root, nm)
+                    written[os.path.relpath(p, wd)] = open(p).read()
+        return g, int(o["rc"]), o["out"], written
 
 
 @pytest.mark.gpu
