@@ -162,10 +162,10 @@ def cpu_baseline(spec: str, threads: int, what: str):
 # PMC calibration of the committed profiles: a 256 MiB device copy reads
 # 131,088 KiB by FETCH_SIZE and a 256 MiB fill writes 262,144 KiB by WRITE_SIZE
 FETCH_SCALE, WRITE_SCALE = 2.0, 1.0
-PROFILE_BASE = os.path.join(ROOT, "profiles", "sgrid_fill_query_N1024_256x256x16x4_fp32_")
+PROFILE_BASE = os.path.join(ROOT, "profiles", "sgrid_fill_query_N1024_256x256x16x4_")
 
 
-def profiled_traffic(kernel: str):
+def profiled_traffic(kernel: str, precision: str = "bf16"):
     """HBM-side bytes per launch of the largest launch of `kernel` from the
     committed FETCH_SIZE / WRITE_SIZE summaries (rocprofv3 --pmc, KiB per
     dispatch, calibrated as above); None if they do not list it."""
@@ -173,7 +173,7 @@ def profiled_traffic(kernel: str):
     for counter, scale in (("fetch_size", FETCH_SCALE), ("write_size", WRITE_SCALE)):
         best = None
         try:
-            with open(PROFILE_BASE + counter + ".md") as f:
+            with open(PROFILE_BASE + precision + "_" + counter + ".md") as f:
                 for line in f:
                     c = [x.strip() for x in line.strip().strip("|").split("|")]
                     if len(c) == 6 and kernel in c[0] and c[2] == counter.upper():
@@ -677,28 +677,47 @@ def main():
     q_ms += prof.get("k_query_sort", (0.0, 1))[0]
     qfi = grid.qform_info()
     if qfi is not None:
-        # dominant kernel: the second-order pair sums as bf16 MFMA quadratic forms
-        f_ms, f_n = prof.get("k_qf_forms", (float("nan"), 1))
-        per_fill = max(f_n // max(a.steps, 1), 1)
-        avg = f_ms / max(f_n, 1) * 1e-3
-        # algorithmic work of the surface forms: 5 quadratic forms x 2 (2N)^2
-        # = 40 N^2 FLOP per column-slot, over the 4 x 256 x 256 column-slots of
-        # one launch; the kernel executes more: every form of the depth basis
-        # (1 + 3 R of them), nlv bf16 products per term and the pair triangle
+        # dominant kernel: the second-order velocity pair sums as bf16 MFMA
+        # quadratic forms (the launch k_qf_forms times; with one bf16 product
+        # per term, k_qf_forms_eta times the separate three-product eta2 launch).
+        # Algorithmic work: 2 (2N)^2 = 8 N^2 FLOP per column-slot and surface
+        # form, over the 4 x 256 x 256 column-slots of a fill: u, v, w (24 N^2)
+        # for the velocity launch, eta2 + u, v, w (32 N^2) for the forms step
+        # (the 40 N^2 of five forms counts a pressure form the s-grid does not
+        # keep).  The kernel executes more: every form of the depth basis
+        # (3 R velocity forms), nlv bf16 products per term and the pair triangle
         # in tiles (exec_flop_per_launch)
         ncomp = len(s["omega"])
-        alg_launch = 40.0 * ncomp * ncomp * 4 * NX * NY / per_fill
-        flop_launch = qfi["mfma_flop"] / per_fill
-        roof = {"bound": "mfma", "kernel": "k_qf_forms", "achieved": alg_launch / avg / 1e12,
+        cs = 4 * NX * NY
+        f_ms, f_n = prof.get("k_qf_forms", (float("nan"), 1))
+        e_ms, e_n = prof.get("k_qf_forms_eta", (0.0, 0))
+        per_fill = max(f_n // max(a.steps, 1), 1)
+        avg = f_ms / max(f_n, 1) * 1e-3
+        fill_exec = qfi["mfma_flop"] / per_fill
+        split = qfi["products"] == 1
+        R = qfi["rank"]
+        alg_launch = (24.0 if split else 32.0) * ncomp * ncomp * cs / per_fill
+        flop_launch = fill_exec * R / (R + 1.0) if split else fill_exec
+        roof = {"bound": "mfma", "kernel": "k_qf_forms_il<1> (velocity forms)" if split else "k_qf_forms_il<3>",
+                "achieved": alg_launch / avg / 1e12,
                 "peak": PEAK_MFMA_BF16 / 1e12, "unit": "TFLOP/s", "frac": alg_launch / avg / PEAK_MFMA_BF16,
-                "traffic": profiled_traffic("k_qf_forms"),
+                "traffic": profiled_traffic("k_qf_forms_il<1>" if split else "k_qf_forms_il<3>", a.precision),
                 "traffic_source": "profiles/*_fetch_size.md + *_write_size.md (rocprofv3 --pmc, calibrated)",
                 "avg_launch_ms": avg * 1e3, "flop_per_launch": alg_launch,
-                "flop_rule": "40 N^2 per column-slot (5 forms x 2 (2N)^2)",
+                "flop_rule": ("24 N^2 per column-slot (u, v, w forms x 2 (2N)^2)" if split
+                              else "32 N^2 per column-slot (eta2, u, v, w forms x 2 (2N)^2)"),
                 "exec_flop_per_launch": flop_launch, "exec_achieved": flop_launch / avg / 1e12,
                 "exec_frac": flop_launch / avg / PEAK_MFMA_BF16, "exec_over_alg": flop_launch / alg_launch,
-                "forms": qfi["nform"], "bf16_products": qfi["products"], "depth_basis_rank": qfi["rank"],
+                "forms": qfi["nform"], "bf16_products": qfi["products"], "depth_basis_rank": R,
                 "depth_basis_err": qfi["basis_err"], "depth_basis_tol": qfi["basis_tol"]}
+        if split and e_n:
+            e_avg = e_ms / e_n * 1e-3
+            step_alg = 32.0 * ncomp * ncomp * cs
+            roof["forms_step"] = {"kernels": "k_qf_forms_il<3> (eta2) + k_qf_forms_il<1> (velocities)",
+                                  "ms": (avg + e_avg) * 1e3, "eta2_launch_ms": e_avg * 1e3,
+                                  "flop": step_alg, "achieved": step_alg / (avg + e_avg) / 1e12,
+                                  "frac": step_alg / (avg + e_avg) / PEAK_MFMA_BF16,
+                                  "exec_flop": fill_exec, "exec_frac": fill_exec / (avg + e_avg) / PEAK_MFMA_BF16}
         # shader clock during the forms launch, from one more fill (untimed)
         # with the in-kernel clock measurement on; the dense peak assumes the
         # peak engine clock, and MFMA power with these operands lowers it

@@ -872,14 +872,10 @@ int Grid::qform_run() {
         Q.ppad = (Q.P + 127) / 128 * 128;
         PROF("k_qf_zgen", launch_qf_zgen(Q, st));
         if (qf_split_) {
-            // eta2 at three products and the velocity forms at one: the two
-            // launches are timed as one forms step
+            // eta2 at three products, then the velocity forms at one
             V.p0 = Q.p0; V.P = Q.P; V.ppad = Q.ppad;
-            Engine::get().prof_begin();
-            hipError_t e1 = launch_qf_forms(Q, st);
-            hipError_t e2 = e1 == hipSuccess ? launch_qf_forms(V, st) : e1;
-            Engine::get().prof_end("k_qf_forms");
-            CFDWM_HIP(e2);
+            PROF("k_qf_forms_eta", launch_qf_forms(Q, st));
+            PROF("k_qf_forms", launch_qf_forms(V, st));
             qf_flop += (qf_fpp_ * Q.nform + qf_fpp1_ * V.nform) * (double)Q.ppad;
         } else {
             PROF("k_qf_forms", launch_qf_forms(Q, st));
