@@ -38,6 +38,9 @@ BATCH_API = [
     "cfdwm_grids_refill_batched", "cfdwm_grid_release_workspace", "wave_PrefetchPoints", "wave_PrefetchClear",
     "wave_PrefetchStats", "cfdwm_eval_fields", "cfdwm_grid_write_vtu", "cfdwm_sea_set_vertical_theory",
     "cfdwm_grid_qform_clock", "cfdwm_grid_qform_mode", "cfdwm_sea_clone", "cfdwm_grid_kinematics_device_f32",
+    "cfdwm_lsgrid_create", "cfdwm_lsgrid_destroy", "cfdwm_lsgrid_check_time", "cfdwm_lsgrid_update",
+    "cfdwm_lsgrid_state", "cfdwm_lsgrid_export", "cfdwm_lsgrid_query", "cfdwm_lsgrid_query_device",
+    "cfdwm_lsgrid_write_vtu", "cfdwm_lsgrid_set_vtk",
 ]
 
 
@@ -190,6 +193,26 @@ def load(path: str | None = None) -> C.CDLL:
     L.cfdwm_surrogate_add.argtypes = [_i64, C.c_void_p, C.c_void_p, C.c_void_p]
     L.cfdwm_grid_write_vtu.restype = C.c_int
     L.cfdwm_grid_write_vtu.argtypes = [C.c_void_p, C.c_char_p, C.c_int, C.c_char_p]
+    L.cfdwm_lsgrid_create.restype = C.c_int
+    L.cfdwm_lsgrid_create.argtypes = [C.POINTER(C.c_void_p), C.c_void_p, C.POINTER(GridDescC), C.c_int]
+    L.cfdwm_lsgrid_destroy.restype = C.c_int
+    L.cfdwm_lsgrid_destroy.argtypes = [C.c_void_p]
+    L.cfdwm_lsgrid_check_time.restype = C.c_int
+    L.cfdwm_lsgrid_check_time.argtypes = [C.c_void_p, _d]
+    L.cfdwm_lsgrid_update.restype = C.c_int
+    L.cfdwm_lsgrid_update.argtypes = [C.c_void_p, _d, C.POINTER(C.c_int)]
+    L.cfdwm_lsgrid_state.restype = C.c_int
+    L.cfdwm_lsgrid_state.argtypes = [C.c_void_p, _dp, C.POINTER(C.c_int)]
+    L.cfdwm_lsgrid_export.restype = _i64
+    L.cfdwm_lsgrid_export.argtypes = [C.c_void_p, C.c_int, _dp]
+    L.cfdwm_lsgrid_query.restype = C.c_int
+    L.cfdwm_lsgrid_query.argtypes = [C.c_void_p, _i64] + [_dp] * 5
+    L.cfdwm_lsgrid_query_device.restype = C.c_int
+    L.cfdwm_lsgrid_query_device.argtypes = [C.c_void_p, _i64] + [C.c_void_p] * 5
+    L.cfdwm_lsgrid_write_vtu.restype = C.c_int
+    L.cfdwm_lsgrid_write_vtu.argtypes = [C.c_void_p, C.c_char_p, C.c_int, C.c_char_p]
+    L.cfdwm_lsgrid_set_vtk.restype = C.c_int
+    L.cfdwm_lsgrid_set_vtk.argtypes = [C.c_void_p, C.c_int, C.c_char_p, C.c_char_p, C.c_char_p]
     L.cfdwm_eval_fields.restype = C.c_int
     L.cfdwm_eval_fields.argtypes = [C.c_void_p, _i64] + [C.c_void_p] * 10 + [C.c_int]
     L.cfdwm_grid_release_workspace.restype = C.c_int

@@ -31,6 +31,9 @@ struct cfdwm_sea {
 struct cfdwm_grid {
     std::unique_ptr<cfdwm::Grid> g;
 };
+struct cfdwm_lsgrid {
+    std::unique_ptr<cfdwm::LsGrid> g;
+};
 
 namespace cfdwm {
 namespace {
@@ -1229,6 +1232,76 @@ int cfdwm_grid_qform_clock(cfdwm_grid* grid, int enable, double* mhz) {
 int cfdwm_grid_write_vtu(cfdwm_grid* grid, const char* path, int last, const char* time_label) {
     if (!grid || !path) { set_error("bad arguments"); return -1; }
     return grid->g->write_vtu(path, last != 0, time_label);
+}
+int cfdwm_lsgrid_create(cfdwm_lsgrid** out, cfdwm_sea* sea, const cfdwm_grid_desc* d, int init_only) {
+    if (!out || !sea || !d) { set_error("bad arguments"); return -1; }
+    GridDesc gd{};
+    std::memcpy(gd.domain, d->domain, sizeof(gd.domain));
+    gd.nx = d->nx; gd.ny = d->ny; gd.nl = d->nl; gd.t0 = d->t0; gd.dt = d->dt;
+    gd.tan_a = d->tan_a; gd.tan_b = d->tan_b; gd.water_depth = d->water_depth; gd.swl = d->swl;
+    std::memcpy(gd.domain_ignore, d->domain_ignore, sizeof(gd.domain_ignore));
+    gd.ignore_at_init = d->ignore_at_init;
+    auto cg = std::make_unique<cfdwm_lsgrid>();
+    cg->g = std::make_unique<LsGrid>(sea->s.get(), gd, init_only != 0);
+    if (int rc = cg->g->init()) return rc;
+    *out = cg.release();
+    return 0;
+}
+int cfdwm_lsgrid_destroy(cfdwm_lsgrid* g) {
+    if (!g) return 0;
+    int rc = 0;
+    if (g->g->vtk.on) rc = g->g->write_vtk_dump(true);     // ~lsGrid (lsgrid.h:25-30)
+    delete g;
+    return rc;
+}
+int cfdwm_lsgrid_check_time(cfdwm_lsgrid* g, double t) {
+    if (!g) { set_error("bad arguments"); return -1; }
+    return g->g->check_time(t) ? 1 : 0;
+}
+int cfdwm_lsgrid_update(cfdwm_lsgrid* g, double t_target, int* reinit) {
+    if (!g) { set_error("bad arguments"); return -1; }
+    return g->g->update(t_target, reinit);
+}
+int cfdwm_lsgrid_state(cfdwm_lsgrid* g, double* t0, int* update_count) {
+    if (!g) { set_error("bad arguments"); return -1; }
+    if (t0) *t0 = g->g->t0;
+    if (update_count) *update_count = g->g->update_count;
+    return 0;
+}
+int64_t cfdwm_lsgrid_export(cfdwm_lsgrid* g, int which, double* out) {
+    if (!g) { set_error("bad arguments"); return -1; }
+    return g->g->export_field(which, out);
+}
+int cfdwm_lsgrid_query(cfdwm_lsgrid* g, int64_t n, const double* t, const double* x, const double* y,
+                       const double* z, double* out4) {
+    if (!g || n < 0 || (n > 0 && (!t || !x || !y || !z || !out4))) { set_error("bad arguments"); return -1; }
+    return g->g->query(n, t, x, y, z, out4, false);
+}
+int cfdwm_lsgrid_query_device(cfdwm_lsgrid* g, int64_t n, const double* t, const double* x, const double* y,
+                              const double* z, double* out4) {
+    if (!g || n < 0 || (n > 0 && (!t || !x || !y || !z || !out4))) { set_error("bad arguments"); return -1; }
+    int rc = g->g->query(n, t, x, y, z, out4, true);
+    if (rc) return rc;
+    CFDWM_HIP(hipStreamSynchronize(Engine::get().stream()));
+    return 0;
+}
+int cfdwm_lsgrid_write_vtu(cfdwm_lsgrid* g, const char* path, int last, const char* time_label) {
+    if (!g || !path) { set_error("bad arguments"); return -1; }
+    return g->g->write_vtu(path, last != 0, time_label);
+}
+int cfdwm_lsgrid_set_vtk(cfdwm_lsgrid* g, int on, const char* dir, const char* prefix, const char* time_label) {
+    if (!g) { set_error("bad arguments"); return -1; }
+    const bool was = g->g->vtk.on;
+    g->g->vtk.on = on != 0;
+    if (dir) g->g->vtk.dir = dir;
+    if (prefix) g->g->vtk.prefix = prefix;
+    if (time_label) g->g->vtk.label = time_label;
+    if (on && !was) {
+        // the dump the fill writes when dump_vtk is set before it (lsgrid.cpp:507-510)
+        if (int rc = g->g->write_vtk_dump(false)) return rc;
+        ++g->g->update_count;
+    }
+    return 0;
 }
 int cfdwm_grid_release_workspace(cfdwm_grid* grid) {
     if (!grid) return -1;

@@ -116,14 +116,6 @@ std::map<std::string, KernelTimes> Engine::prof_read(bool reset) {
     return r;
 }
 
-#define PROF(name, call)                    \
-    do {                                    \
-        Engine::get().prof_begin();         \
-        hipError_t pe_ = (call);            \
-        Engine::get().prof_end(name);       \
-        CFDWM_HIP(pe_);                     \
-    } while (0)
-
 // ------------------------------------------------------------------ Sea
 namespace {
 constexpr double kLog2e = 1.4426950408889634074;
@@ -618,6 +610,7 @@ Grid::~Grid() = default;
 
 int64_t Grid::field_count(int which) const {
     const int64_t c = (int64_t)g.nx * g.ny, v = c * g.nl;
+    if (legacy) return which == 0 ? 2 * c : (which >= 4 && which <= 6 ? 2 * v : 0);
     switch (which) {
     case 0: case 3: return 4 * c;
     case 1: case 2: return 2 * c;
@@ -664,6 +657,7 @@ int Grid::init(bool fill) {
     const size_t esz = prec == kFP32 ? 4 : 8;
     for (int f = 0; f < 19; ++f) {
         const size_t b = (size_t)field_count(f) * esz;
+        if (b == 0) continue;
         if (f_[f].alloc(b)) return -3;
         CFDWM_HIP(hipMemsetAsync(f_[f].get(), 0, b, st));
     }
@@ -679,6 +673,15 @@ int Grid::init(bool fill) {
     if (const char* e = std::getenv("CFDWM_PRUNE")) prune_ = std::atoi(e) != 0;
     if (const char* e = std::getenv("CFDWM_QFORM")) qform_ = std::atoi(e) != 0;
     return fill ? refill() : 0;
+}
+
+int Grid::fill_at(int nslot, const double* slot_t, const int* slot_dst, bool use_ignore) {
+    const int maxslot = legacy ? 2 : 4;
+    if (nslot < 1 || nslot > maxslot) { set_error("fill_at: 1 to " + std::to_string(maxslot) + " slots"); return -1; }
+    for (int s = 0; s < nslot; ++s)
+        if (slot_dst[s] < 0 || slot_dst[s] >= maxslot) { set_error("fill_at: slot out of range"); return -1; }
+    return prec == kFP32 ? fill_slots<float>(nslot, slot_t, slot_dst, use_ignore)
+                         : fill_slots<double>(nslot, slot_t, slot_dst, use_ignore);
 }
 
 template <typename R>
@@ -1305,6 +1308,7 @@ int Grid::pack() {
 }
 
 int Grid::refill() {
+    if (legacy) { set_error("a legacy two-slot grid fills through LsGrid"); return -1; }
     tstep = 0;
     t0 = d.t0;
     // initialize_kinematics (:678) / initialize_kinematics_with_ignore (:767)
@@ -1381,11 +1385,16 @@ int64_t Grid::export_field(int which, double* out) {
 // max(n - 1, 1)), and the time as field data.
 int Grid::write_vtu(const char* path, bool last, const char* time_label) {
     if (!filled) { set_error("the s-grid has not been filled yet (created with defer_fill)"); return -1; }
+    const int tt = last ? (tstep + 2) % 4 : (tstep + 1) % 4;
+    return write_vtu_slot(path, tt, last ? t0 + d.dt : t0, time_label, false);
+}
+
+int Grid::write_vtu_slot(const char* path, int tt, double tv, const char* time_label, bool swl_free) {
     const int nx = g.nx, ny = g.ny, nl = g.nl;
     const int64_t nc = (int64_t)nx * ny, nv = nc * nl;
-    const int tt = last ? (tstep + 2) % 4 : (tstep + 1) % 4;
-    const double tv = last ? t0 + d.dt : t0;
-    std::vector<double> eta(4 * nc), U(4 * nv), V(4 * nv), W(4 * nv);
+    const int ns = legacy ? 2 : 4;
+    if (tt < 0 || tt >= ns) { set_error("write_vtu: slot out of range"); return -1; }
+    std::vector<double> eta(ns * nc), U(ns * nv), V(ns * nv), W(ns * nv);
     if (export_field(0, eta.data()) < 0 || export_field(4, U.data()) < 0 || export_field(5, V.data()) < 0 ||
         export_field(6, W.data()) < 0) return -3;
     FILE* fp = std::fopen(path, "w");
@@ -1412,7 +1421,10 @@ int Grid::write_vtu(const char* path, bool last, const char* time_label) {
             const double yp = g.y0 + g.dy * j;
             const double e = eta[(int64_t)tt * nc + (int64_t)i * ny + j];
             for (int m = 0; m < nl; ++m) {
-                const double z = (e - g.swl) + sigma_[m] * (g.h + (e - g.swl));      // s2z (lsgrid_spline.cpp:96-98)
+                // s2z (lsgrid_spline.cpp:96-98); the legacy writer passes eta
+                // with its swl (lsgrid.cpp:1420-1424, 87-89)
+                const double z = swl_free ? e + sigma_[m] * (g.h + e)
+                                          : (e - g.swl) + sigma_[m] * (g.h + (e - g.swl));
                 std::fprintf(fp, "%12.4f %12.4f %12.4f\n", xp, yp, z);
             }
         }

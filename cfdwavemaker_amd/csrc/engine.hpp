@@ -247,6 +247,15 @@ private:
     hipEvent_t take_event();
 };
 
+// A launch timed with an event pair when timing is on; errors return -3.
+#define PROF(name, call)                    \
+    do {                                    \
+        Engine::get().prof_begin();         \
+        hipError_t pe_ = (call);            \
+        Engine::get().prof_end(name);       \
+        CFDWM_HIP(pe_);                     \
+    } while (0)
+
 enum Precision { kFP64 = 0, kFP32 = 1 };
 
 // A sea state resident on the device.
@@ -356,6 +365,9 @@ class Grid {
 public:
     Grid(Sea* sea, const GridDesc& d);
     ~Grid();
+    // storage of the legacy two-slot grid (LsGrid): ETA and UX/UY/UZ in two
+    // slots, no derivative fields; set before init
+    bool legacy = false;
     int init(bool fill = true);              // allocate (+ initialize_kinematics)
     bool filled = false;                     // a fill has run (queries need the packed slots)
     int refill();                            // initialize_kinematics again (tstep back to 0)
@@ -363,6 +375,9 @@ public:
     int64_t export_field(int which, double* out);
     // VTK XML unstructured grid of one time slot (lsgrid_spline.cpp:1200-1403 role)
     int write_vtu(const char* path, bool last, const char* time_label);
+    // the writer of one slot at time label tv; swl_free: node z = eta +
+    // sigma (h + eta) as the legacy grid's export_vtu (lsgrid.cpp:1404-1428)
+    int write_vtu_slot(const char* path, int slot, double tv, const char* time_label, bool swl_free);
     // [lsgrid2vtk] dumps (lsgrid_spline.h:58-61): after every fill and update,
     // and of the following slot at the end (last)
     struct VtkDump {
@@ -401,6 +416,14 @@ public:
     // shader clock of the forms launches of later fills (diagnostic)
     void qform_clock_enable(bool on) { qf_clock_on_ = on; }
     int qform_clock(double* mhz);
+    // the column body (eta1 + eta2 + swl, Taylor gradients, levels) of nslot
+    // time levels into the given slots, all columns or (use_ignore) those
+    // outside the ignore region; no gradients, no query records
+    int fill_at(int nslot, const double* slot_t, const int* slot_dst, bool use_ignore);
+    void* field(int f) const { return f_[f].get(); }
+    const int32_t* active_cols() const { return (const int32_t*)cols_act_d_.get(); }
+    int64_t active_count() const { return nact_; }
+    int precision() const { return prec; }
 private:
     int prec;
     std::vector<int32_t> ignore_;
@@ -460,6 +483,46 @@ private:
                   TQ* eta, TQ* u, TQ* v, TQ* w, const double* ramp, TQ* aos = nullptr,
                   void* ws = nullptr, size_t ws_bytes = 0);
     int64_t field_count(int which) const;
+};
+
+// The legacy linear s-grid (lsGrid, src/lsgrid.h:15-114) resident on the
+// device: slot 0 at t0 and slot 1 at t0 + dt of ETA (incl. swl) and UX/UY/UZ,
+// filled by the same column body as the spline grid, queried by
+// tri/bilinear interpolation with the swl-free sigma maps.  With init_only
+// the two slots always hold the same values (the reference aliases them).
+class LsGrid {
+public:
+    LsGrid(Sea* sea, const GridDesc& d, bool init_only);
+    // initialize_surface_elevation(t0) + initialize_kinematics, the
+    // _with_ignore pair when d.ignore_at_init (lsgrid.cpp:419-747)
+    int init();
+    // CheckTime (lsgrid.cpp:378-386): true while t <= t0 + dt
+    bool check_time(double t) const { return !(t > t0 + d.dt); }
+    // update (lsgrid.cpp:749-827): *reinit = 1 when t_target is more than a
+    // step ahead (every column re-initialised at dt floor(t_target / dt)),
+    // else one step forward for the columns outside the ignore region
+    int update(double t_target, int* reinit);
+    // eta (bilinear_interpolation2, :322-357) and u, v, w
+    // (trilinear_interpolation2, :192-287) -> out4 = {eta, u, v, w} per point;
+    // device: the five arrays are device pointers
+    int query(int64_t n, const double* t, const double* x, const double* y, const double* z, double* out4,
+              bool device);
+    // which: 0 ETA, 4..6 UX/UY/UZ; slot 0 then slot 1 (2 x cells or nodes)
+    int64_t export_field(int which, double* out) { return g->export_field(which, out); }
+    // export_vtu (lsgrid.cpp:1354-1552): slot 0 at t0, or slot 1 at t0 + dt (last)
+    int write_vtu(const char* path, bool last, const char* time_label);
+    // write_vtk (lsgrid.cpp:1329-1351): after every fill and step while on
+    Grid::VtkDump vtk;
+    int write_vtk_dump(bool last);
+    double t0 = 0;
+    int update_count = 0;
+    bool init_only;
+    GridDesc d;
+    std::unique_ptr<Grid> g;
+private:
+    int fill_both(bool use_ignore);
+    int copy_slot(int src, int dst, bool active_only);
+    DevBuf qin_, qout_;
 };
 
 // Refill many s-grids (sea states) with their second-order forms evaluated

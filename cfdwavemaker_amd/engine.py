@@ -108,6 +108,12 @@ class Sea:
         return Grid(self, domain, nx, ny, nl, t0, dt, tan_a, tan_b, water_depth, swl, domain_ignore,
                     ignore_at_init, defer_fill)
 
+    def lsgrid(self, domain, nx, ny, nl, t0, dt, tan_a, tan_b, water_depth, swl=0.0,
+               domain_ignore=(1e12, -1e12, 1e12, -1e12), ignore_at_init=0, init_only=False) -> "LsGrid":
+        """The legacy two-slot grid (lsGrid): filled at t0 and t0 + dt."""
+        return LsGrid(self, domain, nx, ny, nl, t0, dt, tan_a, tan_b, water_depth, swl, domain_ignore,
+                      ignore_at_init, init_only)
+
     def clone(self, precision: int) -> "Sea":
         """The same sea at another precision (host coefficient tables copied)."""
         h = C.c_void_p()
@@ -221,6 +227,114 @@ class Grid:
         """Device pointers (e.g. torch tensors' data_ptr()), enqueued on the library stream."""
         check(self.L.cfdwm_grid_query_device_f32(self.h, int(n), float(t), x_ptr, y_ptr, z_ptr,
                                                  eta_ptr, u_ptr, v_ptr, w_ptr), "cfdwm_grid_query_device_f32")
+
+
+def _grid_desc(domain, nx, ny, nl, t0, dt, tan_a, tan_b, water_depth, swl, domain_ignore, ignore_at_init,
+               defer_fill=False):
+    d = GridDescC()
+    for q in range(4):
+        d.domain[q] = float(domain[q])
+        d.domain_ignore[q] = float(domain_ignore[q])
+    d.nx, d.ny, d.nl = int(nx), int(ny), int(nl)
+    d.t0, d.dt, d.tan_a, d.tan_b = float(t0), float(dt), float(tan_a), float(tan_b)
+    d.water_depth, d.swl, d.ignore_at_init = float(water_depth), float(swl), int(ignore_at_init)
+    d.defer_fill = int(bool(defer_fill))
+    return d
+
+
+class LsGrid:
+    """The legacy two-slot s-grid (lsGrid, src/lsgrid.h:15-114) on the GPU.
+
+    Method names follow the reference class: eta(t, x, y) and u/v/w(t, x, y, z)
+    (lsgrid.cpp:360-376), CheckTime (:378-386), update (:749-827),
+    export_vtu (:1354-1552); query() evaluates all four at once.  Slot 0
+    holds t0, slot 1 t0 + dt; export() gives slot 0 then slot 1.
+    """
+
+    def __init__(self, sea: Sea, domain, nx, ny, nl, t0, dt, tan_a, tan_b, water_depth, swl=0.0,
+                 domain_ignore=(1e12, -1e12, 1e12, -1e12), ignore_at_init=0, init_only=False):
+        self.sea, self.L = sea, sea.L
+        d = _grid_desc(domain, nx, ny, nl, t0, dt, tan_a, tan_b, water_depth, swl, domain_ignore, ignore_at_init)
+        h = C.c_void_p()
+        check(self.L.cfdwm_lsgrid_create(C.byref(h), sea.h, C.byref(d), int(bool(init_only))),
+              "cfdwm_lsgrid_create")
+        self.h = h
+        self.nx, self.ny, self.nl = nx, ny, nl
+
+    def close(self):
+        if getattr(self, "h", None):
+            rc = self.L.cfdwm_lsgrid_destroy(self.h)
+            self.h = None
+            check(rc, "cfdwm_lsgrid_destroy")
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def CheckTime(self, t) -> bool:
+        r = self.L.cfdwm_lsgrid_check_time(self.h, float(t))
+        check(min(r, 0), "cfdwm_lsgrid_check_time")
+        return r == 1
+
+    def update(self, t_target) -> int:
+        """1 when the grid was re-initialised (a jump of more than one step), 0 after one step."""
+        r = C.c_int()
+        check(self.L.cfdwm_lsgrid_update(self.h, float(t_target), C.byref(r)), "cfdwm_lsgrid_update")
+        return r.value
+
+    def state(self):
+        t0, uc = C.c_double(), C.c_int()
+        check(self.L.cfdwm_lsgrid_state(self.h, C.byref(t0), C.byref(uc)), "cfdwm_lsgrid_state")
+        return dict(t0=t0.value, update_count=uc.value)
+
+    @property
+    def t0(self):
+        return self.state()["t0"]
+
+    def export(self, which):
+        w = GRID_FIELDS.index(which) if isinstance(which, str) else int(which)
+        n = self.L.cfdwm_lsgrid_export(self.h, w, None)
+        if n < 0:
+            check(int(n), "cfdwm_lsgrid_export")
+        out = np.zeros(n)
+        got = self.L.cfdwm_lsgrid_export(self.h, w, _p(out))
+        if got != n:
+            raise RuntimeError("cfdwm_lsgrid_export failed")
+        return out
+
+    def query(self, t, x, y, z):
+        t, x, y, z = (_f64(np.ravel(a)) for a in np.broadcast_arrays(t, x, y, z))
+        out = np.zeros((t.size, 4))
+        check(self.L.cfdwm_lsgrid_query(self.h, t.size, _p(t), _p(x), _p(y), _p(z), _p(out)),
+              "cfdwm_lsgrid_query")
+        return out
+
+    def query_device(self, n, t_ptr, x_ptr, y_ptr, z_ptr, out4_ptr):
+        """Device pointers of doubles; out4 receives {eta, u, v, w} per point."""
+        check(self.L.cfdwm_lsgrid_query_device(self.h, int(n), t_ptr, x_ptr, y_ptr, z_ptr, out4_ptr),
+              "cfdwm_lsgrid_query_device")
+
+    def eta(self, t, x, y):
+        return self.query(t, x, y, 0.0)[:, 0]
+
+    def u(self, t, x, y, z):
+        return self.query(t, x, y, z)[:, 1]
+
+    def v(self, t, x, y, z):
+        return self.query(t, x, y, z)[:, 2]
+
+    def w(self, t, x, y, z):
+        return self.query(t, x, y, z)[:, 3]
+
+    def write_vtu(self, path, last=False, time_label="TimeValue"):
+        check(self.L.cfdwm_lsgrid_write_vtu(self.h, str(path).encode(), int(bool(last)), time_label.encode()),
+              "cfdwm_lsgrid_write_vtu")
+
+    def set_vtk(self, on=True, directory="./", prefix="kinematics", time_label="TimeValue"):
+        check(self.L.cfdwm_lsgrid_set_vtk(self.h, int(bool(on)), directory.encode(), prefix.encode(),
+                                          time_label.encode()), "cfdwm_lsgrid_set_vtk")
 
 
 def sync():

@@ -145,8 +145,11 @@ typedef struct {
     int normalize;    /* Irregular::normalize_data: A = ampl A / sum(A), phases 0 */
     int bw_auto;      /* Irregular::bandwidth_estimator */
     int precision;    /* 0 FP64, 1 FP32 (FP64 phase reduction), 2 FP32 whose MFMA
-                         quadratic forms take one bf16 product per term instead
-                         of three (elevation error ~3e-4 Hs at N=1024) */
+                         velocity forms take one bf16 product per term instead
+                         of three on a coarser depth basis (tolerance 1e-4;
+                         velocity errors ~3e-4 to 4e-3 of the field scale at
+                         N=1024); the elevation form keeps three products
+                         (error ~1e-6 Hs) */
     int with_direct;  /* also build the unfiltered table of u2/v2/w2/dp2 */
 } cfdwm_sea_params;
 
@@ -287,6 +290,57 @@ typedef struct {
  * time as field data named time_label (NULL: "TimeValue"), velocity vectors at
  * the nodes, node positions from the sigma map, hexahedral cells. */
 CFDWM_EXPORT int cfdwm_grid_write_vtu(cfdwm_grid* grid, const char* path, int last, const char* time_label);
+/* Legacy two-slot s-grid (lsGrid, src/lsgrid.h:15-114, src/lsgrid.cpp; the
+ * reference compiles it only into its unit tests, unittests/lsgrid_tests.cpp).
+ * Slot 0 holds t0, slot 1 t0 + dt: ETA (with swl) and UX/UY/UZ on the grid of
+ * desc, filled by the same column body as cfdwm_grid (the sea's precision and
+ * forms), queried by tri/bilinear interpolation with the swl-free sigma maps
+ * (z2s / s2z, lsgrid.cpp:82-89).  desc.defer_fill is ignored. */
+typedef struct cfdwm_lsgrid cfdwm_lsgrid;
+/* allocate (lsgrid.cpp:23-56, every array zero), set_ignore (:1554-1573),
+ * initialize_surface_elevation(t0) (:622-680) and initialize_kinematics
+ * (:419-516); with desc.ignore_at_init the _with_ignore pair (:682-747,
+ * :518-620: ignored columns keep eta = swl and zero velocities).  init_only:
+ * slot 1 is slot 0 (the reference aliases the arrays, :466-470, :665-667), so
+ * both always hold the same values. */
+CFDWM_EXPORT int cfdwm_lsgrid_create(cfdwm_lsgrid** out, cfdwm_sea* sea, const cfdwm_grid_desc* desc,
+                                     int init_only);
+/* Frees the grid; with VTK dumps on, first writes the last dump (slot 1 at
+ * t0 + dt) as the reference's destructor does (lsgrid.h:25-30). */
+CFDWM_EXPORT int cfdwm_lsgrid_destroy(cfdwm_lsgrid* grid);
+/* CheckTime (lsgrid.cpp:378-386): 1 while t <= t0 + dt, 0 when an update is due. */
+CFDWM_EXPORT int cfdwm_lsgrid_check_time(cfdwm_lsgrid* grid, double t);
+/* update (lsgrid.cpp:749-827): when t_target / dt - (t0 + 2 dt) / dt > 0,
+ * every column is re-initialised at dt floor(t_target / dt) (*reinit = 1);
+ * otherwise t0 += dt, and for the columns outside the ignore region slot 1
+ * moves to slot 0 and slot 1 is filled at the new t0 + dt (*reinit = 0). */
+CFDWM_EXPORT int cfdwm_lsgrid_update(cfdwm_lsgrid* grid, double t_target, int* reinit);
+CFDWM_EXPORT int cfdwm_lsgrid_state(cfdwm_lsgrid* grid, double* t0, int* update_count);
+/* which: 0 ETA, 4 UX, 5 UY, 6 UZ; slot 0 then slot 1 ([2][nx][ny] or
+ * [2][nx][ny][nl]).  Returns the element count (out may be NULL). */
+CFDWM_EXPORT int64_t cfdwm_lsgrid_export(cfdwm_lsgrid* grid, int which, double* out);
+/* lsGrid::eta (bilinear_interpolation2, lsgrid.cpp:322-357) and u, v, w
+ * (trilinear_interpolation2, :192-287) for n host points -> out4[4n] =
+ * {eta, u, v, w} per point.  Fractions of a cell through float as the
+ * reference's std::modf(double, float*); coordinates outside the grid clamp
+ * to its edge cells, t outside [t0, t0 + dt] to the nearer slot. */
+CFDWM_EXPORT int cfdwm_lsgrid_query(cfdwm_lsgrid* grid, int64_t n, const double* t, const double* x,
+                                    const double* y, const double* z, double* out4);
+/* The same with all six arrays in device memory (synchronises before returning). */
+CFDWM_EXPORT int cfdwm_lsgrid_query_device(cfdwm_lsgrid* grid, int64_t n, const double* t, const double* x,
+                                           const double* y, const double* z, double* out4);
+/* export_vtu (lsgrid.cpp:1354-1552): slot 0 at t0 (last != 0: slot 1 at
+ * t0 + dt), node z = eta + s (h + eta) with eta including swl as the
+ * reference writes it. */
+CFDWM_EXPORT int cfdwm_lsgrid_write_vtu(cfdwm_lsgrid* grid, const char* path, int last, const char* time_label);
+/* dump_vtk (lsgrid.h:43-46): on != 0 writes dir + prefix + %05d + ".vtu"
+ * after every fill and update (write_vtk, lsgrid.cpp:1329-1351, counter
+ * update_count) and the last dump at destroy; NULL strings keep "./",
+ * "kinematics", "TimeValue".  Switching dumps on writes the current state at
+ * once, as the fill does when dump_vtk is set before it (:507-510), so
+ * switching on right after cfdwm_lsgrid_create gives the reference's files. */
+CFDWM_EXPORT int cfdwm_lsgrid_set_vtk(cfdwm_lsgrid* grid, int on, const char* dir, const char* prefix,
+                                      const char* time_label);
 /* Free the sea's form tables, the grid's operand workspace (rebuilt on the
  * next FP32 fill) and its query-ordering buffers (rebuilt by the next batch of
  * 65,536 or more queries); the grid's fields are kept. */

@@ -118,6 +118,22 @@ class Oracle:
         L.or_sigmaS2cart.argtypes = [C.c_double] * 6
         L.or_cart2sigmaS.restype = C.c_double
         L.or_cart2sigmaS.argtypes = [C.c_double] * 6
+        L.or_lsgrid_create.restype = C.c_void_p
+        L.or_lsgrid_create.argtypes = [C.c_void_p, _dp, C.c_int, C.c_int, C.c_int, C.c_double,
+                                       C.c_double, C.c_double, C.c_double, C.c_double, C.c_double,
+                                       _dp, C.c_int, C.c_int]
+        L.or_lsgrid_destroy.argtypes = [C.c_void_p]
+        L.or_lsgrid_from_fields.restype = C.c_void_p
+        L.or_lsgrid_from_fields.argtypes = [_dp, C.c_int, C.c_int, C.c_int] + [C.c_double] * 6 + [_dp] * 4
+        L.or_lsgrid_check_time.restype = C.c_int
+        L.or_lsgrid_check_time.argtypes = [C.c_void_p, C.c_double]
+        L.or_lsgrid_update.restype = C.c_int
+        L.or_lsgrid_update.argtypes = [C.c_void_p, C.c_double]
+        L.or_lsgrid_t0.restype = C.c_double
+        L.or_lsgrid_t0.argtypes = [C.c_void_p]
+        L.or_lsgrid_export.restype = C.c_long
+        L.or_lsgrid_export.argtypes = [C.c_void_p, C.c_int, _dp]
+        L.or_lsgrid_query.argtypes = [C.c_void_p, C.c_long, _dp, _dp, _dp, _dp, _dp]
         L.or_column_fill.restype = C.c_double
         L.or_column_fill.argtypes = [C.c_void_p, C.c_long, _dp, _dp, C.c_double, C.c_int,
                                      C.c_double, C.c_double, C.c_double, C.c_double,
@@ -138,6 +154,14 @@ class Oracle:
 
     def sigmaS2cart(self, s, eta, depth, swl, a, b):
         return self.lib.or_sigmaS2cart(s, eta, depth, swl, a, b)
+
+    def lsgrid_from_fields(self, domain, nx, ny, nl, t0, dt, tan_a, tan_b, water_depth, swl, eta, u, v, w):
+        """Legacy-grid interpolation over given slots (eta [2 nx ny], u/v/w [2 nx ny nl])."""
+        d = _arr(np.asarray(domain, float))
+        keep = [_arr(np.ravel(a)) for a in (eta, u, v, w)]
+        h = self.lib.or_lsgrid_from_fields(_ptr(d), nx, ny, nl, t0, dt, tan_a, tan_b, water_depth, swl,
+                                           *[_ptr(a) for a in keep])
+        return OracleLsGrid(self, h)
 
     def cart2sigmaS(self, z, eta, depth, swl, a, b):
         return self.lib.or_cart2sigmaS(z, eta, depth, swl, a, b)
@@ -205,6 +229,16 @@ class OracleSea:
                                       water_depth, swl, _ptr(di), int(ignore_at_init))
         return OracleGrid(self, h, nx, ny, nl)
 
+    def lsgrid(self, domain, nx, ny, nl, t0, dt, tan_a, tan_b, water_depth, swl=0.0,
+               domain_ignore=(1e12, -1e12, 1e12, -1e12), ignore_at_init=0, init_only=0):
+        """Legacy two-slot grid (lsGrid): initialize_surface_elevation(t0) +
+        initialize_kinematics[_with_ignore]."""
+        d = _arr(np.asarray(domain, float))
+        di = _arr(np.asarray(domain_ignore, float))
+        h = self.o.lib.or_lsgrid_create(self.h, _ptr(d), nx, ny, nl, t0, dt, tan_a, tan_b,
+                                        water_depth, swl, _ptr(di), int(ignore_at_init), int(init_only))
+        return OracleLsGrid(self, h)
+
     def column_fill(self, x, y, t, nl, tan_a, tan_b, water_depth, swl=0.0):
         x = _arr(x)
         y = _arr(y)
@@ -250,6 +284,44 @@ class OracleGrid:
         return out
 
 
+class OracleLsGrid:
+    """Legacy two-slot grid of the oracle; export(which) gives slot 0 then slot 1
+    (which: 0 ETA, 4..6 UX/UY/UZ), query gives rows {eta, u, v, w}."""
+
+    def __init__(self, sea, handle):
+        self.sea, self.h = sea, handle
+        self.lib = sea.lib if isinstance(sea, Oracle) else sea.o.lib
+
+    def __del__(self):
+        try:
+            self.lib.or_lsgrid_destroy(self.h)
+        except Exception:
+            pass
+
+    def check_time(self, t):
+        return bool(self.lib.or_lsgrid_check_time(self.h, t))
+
+    def update(self, t_target):
+        return self.lib.or_lsgrid_update(self.h, t_target)
+
+    @property
+    def t0(self):
+        return self.lib.or_lsgrid_t0(self.h)
+
+    def export(self, which):
+        w = GRID_FIELDS.index(which) if isinstance(which, str) else which
+        n = self.lib.or_lsgrid_export(self.h, w, None)
+        out = np.zeros(n)
+        self.lib.or_lsgrid_export(self.h, w, _ptr(out))
+        return out
+
+    def query(self, t, x, y, z):
+        t, x, y, z = (_arr(np.ravel(a)) for a in np.broadcast_arrays(t, x, y, z))
+        out = np.zeros((t.size, 4))
+        self.lib.or_lsgrid_query(self.h, t.size, _ptr(t), _ptr(x), _ptr(y), _ptr(z), _ptr(out))
+        return out
+
+
 class RefHarness:
     """The compiled reference classes (oracle/_ref/libcfdwm_refh.so).
 
@@ -285,6 +357,19 @@ class RefHarness:
         L.refh_grid_export.restype = C.c_long
         L.refh_grid_export.argtypes = [C.c_int, _dp]
         L.refh_grid_query.argtypes = [C.c_long, _dp, _dp, _dp, _dp, _dp]
+        L.refh_lsgrid_init.restype = C.c_double
+        L.refh_lsgrid_init.argtypes = [_dp, C.c_int, C.c_int, C.c_int, C.c_double, C.c_double,
+                                       C.c_double, C.c_double, C.c_double, C.c_double, _dp,
+                                       C.c_int, C.c_int]
+        L.refh_lsgrid_update.argtypes = [C.c_double]
+        L.refh_lsgrid_check_time.restype = C.c_int
+        L.refh_lsgrid_check_time.argtypes = [C.c_double]
+        L.refh_lsgrid_t0.restype = C.c_double
+        L.refh_lsgrid_export.restype = C.c_long
+        L.refh_lsgrid_export.argtypes = [C.c_int, _dp]
+        L.refh_lsgrid_query.argtypes = [C.c_long, _dp, _dp, _dp, _dp, _dp]
+        L.refh_lsgrid_write_vtu.restype = C.c_int
+        L.refh_lsgrid_write_vtu.argtypes = [C.c_char_p, C.c_int]
         L.refh_sigmaS2cart.restype = C.c_double
         L.refh_sigmaS2cart.argtypes = [C.c_double] * 3
         L.refh_cart2sigmaS.restype = C.c_double
@@ -348,6 +433,39 @@ class RefHarness:
 
     def grid_force_update(self, t):
         return self.lib.refh_grid_force_update(t)
+
+    # -- legacy two-slot grid (lsGrid) ---------------------------------------
+    def lsgrid_init(self, domain, nx, ny, nl, t0, dt, tan_a, tan_b, water_depth, swl=0.0,
+                    domain_ignore=(1e12, -1e12, 1e12, -1e12), ignore_at_init=0, init_only=0):
+        self._ld = _arr(np.asarray(domain, float))
+        self._ldi = _arr(np.asarray(domain_ignore, float))
+        return self.lib.refh_lsgrid_init(_ptr(self._ld), nx, ny, nl, t0, dt, tan_a, tan_b,
+                                         water_depth, swl, _ptr(self._ldi), int(ignore_at_init), int(init_only))
+
+    def lsgrid_update(self, t_target):
+        self.lib.refh_lsgrid_update(t_target)
+
+    def lsgrid_check_time(self, t):
+        return bool(self.lib.refh_lsgrid_check_time(t))
+
+    def lsgrid_t0(self):
+        return self.lib.refh_lsgrid_t0()
+
+    def lsgrid_export(self, which):
+        w = GRID_FIELDS.index(which) if isinstance(which, str) else which
+        n = self.lib.refh_lsgrid_export(w, None)
+        out = np.zeros(n)
+        self.lib.refh_lsgrid_export(w, _ptr(out))
+        return out
+
+    def lsgrid_query(self, t, x, y, z):
+        t, x, y, z = (_arr(np.ravel(a)) for a in np.broadcast_arrays(t, x, y, z))
+        out = np.zeros((t.size, 4))
+        self.lib.refh_lsgrid_query(t.size, _ptr(t), _ptr(x), _ptr(y), _ptr(z), _ptr(out))
+        return out
+
+    def lsgrid_write_vtu(self, path, last=False):
+        return self.lib.refh_lsgrid_write_vtu(str(path).encode(), int(last))
 
     def grid_state(self):
         ts = C.c_int()

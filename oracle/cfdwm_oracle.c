@@ -1102,3 +1102,260 @@ double or_column_fill(const or_sea* s, long ncol, const double* x, const double*
                     &eta[c], &ux[c * nl], &uy[c * nl], &uz[c * nl]);
     return omp_get_wtime() - t_start;
 }
+
+/* ============================================================ legacy lsGrid */
+/* Two time slots (slot 0 at t0, slot 1 at t0 + dt), filled by the same column
+ * body as the spline grid, and queried by tri/bilinear interpolation
+ * (src/lsgrid.h:15-114, src/lsgrid.cpp).  With init_only, slot 1 is the same
+ * storage as slot 0 (lsgrid.cpp:466-470, 665-667: the arrays alias). */
+struct or_lsgrid {
+    const or_sea* sea;
+    int nx, ny, nl, init_only;
+    double dom[4], domi[4], t0, dt, a, b, depth, swl, dx, dy, ds;
+    int* ign;
+    double *ETA0, *ETA1, *F0[3], *F1[3];
+};
+
+/* lsGrid::s2tan / tan2s / z2s / s2z (lsgrid.cpp:82-101): the swl-free maps */
+static double ls_s2tan(const or_lsgrid* g, double s) { return -pow(tan(-s * g->a), g->b) / pow(tan(g->a), g->b); }
+static double ls_tan2s(const or_lsgrid* g, double t) { return -atan(pow(-t * pow(tan(g->a), g->b), 1. / g->b)) / g->a; }
+static double ls_z2s(double z, double we, double d) { return (z - we) / (d + we); }
+static double ls_s2z(double s, double we, double d) { return we + s * (d + we); }
+
+/* the level loop of initialize_kinematics (lsgrid.cpp:445-459) for one column
+ * at time t whose surface slot holds eta (incl. swl) */
+static void ls_levels(const or_lsgrid* g, double t, double x, double y, double eta, double* ux, double* uy, double* uz) {
+    const or_sea* s = g->sea;
+    const double et = eta - g->swl;
+    const double gx = phi1_dxdz(s, t, x, y), gy = phi1_dydz(s, t, x, y), gz = phi1_dzdz(s, t, x, y);
+    for (int m = 0; m < g->nl; ++m) {
+        const double spt = ls_s2tan(g, -1. + g->ds * m);
+        const double z = ls_s2z(spt, et, g->depth);
+        const double zmax = dmax(0., z - g->swl);
+        const double zmin = dmin(0., z);
+        double o4[4];
+        uvw2(s, t, x, y, zmin, o4);
+        ux[m] = u1(s, t, x, y, zmin) + o4[0] + gx * zmax;
+        uy[m] = v1(s, t, x, y, zmin) + o4[1] + gy * zmax;
+        uz[m] = w1(s, t, x, y, zmin) + o4[2] + gz * zmax;
+    }
+}
+
+/* initialize_surface_elevation[_with_ignore] (lsgrid.cpp:622-747) */
+static void ls_surface(or_lsgrid* g, double t_target, int use_ignore) {
+    g->t0 = t_target;
+    #pragma omp parallel for collapse(2)
+    for (int i = 0; i < g->nx; ++i)
+        for (int j = 0; j < g->ny; ++j) {
+            const long c = (long)i * g->ny + j;
+            const double x = g->dom[0] + g->dx * i, y = g->dom[2] + g->dy * j;
+            if (use_ignore && g->ign[c]) { g->ETA0[c] = g->swl; continue; }
+            g->ETA0[c] = eta1(g->sea, g->t0, x, y) + eta2(g->sea, g->t0, x, y) + g->swl;
+        }
+    if (g->init_only) return;
+    #pragma omp parallel for collapse(2)
+    for (int i = 0; i < g->nx; ++i)
+        for (int j = 0; j < g->ny; ++j) {
+            const long c = (long)i * g->ny + j;
+            const double x = g->dom[0] + g->dx * i, y = g->dom[2] + g->dy * j;
+            if (use_ignore && g->ign[c]) { g->ETA1[c] = g->swl; continue; }
+            g->ETA1[c] = eta1(g->sea, g->t0 + g->dt, x, y) + eta2(g->sea, g->t0 + g->dt, x, y) + g->swl;
+        }
+}
+
+/* initialize_kinematics[_with_ignore] (lsgrid.cpp:419-620) */
+static void ls_kinematics(or_lsgrid* g, int use_ignore) {
+    const int nl = g->nl;
+    for (int slot = 0; slot < (g->init_only ? 1 : 2); ++slot) {
+        const double t = slot ? g->t0 + g->dt : g->t0;
+        double* E = slot ? g->ETA1 : g->ETA0;
+        double** F = slot ? g->F1 : g->F0;
+        #pragma omp parallel for schedule(dynamic, 1) collapse(2)
+        for (int i = 0; i < g->nx; ++i)
+            for (int j = 0; j < g->ny; ++j) {
+                const long c = (long)i * g->ny + j;
+                if (use_ignore && g->ign[c]) continue;
+                const double x = g->dom[0] + g->dx * i, y = g->dom[2] + g->dy * j;
+                ls_levels(g, t, x, y, E[c], &F[0][c * nl], &F[1][c * nl], &F[2][c * nl]);
+            }
+    }
+}
+
+or_lsgrid* or_lsgrid_create(const or_sea* s, const double* domain, int nx, int ny, int nl,
+                            double t0, double dt, double tan_a, double tan_b,
+                            double water_depth, double swl, const double* domain_ignore,
+                            int ignore_at_init, int init_only) {
+    or_lsgrid* g = (or_lsgrid*)calloc(1, sizeof(or_lsgrid));
+    g->sea = s; g->nx = nx; g->ny = ny; g->nl = nl; g->dt = dt; g->init_only = init_only != 0;
+    g->a = tan_a; g->b = tan_b; g->depth = water_depth; g->swl = swl;
+    for (int q = 0; q < 4; ++q) { g->dom[q] = domain[q]; g->domi[q] = domain_ignore[q]; }
+    const long c = (long)nx * ny, v = c * nl;
+    /* allocate (lsgrid.cpp:23-56): every array zeroed */
+    g->ETA0 = (double*)calloc(c, sizeof(double));
+    g->ETA1 = g->init_only ? g->ETA0 : (double*)calloc(c, sizeof(double));
+    for (int f = 0; f < 3; ++f) {
+        g->F0[f] = (double*)calloc(v, sizeof(double));
+        g->F1[f] = g->init_only ? g->F0[f] : (double*)calloc(v, sizeof(double));
+    }
+    g->ign = (int*)calloc(c, sizeof(int));
+    /* set_ignore (lsgrid.cpp:1554-1573) */
+    g->dx = (g->dom[1] - g->dom[0]) / dmax(1., (double)(nx - 1));
+    g->dy = (g->dom[3] - g->dom[2]) / dmax(1., (double)(ny - 1));
+    for (int i = 0; i < nx; ++i) {
+        const double xp = g->dom[0] + g->dx * i;
+        for (int j = 0; j < ny; ++j) {
+            const double yp = g->dom[2] + g->dy * j;
+            g->ign[i * ny + j] = (xp >= g->domi[0] && xp <= g->domi[1] && yp >= g->domi[2] && yp <= g->domi[3]) ? 1 : 0;
+        }
+    }
+    g->ds = 1. / dmax(1., (double)(nl - 1));
+    ls_surface(g, t0, ignore_at_init);
+    ls_kinematics(g, ignore_at_init);
+    return g;
+}
+
+/* A grid holding given slot data (no sea): for checking the interpolation on
+ * fields filled elsewhere.  eta [2][nx][ny], u/v/w [2][nx][ny][nl]. */
+or_lsgrid* or_lsgrid_from_fields(const double* domain, int nx, int ny, int nl, double t0, double dt,
+                                 double tan_a, double tan_b, double water_depth, double swl,
+                                 const double* eta, const double* u, const double* v, const double* w) {
+    or_lsgrid* g = (or_lsgrid*)calloc(1, sizeof(or_lsgrid));
+    g->nx = nx; g->ny = ny; g->nl = nl; g->t0 = t0; g->dt = dt;
+    g->a = tan_a; g->b = tan_b; g->depth = water_depth; g->swl = swl;
+    for (int q = 0; q < 4; ++q) g->dom[q] = domain[q];
+    g->dx = (g->dom[1] - g->dom[0]) / dmax(1., (double)(nx - 1));
+    g->dy = (g->dom[3] - g->dom[2]) / dmax(1., (double)(ny - 1));
+    g->ds = 1. / dmax(1., (double)(nl - 1));
+    const long c = (long)nx * ny, vv = c * nl;
+    g->ETA0 = (double*)malloc(c * sizeof(double)); g->ETA1 = (double*)malloc(c * sizeof(double));
+    memcpy(g->ETA0, eta, c * sizeof(double)); memcpy(g->ETA1, eta + c, c * sizeof(double));
+    const double* src[3] = {u, v, w};
+    for (int f = 0; f < 3; ++f) {
+        g->F0[f] = (double*)malloc(vv * sizeof(double)); g->F1[f] = (double*)malloc(vv * sizeof(double));
+        memcpy(g->F0[f], src[f], vv * sizeof(double)); memcpy(g->F1[f], src[f] + vv, vv * sizeof(double));
+    }
+    g->ign = (int*)calloc(c, sizeof(int));
+    return g;
+}
+
+void or_lsgrid_destroy(or_lsgrid* g) {
+    if (!g) return;
+    if (!g->init_only) { free(g->ETA1); for (int f = 0; f < 3; ++f) free(g->F1[f]); }
+    free(g->ETA0); for (int f = 0; f < 3; ++f) free(g->F0[f]);
+    free(g->ign); free(g);
+}
+
+/* lsGrid::CheckTime (lsgrid.cpp:378-386): 1 while tpt <= t0 + dt */
+int or_lsgrid_check_time(const or_lsgrid* g, double t) { return t > g->t0 + g->dt ? 0 : 1; }
+
+/* lsGrid::update (lsgrid.cpp:749-827): a jump of more than one step
+ * re-initialises every column at dt * floor(t_target / dt) (returns 1);
+ * otherwise one step forward, slot 1 -> slot 0 and a new slot 1, for the
+ * columns outside the ignore region (returns 0) */
+int or_lsgrid_update(or_lsgrid* g, double t_target) {
+    const double dt = g->dt;
+    if ((t_target / dt - (g->t0 + 2 * dt) / dt) > 0.) {
+        ls_surface(g, dt * floor(t_target / dt), 0);
+        ls_kinematics(g, 0);
+        return 1;
+    }
+    g->t0 += dt;
+    const int nl = g->nl;
+    #pragma omp parallel for schedule(dynamic, 1) collapse(2)
+    for (int j = 0; j < g->ny; ++j)
+        for (int i = 0; i < g->nx; ++i) {
+            const long c = (long)i * g->ny + j;
+            if (g->ign[c]) continue;
+            const double x = g->dom[0] + g->dx * i, y = g->dom[2] + g->dy * j;
+            g->ETA0[c] = g->ETA1[c];
+            g->ETA1[c] = eta1(g->sea, g->t0 + dt, x, y) + eta2(g->sea, g->t0 + dt, x, y) + g->swl;
+            for (int f = 0; f < 3; ++f)
+                for (int m = 0; m < nl; ++m) g->F0[f][c * nl + m] = g->F1[f][c * nl + m];
+            ls_levels(g, g->t0 + dt, x, y, g->ETA1[c], &g->F1[0][c * nl], &g->F1[1][c * nl], &g->F1[2][c * nl]);
+        }
+    return 0;
+}
+
+double or_lsgrid_t0(const or_lsgrid* g) { return g->t0; }
+
+/* which: 0 ETA, 4 UX, 5 UY, 6 UZ; slot 0 then slot 1 */
+long or_lsgrid_export(const or_lsgrid* g, int which, double* out) {
+    const long c = (long)g->nx * g->ny, v = c * g->nl;
+    const double *s0, *s1; long n;
+    if (which == 0) { s0 = g->ETA0; s1 = g->ETA1; n = c; }
+    else if (which >= 4 && which <= 6) { s0 = g->F0[which - 4]; s1 = g->F1[which - 4]; n = v; }
+    else return 0;
+    if (out) { memcpy(out, s0, n * sizeof(double)); memcpy(out + n, s1, n * sizeof(double)); }
+    return 2 * n;
+}
+
+/* clip (lsgrid.cpp:187-190): std::max(lower, std::min(n, upper)) on ints */
+static int ls_clipi(int n, int lo, int hi) { const int m = (hi < n) ? hi : n; return (lo < m) ? m : lo; }
+
+/* bilinear_interpolation2 (lsgrid.cpp:322-357) */
+static double ls_bilinear2(const or_lsgrid* g, const double* V0, const double* V1, double tpt, double xpt, double ypt) {
+    const int nx = g->nx, ny = g->ny;
+    float nxf, nyf;
+    const double xd = modff((float)dmax(0., dmin((xpt - g->dom[0]) / g->dx, nx - 1.)), &nxf);
+    const double yd = modff((float)dmax(0., dmin((ypt - g->dom[2]) / g->dy, ny - 1.)), &nyf);
+    const int i = (int)nxf, j = (int)nyf;
+    const int i1 = ls_clipi(i + 1, 0, nx - 1), j1 = ls_clipi(j + 1, 0, ny - 1);
+    const double C00 = V0[i * ny + j], C01 = V0[i * ny + j1], C10 = V0[i1 * ny + j], C11 = V0[i1 * ny + j1];
+    const double D00 = V1[i * ny + j], D01 = V1[i * ny + j1], D10 = V1[i1 * ny + j], D11 = V1[i1 * ny + j1];
+    const double td = dmin(1., dmax(0., (tpt - g->t0) / g->dt));
+    const double C0 = C00 * (1. - xd) + C10 * xd, C1 = C01 * (1. - xd) + C11 * xd;
+    const double D0 = D00 * (1. - xd) + D10 * xd, D1 = D01 * (1. - xd) + D11 * xd;
+    return (C0 * (1. - yd) + C1 * yd) * (1. - td) + (D0 * (1. - yd) + D1 * yd) * td;
+}
+
+/* trilinear_interpolation2 (lsgrid.cpp:192-287) */
+static double ls_trilinear2(const or_lsgrid* g, const double* V0, const double* V1, double tpt, double xpt,
+                            double ypt, double zpt) {
+    const long nx = g->nx, ny = g->ny, nl = g->nl;
+    float nxf, nyf;
+    const double xd = modff((float)dmax(0., dmin((xpt - g->dom[0]) / g->dx, nx - 1.)), &nxf);
+    const double yd = modff((float)dmax(0., dmin((ypt - g->dom[2]) / g->dy, ny - 1.)), &nyf);
+    const long i = (int)nxf, j = (int)nyf;
+    const long i1 = ls_clipi((int)i + 1, 0, (int)nx - 1), j1 = ls_clipi((int)j + 1, 0, (int)ny - 1);
+    const double C00 = g->ETA0[i * ny + j], C01 = g->ETA0[i * ny + j1];
+    const double C10 = g->ETA0[i1 * ny + j], C11 = g->ETA0[i1 * ny + j1];
+    const double D00 = g->ETA1[i * ny + j], D01 = g->ETA1[i * ny + j1];
+    const double D10 = g->ETA1[i1 * ny + j], D11 = g->ETA1[i1 * ny + j1];
+    const double td = dmin(1., dmax(0., (tpt - g->t0) / g->dt));
+    double C0 = C00 * (1. - xd) + C10 * xd, C1 = C01 * (1. - xd) + C11 * xd;
+    double D0 = D00 * (1. - xd) + D10 * xd, D1 = D01 * (1. - xd) + D11 * xd;
+    const double we0 = C0 * (1. - yd) + C1 * yd, we1 = D0 * (1. - yd) + D1 * yd;
+    const double swl = g->swl, d = g->depth;
+    const double spt0 = ls_tan2s(g, dmax(ls_z2s(dmin(zpt - swl, we0 - swl), we0 - swl, d), -1.));
+    const double spt1 = ls_tan2s(g, dmax(ls_z2s(dmin(zpt - swl, we1 - swl), we1 - swl, d), -1.));
+    float s0f, s1f;
+    const double sd0 = modff((float)((spt0 + 1.) / g->ds), &s0f);
+    const double sd1 = modff((float)((spt1 + 1.) / g->ds), &s1f);
+    const long l0 = (int)s0f, l1 = (int)s1f;
+    const long l0p = ls_clipi((int)l0 + 1, 0, (int)nl - 1), l1p = ls_clipi((int)l1 + 1, 0, (int)nl - 1);
+    const double C000 = V0[i * ny * nl + j * nl + l0], C001 = V0[i * ny * nl + j * nl + l0p];
+    const double C010 = V0[i * ny * nl + j1 * nl + l0], C011 = V0[i * ny * nl + j1 * nl + l0p];
+    const double C100 = V0[i1 * ny * nl + j * nl + l0], C101 = V0[i1 * ny * nl + j * nl + l0p];
+    const double C110 = V0[i1 * ny * nl + j1 * nl + l0], C111 = V0[i1 * ny * nl + j1 * nl + l0p];
+    const double D000 = V1[i * ny * nl + j * nl + l1], D001 = V1[i * ny * nl + j * nl + l1p];
+    const double D010 = V1[i * ny * nl + j1 * nl + l1], D011 = V1[i * ny * nl + j1 * nl + l1p];
+    const double D100 = V1[i1 * ny * nl + j * nl + l1], D101 = V1[i1 * ny * nl + j * nl + l1p];
+    const double D110 = V1[i1 * ny * nl + j1 * nl + l1], D111 = V1[i1 * ny * nl + j1 * nl + l1p];
+    const double c00 = C000 * (1. - xd) + C100 * xd, c01 = C001 * (1. - xd) + C101 * xd;
+    const double c10 = C010 * (1. - xd) + C110 * xd, c11 = C011 * (1. - xd) + C111 * xd;
+    const double d00 = D000 * (1. - xd) + D100 * xd, d01 = D001 * (1. - xd) + D101 * xd;
+    const double d10 = D010 * (1. - xd) + D110 * xd, d11 = D011 * (1. - xd) + D111 * xd;
+    C0 = c00 * (1. - yd) + c10 * yd; C1 = c01 * (1. - yd) + c11 * yd;
+    D0 = d00 * (1. - yd) + d10 * yd; D1 = d01 * (1. - yd) + d11 * yd;
+    return (C0 * (1. - sd0) + C1 * sd0) * (1. - td) + (D0 * (1. - sd1) + D1 * sd1) * td;
+}
+
+/* lsGrid::eta / u / v / w (lsgrid.cpp:360-376) -> out4 = {eta, u, v, w} */
+void or_lsgrid_query(const or_lsgrid* g, long n, const double* t, const double* x, const double* y,
+                     const double* z, double* out4) {
+    #pragma omp parallel for schedule(static)
+    for (long p = 0; p < n; ++p) {
+        out4[4 * p] = ls_bilinear2(g, g->ETA0, g->ETA1, t[p], x[p], y[p]);
+        for (int f = 0; f < 3; ++f) out4[4 * p + 1 + f] = ls_trilinear2(g, g->F0[f], g->F1[f], t[p], x[p], y[p], z[p]);
+    }
+}

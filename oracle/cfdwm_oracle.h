@@ -2,7 +2,7 @@
  *
  * TEST INFRASTRUCTURE ONLY.  This is an FP64 restatement of the reference
  * CFDwavemaker algorithm (src/Irregular.cpp, src/lsgrid_spline.cpp,
- * src/Utils.cpp), used by tests/, __graft_entry__.smoke() and bench.py's
+ * src/lsgrid.cpp, src/Utils.cpp), used by tests/, __graft_entry__.smoke() and bench.py's
  * cpu_baseline leg as the checker.  The product library (libcfdwm) never links,
  * loads or calls it.
  *
@@ -76,6 +76,30 @@ void or_grid_query(const or_grid* g, long n, const double* t, const double* x, c
                    const double* z, double* out4);
 double or_sigmaS2cart(double s, double eta, double depth, double swl, double tan_a, double tan_b);
 double or_cart2sigmaS(double z, double eta, double depth, double swl, double tan_a, double tan_b);
+
+/* Legacy two-slot grid (lsGrid, src/lsgrid.h:15-114): slot 0 at t0, slot 1 at
+ * t0 + dt, tri/bilinear queries.  Created as initialize_surface_elevation(t0)
+ * + initialize_kinematics (the _with_ignore pair when ignore_at_init); with
+ * init_only slot 1 aliases slot 0.  update returns 1 when it re-initialised
+ * (a jump of more than one step), 0 after one step forward.  export: which 0
+ * ETA, 4..6 UX/UY/UZ, slot 0 then slot 1.  query: out4 = {eta, u, v, w}. */
+typedef struct or_lsgrid or_lsgrid;
+or_lsgrid* or_lsgrid_create(const or_sea* s, const double* domain, int nx, int ny, int nl,
+                            double t0, double dt, double tan_a, double tan_b,
+                            double water_depth, double swl, const double* domain_ignore,
+                            int ignore_at_init, int init_only);
+/* A legacy grid holding given slots (eta [2][nx][ny], u/v/w [2][nx][ny][nl])
+ * and no sea: for queries on fields filled elsewhere (no update). */
+or_lsgrid* or_lsgrid_from_fields(const double* domain, int nx, int ny, int nl, double t0, double dt,
+                                 double tan_a, double tan_b, double water_depth, double swl,
+                                 const double* eta, const double* u, const double* v, const double* w);
+void or_lsgrid_destroy(or_lsgrid* g);
+int or_lsgrid_check_time(const or_lsgrid* g, double t);
+int or_lsgrid_update(or_lsgrid* g, double t_target);
+double or_lsgrid_t0(const or_lsgrid* g);
+long or_lsgrid_export(const or_lsgrid* g, int which, double* out);
+void or_lsgrid_query(const or_lsgrid* g, long n, const double* t, const double* x, const double* y,
+                     const double* z, double* out4);
 
 /* Fill only a subset of columns of one time level (used as the timed CPU
  * baseline "port" leg): evaluates ncol columns of the s-grid column body at

@@ -7,6 +7,7 @@
 // reference's OpenMP s-grid fill).  It drives:
 //   Irregular        (src/Irregular.h:47-193, src/Irregular.cpp)
 //   lsGridSpline     (src/lsgrid_spline.h:15-129, src/lsgrid_spline.cpp)
+//   lsGrid           (src/lsgrid.h:15-114, src/lsgrid.cpp; the legacy two-slot grid)
 //   Ramp             (src/Utils.h:37-56, src/Utils.cpp:26-68)
 // through a plain C ABI so Python (ctypes) can call it.
 #include <cstring>
@@ -14,12 +15,31 @@
 #include <chrono>
 #include "Irregular.h"
 #include "lsgrid_spline.h"
+#include "lsgrid.h"
 #include "Utils.h"
+#include <cstdio>
 
 static Irregular g_irr;     // static storage: zero-initialised like the reference's global
 static lsGridSpline g_grid;
 static Ramp g_ramp;
 static bool g_grid_alloc = false;
+static lsGrid g_lg;         // its destructor frees UX0 only (lsgrid.h:25-30)
+static bool g_lg_alloc = false;
+
+// Frees the legacy grid's arrays; with init_only the slot-1 pointers alias
+// slot 0 (lsgrid.cpp:466-470, 665-667) and the original slot-1 arrays leaked.
+static void lg_free() {
+    if (!g_lg_alloc) return;
+    const bool ea = g_lg.ETA1 == g_lg.ETA0, ua = g_lg.UX1 == g_lg.UX0;
+    delete[] g_lg.ETA0; if (!ea) delete[] g_lg.ETA1;
+    delete[] g_lg.UX0; delete[] g_lg.UY0; delete[] g_lg.UZ0;
+    if (!ua) { delete[] g_lg.UX1; delete[] g_lg.UY1; delete[] g_lg.UZ1; }
+    delete[] g_lg.IGNORE;
+    g_lg.ETA0 = g_lg.ETA1 = nullptr;
+    g_lg.UX0 = g_lg.UY0 = g_lg.UZ0 = g_lg.UX1 = g_lg.UY1 = g_lg.UZ1 = nullptr;
+    g_lg.IGNORE = nullptr;
+    g_lg_alloc = false;
+}
 
 static void assign(std::vector<double>& v, const double* p, int n, double dflt) {
     v.assign(n, dflt);
@@ -270,5 +290,76 @@ void refh_grid_query(long n, const double* t, const double* x, const double* y, 
 // Sigma-map helpers (src/lsgrid_spline.cpp:89-122).
 double refh_sigmaS2cart(double s, double eta, double depth) { return g_grid.sigmaS2cart(s, eta, depth); }
 double refh_cart2sigmaS(double z, double eta, double depth) { return g_grid.cart2sigmaS(z, eta, depth); }
+
+// Legacy lsGrid as the unit tests drive it (unittests/lsgrid_tests.cpp:64-83):
+// allocate, set_ignore, initialize_surface_elevation[_with_ignore](t0),
+// initialize_kinematics[_with_ignore].  Returns the wall time of the two fills.
+double refh_lsgrid_init(const double* domain, int nx, int ny, int nl, double t0, double dt,
+                        double tan_a, double tan_b, double water_depth, double swl,
+                        const double* domain_ignore, int ignore_at_init, int init_only)
+{
+    lg_free();
+    for (int q = 0; q < 4; ++q) { g_lg.domain[q] = domain[q]; g_lg.domain_ignore[q] = domain_ignore[q]; }
+    g_lg.nx = nx; g_lg.ny = ny; g_lg.nl = nl;
+    g_lg.t0 = t0; g_lg.dt = dt; g_lg.tan_a = tan_a; g_lg.tan_b = tan_b;
+    g_lg.water_depth = water_depth; g_lg.swl = swl;
+    g_lg.ignore_at_init = ignore_at_init != 0;
+    g_lg.init_only = init_only != 0;
+    g_lg.update_count = 0;
+    g_lg.dump_vtk = false;
+    g_lg.allocate();
+    g_lg_alloc = true;
+    g_lg.set_ignore();
+    auto t_start = std::chrono::steady_clock::now();
+    if (g_lg.ignore_at_init) {
+        g_lg.initialize_surface_elevation_with_ignore(g_irr, t0);
+        g_lg.initialize_kinematics_with_ignore(g_irr);
+    } else {
+        g_lg.initialize_surface_elevation(g_irr, t0);
+        g_lg.initialize_kinematics(g_irr);
+    }
+    return std::chrono::duration<double>(std::chrono::steady_clock::now() - t_start).count();
+}
+
+void refh_lsgrid_update(double t_target) { g_lg.update(g_irr, t_target); }
+int refh_lsgrid_check_time(double t) { return g_lg.CheckTime(t) ? 1 : 0; }
+double refh_lsgrid_t0() { return g_lg.t0; }
+
+// which: 0 ETA, 4 UX, 5 UY, 6 UZ; slot 0 then slot 1
+long refh_lsgrid_export(int which, double* out) {
+    const long c = (long)g_lg.nx * g_lg.ny, v = c * g_lg.nl;
+    const double *s0 = nullptr, *s1 = nullptr; long n = 0;
+    switch (which) {
+    case 0: s0 = g_lg.ETA0; s1 = g_lg.ETA1; n = c; break;
+    case 4: s0 = g_lg.UX0; s1 = g_lg.UX1; n = v; break;
+    case 5: s0 = g_lg.UY0; s1 = g_lg.UY1; n = v; break;
+    case 6: s0 = g_lg.UZ0; s1 = g_lg.UZ1; n = v; break;
+    default: return 0;
+    }
+    if (out) { std::memcpy(out, s0, n * sizeof(double)); std::memcpy(out + n, s1, n * sizeof(double)); }
+    return 2 * n;
+}
+
+// lsGrid::eta / u / v / w (lsgrid.cpp:360-376) over n points -> out[4n].
+void refh_lsgrid_query(long n, const double* t, const double* x, const double* y, const double* z,
+                       double* out)
+{
+    #pragma omp parallel for schedule(static)
+    for (long p = 0; p < n; ++p) {
+        out[4 * p] = g_lg.eta(t[p], x[p], y[p]);
+        out[4 * p + 1] = g_lg.u(t[p], x[p], y[p], z[p]);
+        out[4 * p + 2] = g_lg.v(t[p], x[p], y[p], z[p]);
+        out[4 * p + 3] = g_lg.w(t[p], x[p], y[p], z[p]);
+    }
+}
+
+// lsGrid::export_vtu (lsgrid.cpp:1354-1552) into path.
+int refh_lsgrid_write_vtu(const char* path, int last) {
+    FILE* fp = std::fopen(path, "w");
+    if (!fp) return -1;
+    g_lg.export_vtu(fp, last != 0);
+    std::fclose(fp);
+    return 0;
+}
 
 } // extern "C"

@@ -16,6 +16,10 @@ Outputs (small npz fixtures, committed):
   grid.npz    lsGridSpline fields after initialize_kinematics and three
               wave_force_update steps (plain and ignore_at_init), plus spline
               queries at each step;
+  lsgrid.npz  the legacy two-slot lsGrid (src/lsgrid.cpp) of the cases in
+              cases.LSGRID_CASES: both slots of ETA/UX/UY/UZ after the fill and
+              after each update, queries and CheckTime probes at every stage,
+              and the reference-written .vtu of case A (both slots);
   abi_*.npz   a waveinput.dat and a program of C-ABI calls (ref_abi.py) with
               the reference's answers: direct mode with ramps, s-grid mode
               with updates, direct mode with vertical_theory smit, and the
@@ -34,7 +38,7 @@ sys.path.insert(0, ROOT)
 
 from oracle import KINDS, GRID_FIELDS, RefHarness, REF_DIR  # noqa: E402
 from cfdwavemaker_amd.seastate import make_sea, write_waveinput  # noqa: E402
-from tests.golden.cases import sea_cases, points_for, GRID_CASE  # noqa: E402
+from tests.golden.cases import sea_cases, points_for, GRID_CASE, LSGRID_CASES, lsgrid_sea, lsgrid_points  # noqa: E402
 
 REF_SRC = "/root/reference"
 
@@ -128,6 +132,38 @@ def make_grid(R):
             out[f"ign{ign}__s{step}__qout"] = R.grid_query(*q)
             out[f"ign{ign}__s{step}__state"] = np.array([st["tstep"], st["t0"], st["dx"], st["dy"], st["ds"]])
     np.savez_compressed(os.path.join(HERE, "grid.npz"), **out)
+
+
+def make_lsgrid(R):
+    """Legacy lsGrid through the compiled reference (ref_harness.cpp)."""
+    out = {}
+    for name, c in LSGRID_CASES.items():
+        R.setup(lsgrid_sea(c["sea"]))
+        R.lsgrid_init(c["domain"], c["nx"], c["ny"], c["nl"], c["t0"], c["dt"], c["tan_a"], c["tan_b"],
+                      c["depth"], c["swl"], c["ignore"], c["ignore_at_init"], c["init_only"])
+        for stage, tt in enumerate((None,) + tuple(c["updates"])):
+            if tt is not None:
+                R.lsgrid_update(tt)
+            t0 = R.lsgrid_t0()
+            key = f"{name}__s{stage}"
+            out[key + "__t0"] = np.float64(t0)
+            fields = ("ETA",) if name == "vtu" else ("ETA", "UX", "UY", "UZ")
+            for f in fields:
+                out[key + "__" + f] = R.lsgrid_export(f)
+            q = np.stack(lsgrid_points(c, t0, seed=1000 + 10 * stage + len(name)))
+            out[key + "__q"] = q
+            out[key + "__qout"] = R.lsgrid_query(*q)
+            probes = t0 + c["dt"] * np.array([0.0, 0.5, 1.0, 1.0 + 1e-9, 1.5, 3.0])
+            out[key + "__ct_t"] = probes
+            out[key + "__ct"] = np.array([R.lsgrid_check_time(v) for v in probes], dtype=np.int8)
+            if name == "A" and stage == 2:
+                with tempfile.TemporaryDirectory() as td:
+                    for last in (0, 1):
+                        path = os.path.join(td, f"k{last}.vtu")
+                        R.lsgrid_write_vtu(path, last)
+                        with open(path, "rb") as fh:
+                            out[f"{key}__vtu{last}"] = np.frombuffer(fh.read(), dtype=np.uint8)
+    np.savez_compressed(os.path.join(HERE, "lsgrid.npz"), **out)
 
 
 def run_abi(lib, waveinput_text, prog, files=False):
@@ -345,12 +381,17 @@ def main():
         make_abi_inputs()
         print("input-format fixtures written to", HERE)
         return
+    if sys.argv[1:] == ["lsgrid"]:
+        make_lsgrid(RefHarness())
+        print("legacy-grid fixtures written to", HERE)
+        return
     R = RefHarness()
     make_points(R)
     make_linear(R)
     make_pairs(R)
     make_smit(R)
     make_grid(R)
+    make_lsgrid(R)
     make_abi()
     make_abi_inputs()
     print("golden fixtures written to", HERE)

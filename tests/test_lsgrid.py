@@ -1,0 +1,262 @@
+"""The legacy two-slot grid (lsGrid, src/lsgrid.cpp) on the GPU through the C
+ABI (cfdwm_lsgrid_*): fills, one-step updates, re-initialising jumps,
+init_only, ignore regions, CheckTime, tri/bilinear queries and the .vtu
+writer against the compiled reference (lsgrid.npz); the reference's own
+unit-test known answers; and at the full 256 x 256 x 16, N = 1024 size the
+properties the interpolation fixes plus the oracle's interpolation on the
+GPU's fields.
+
+Stated bounds (scale-normalised error |err| / max|ref|): FP64 1e-9; FP32
+(three-product forms) 2e-5; BF16 (one-product velocity forms) 2e-3 for
+velocities and the elevation gate 1e-3 Hs.
+"""
+import os
+import tempfile
+import xml.etree.ElementTree as ET
+
+import numpy as np
+import pytest
+
+from tests.conftest import golden, scaled_err
+from tests.golden.cases import LSGRID_CASES, lsgrid_sea
+
+pytestmark = pytest.mark.gpu
+
+FP64_TOL = 1e-9
+FP32_TOL = 2e-5
+BF16_VEL_TOL = 2e-3
+FIELDS = ("ETA", "UX", "UY", "UZ")
+
+
+@pytest.fixture(scope="module")
+def eng():
+    from cfdwavemaker_amd import engine
+    engine.load()
+    return engine
+
+
+def _make(eng, c, prec):
+    sea = eng.Sea(lsgrid_sea(c["sea"]), precision=prec, with_direct=False)
+    g = sea.lsgrid(c["domain"], c["nx"], c["ny"], c["nl"], c["t0"], c["dt"], c["tan_a"], c["tan_b"], c["depth"],
+                   c["swl"], c["ignore"], c["ignore_at_init"], bool(c["init_only"]))
+    return sea, g
+
+
+def _stages(c):
+    return enumerate((None,) + tuple(c["updates"]))
+
+
+@pytest.mark.parametrize("name", list(LSGRID_CASES))
+def test_lsgrid_fp64_vs_reference(eng, name):
+    gd = golden("lsgrid.npz")
+    c = LSGRID_CASES[name]
+    sea, g = _make(eng, c, eng.FP64)
+    for stage, tt in _stages(c):
+        key = f"{name}__s{stage}"
+        print(f"lsgrid fp64 {name} stage {stage}", flush=True)
+        if tt is not None:
+            jump = (tt / c["dt"] - (g.t0 + 2 * c["dt"]) / c["dt"]) > 0.0
+            assert g.update(tt) == int(jump)
+        assert g.t0 == gd[key + "__t0"]
+        for f in FIELDS:
+            if key + "__" + f in gd:
+                err = scaled_err(g.export(f), gd[key + "__" + f])
+                assert err <= FP64_TOL, (stage, f, err)
+        got = g.query(*gd[key + "__q"])
+        ref = gd[key + "__qout"]
+        for col in range(4):
+            err = scaled_err(got[:, col], ref[:, col])
+            assert err <= FP64_TOL, (stage, col, err)
+        ct = np.array([g.CheckTime(v) for v in gd[key + "__ct_t"]], dtype=np.int8)
+        assert np.array_equal(ct, gd[key + "__ct"])
+    g.close()
+
+
+@pytest.mark.parametrize("prec_name", ["fp32", "bf16"])
+@pytest.mark.parametrize("name", ["unit", "A", "C"])
+def test_lsgrid_fp32_bf16_vs_reference(eng, name, prec_name):
+    gd = golden("lsgrid.npz")
+    c = LSGRID_CASES[name]
+    prec = eng.FP32 if prec_name == "fp32" else eng.BF16
+    sea, g = _make(eng, c, prec)
+    hs = 4.0 * np.sqrt(0.5 * np.sum(np.asarray(lsgrid_sea(c["sea"])["A"]) ** 2))
+    for stage, tt in _stages(c):
+        key = f"{name}__s{stage}"
+        print(f"lsgrid {prec_name} {name} stage {stage}", flush=True)
+        if tt is not None:
+            g.update(tt)
+        ref = gd[key + "__qout"]
+        got = g.query(*gd[key + "__q"])
+        eta_e = np.abs(g.export("ETA") - gd[key + "__ETA"]).max()
+        eta_q = np.abs(got[:, 0] - ref[:, 0]).max()
+        if prec_name == "fp32":
+            assert scaled_err(g.export("ETA"), gd[key + "__ETA"]) <= FP32_TOL
+            assert scaled_err(got[:, 0], ref[:, 0]) <= FP32_TOL
+        else:
+            assert eta_e <= 1e-3 * hs and eta_q <= 1e-3 * hs, (eta_e, eta_q, hs)
+        tol = FP32_TOL if prec_name == "fp32" else BF16_VEL_TOL
+        for k, f in enumerate(("UX", "UY", "UZ")):
+            err_f = scaled_err(g.export(f), gd[key + "__" + f])
+            err_q = scaled_err(got[:, 1 + k], ref[:, 1 + k])
+            print(f"  {f} field {err_f:.2e} query {err_q:.2e}", flush=True)
+            assert err_f <= tol and err_q <= tol, (stage, f, err_f, err_q)
+    g.close()
+
+
+def test_lsgrid_query_device_equals_host(eng):
+    import torch
+    gd = golden("lsgrid.npz")
+    c = LSGRID_CASES["A"]
+    sea, g = _make(eng, c, eng.FP64)
+    q = gd["A__s0__q"]
+    host = g.query(*q)
+    dev = [torch.tensor(np.ascontiguousarray(a), dtype=torch.float64, device="cuda") for a in q]
+    out = torch.empty((q.shape[1], 4), dtype=torch.float64, device="cuda")
+    torch.cuda.synchronize()
+    g.query_device(q.shape[1], *[a.data_ptr() for a in dev], out.data_ptr())
+    assert np.array_equal(out.cpu().numpy(), host)
+    assert g.query(np.zeros(0), np.zeros(0), np.zeros(0), np.zeros(0)).shape == (0, 4)
+    g.close()
+
+
+def _vtu_lines(text):
+    return text.decode().splitlines() if isinstance(text, bytes) else text.splitlines()
+
+
+@pytest.mark.parametrize("last", [0, 1])
+def test_lsgrid_vtu_vs_reference_writer(eng, last):
+    """export_vtu (lsgrid.cpp:1354-1552) of case A after two updates against
+    the reference-written file: identical structure lines, velocities at the
+    %g print precision, node positions at %12.4f, identical cells."""
+    gd = golden("lsgrid.npz")
+    c = LSGRID_CASES["A"]
+    sea, g = _make(eng, c, eng.FP64)
+    for tt in c["updates"][:2]:
+        g.update(tt)
+    ref = _vtu_lines(gd[f"A__s2__vtu{last}"].tobytes())
+    with tempfile.TemporaryDirectory() as d:
+        path = os.path.join(d, "k.vtu")
+        g.write_vtu(path, last=bool(last))
+        with open(path) as fh:
+            got = _vtu_lines(fh.read())
+        ET.parse(path)
+    assert len(got) == len(ref)
+    section = None
+    for a, b in zip(got, ref):
+        s = b.strip()
+        if s.startswith("<") or not s:
+            assert a == b
+            if 'Name="Velocity"' in s:
+                section = "vel"
+            elif "<Points>" in s:
+                section = "pts"
+            elif 'Name="connectivity"' in s:
+                section = "cells"
+            continue
+        va = np.array(a.split(), dtype=float)
+        vb = np.array(b.split(), dtype=float)
+        if section == "vel":
+            assert np.allclose(va, vb, rtol=6e-6, atol=1e-12), (a, b)
+        elif section == "pts":
+            assert np.allclose(va, vb, rtol=0, atol=1.01e-4), (a, b)
+        else:
+            assert a == b, (a, b)
+    g.close()
+
+
+def test_lsgrid_vtk_dumps(eng):
+    """dump_vtk: the fill's dump, one per update (step or jump) and the last
+    dump (slot 1 at t0 + dt) when the grid is destroyed."""
+    c = LSGRID_CASES["A"]
+    sea, g = _make(eng, c, eng.FP64)
+    with tempfile.TemporaryDirectory() as d:
+        out = os.path.join(d, "vtk") + "/"
+        g.set_vtk(True, out, "waves_", "Time")
+        t_fill = g.t0
+        g.update(0.6)
+        t_step = g.t0
+        g.update(3.0)
+        t_jump = g.t0
+        g.close()
+        files = sorted(os.listdir(out))
+        assert files == [f"waves_{k:05d}.vtu" for k in range(4)]
+        times = []
+        for f in files:
+            fd = ET.parse(os.path.join(out, f)).getroot().find("UnstructuredGrid/FieldData/DataArray")
+            assert fd.get("Name") == "Time"
+            times.append(float(fd.text))
+        assert np.allclose(times, [t_fill, t_step, t_jump, t_jump + c["dt"]], atol=5e-4)
+
+
+def test_lsgrid_unit_test_known_answers(eng, oracle):
+    """lsgrid_tests.cpp:61-123: the grid's elevation at a node equals the direct
+    Irregular::eta; velocities at the seabed within 1e-5 of the direct u, v, w,
+    at z = -10 within 1e-1 (direct values from the oracle)."""
+    c = LSGRID_CASES["unit"]
+    sea, g = _make(eng, c, eng.FP64)
+    osea = oracle.sea(lsgrid_sea("unit"))
+    t, x, y = 4.0, 0.0, 0.0
+    eta = g.query(t, x, y, 0.0)[0, 0]
+    ref = osea.eval("eta", t, x, y)[()]
+    assert abs(eta - ref) <= 1e-12 * max(1.0, abs(ref))
+    for z, tol in ((-300.0, 1e-5), (-10.0, 1e-1)):
+        q = g.query(t, x, y, z)[0, 1:]
+        d = np.array([osea.eval(k, t, x, y, z)[()] for k in ("u", "v", "w")])
+        assert np.all(np.abs(q - d) <= tol), (z, q - d)
+    g.close()
+
+
+def test_lsgrid_full_size(eng, oracle):
+    """N = 1024 short-crested sea, 256 x 256 x 16 legacy grid in the bf16 mode:
+    surface-node queries return the slots' top levels exactly at t0 and
+    t0 + dt; 2e5 random queries equal the oracle's interpolation over the
+    GPU's own fields; a one-step update moves slot 1 into slot 0 bit for bit;
+    slots agree with the spline grid's fill at the same times."""
+    from cfdwavemaker_amd.seastate import config_sea
+    s = config_sea(3)
+    d = s.as_dict()
+    sea = eng.Sea(d, precision=eng.BF16, with_direct=False)
+    dom, nx, ny, nl, t0, dt, a, b, h = (0.0, 1000.0, 0.0, 1000.0), 256, 256, 16, 0.0, 0.5, 0.7, 1.5, 100.0
+    g = sea.lsgrid(dom, nx, ny, nl, t0, dt, a, b, h)
+    print("lsgrid full size: filled", flush=True)
+    E = g.export("ETA").reshape(2, nx, ny)
+    U = [g.export(f) for f in ("UX", "UY", "UZ")]
+    dx = (dom[1] - dom[0]) / (nx - 1)
+    ii, jj = np.meshgrid(np.arange(nx), np.arange(ny), indexing="ij")
+    xs, ys = dom[0] + dx * ii.ravel(), dom[2] + dx * jj.ravel()
+    for slot, tq in ((0, t0), (1, t0 + dt)):
+        got = g.query(np.full(xs.size, tq), xs, ys, E[slot].ravel())
+        assert np.array_equal(got[:, 0], E[slot].ravel())
+        for k in range(3):
+            top = U[k].reshape(2, nx, ny, nl)[slot, :, :, nl - 1].ravel()
+            assert np.array_equal(got[:, 1 + k], top)
+    rng = np.random.default_rng(7)
+    n = 200_000
+    q = np.stack([rng.uniform(t0 - 0.1, t0 + 0.6, n), rng.uniform(-20, 1020, n), rng.uniform(-20, 1020, n),
+                  rng.uniform(-h - 5, 8, n)])
+    got = g.query(*q)
+    og = oracle.lsgrid_from_fields(dom, nx, ny, nl, t0, dt, a, b, h, 0.0, E.ravel(), *U)
+    ref = og.query(*q)
+    # the cell fractions go through float: a 1-ulp difference of the device's
+    # atan / pow can move one across a float rounding step (a 6e-8 change of
+    # the fraction) for about one point in 1e9; allow a few such points
+    for col in range(4):
+        err = scaled_err(got[:, col], ref[:, col])
+        scale = max(np.abs(ref[:, col]).max(), 1e-300)
+        off = int(np.count_nonzero(np.abs(got[:, col] - ref[:, col]) > 1e-12 * scale))
+        print(f"  query col {col} vs oracle interpolation {err:.2e} ({off} points above 1e-12)", flush=True)
+        assert err <= 1e-6 and off <= 3, (col, err, off)
+    grid = sea.grid(dom, nx, ny, nl, t0, dt, a, b, h)
+    for slot, ring in ((0, 1), (1, 2)):          # the spline grid's ring slots at t0 and t0 + dt
+        for k, f in enumerate(("ETA", "UX", "UY", "UZ")):
+            ref = grid.export(f).reshape(4, -1)[ring]
+            mine = (E.reshape(2, -1) if k == 0 else U[k - 1].reshape(2, -1))[slot]
+            err = scaled_err(mine, ref)
+            assert err <= 1e-5, (slot, f, err)
+    grid.close()
+    before = [g.export(f).reshape(2, -1) for f in FIELDS]
+    assert g.update(t0 + dt + 0.1) == 0
+    after = [g.export(f).reshape(2, -1) for f in FIELDS]
+    for k in range(4):
+        assert np.array_equal(after[k][0], before[k][1])
+    g.close()
