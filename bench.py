@@ -510,6 +510,71 @@ def update_leg(engine, grid, updates: int) -> dict:
             "kernels_ms": {k: v[0] / max(v[1], 1) for k, v in prof.items()}}
 
 
+def lsgrid_leg(engine, sea, nq: int, steps: int) -> dict:
+    """The legacy two-slot grid (lsGrid) on the same sea: a full fill (a
+    re-initialising update: 2 time levels of 256 x 256 columns x 16 levels),
+    one-step updates (1 time level) and nq tri/bilinear queries with FP64
+    coordinates resident on the device (k_lsquery: 32 B in + 32 B out per
+    query plus its share of one pass over the 25.7 MB of slots)."""
+    import torch
+    g = sea.lsgrid(DOMAIN, NX, NY, NL, T0, DT, TAN_A, TAN_B, DEPTH)
+    engine.sync()
+    engine.prof_enable(True)
+    engine.prof_read(reset=True)
+    t_fill = 0.0
+    for k in range(steps):
+        tj = T0 + (10 * (k + 1)) * DT + 0.01
+        c0 = time.perf_counter()
+        if g.update(tj) != 1:
+            raise RuntimeError("lsgrid leg: expected a re-initialisation")
+        engine.sync()
+        t_fill += time.perf_counter() - c0
+    t_step = 0.0
+    for k in range(steps):
+        c0 = time.perf_counter()
+        if g.update(g.t0 + 1.5 * DT) != 0:
+            raise RuntimeError("lsgrid leg: expected one step")
+        engine.sync()
+        t_step += time.perf_counter() - c0
+    fill_prof = engine.prof_read(reset=True)
+    dev = torch.device("cuda:0")
+    gen = torch.Generator(device="cpu").manual_seed(5)
+    t0 = g.t0
+    qt = (t0 + torch.rand(nq, generator=gen, dtype=torch.float64) * DT).to(dev)
+    qx = (torch.rand(nq, generator=gen, dtype=torch.float64) * 1000.0).to(dev)
+    qy = (torch.rand(nq, generator=gen, dtype=torch.float64) * 1000.0).to(dev)
+    qz = torch.zeros(nq, dtype=torch.float64, device=dev)
+    out = torch.empty((nq, 4), dtype=torch.float64, device=dev)
+    torch.cuda.synchronize()
+    g.query_device(nq, qt.data_ptr(), qx.data_ptr(), qy.data_ptr(), qz.data_ptr(), out.data_ptr())
+    u = torch.rand(nq, generator=gen, dtype=torch.float64).to(dev)
+    qz.copy_(-DEPTH + u * (out[:, 0] + DEPTH))
+    torch.cuda.synchronize()
+    engine.prof_read(reset=True)
+    c0 = time.perf_counter()
+    for _ in range(steps):
+        g.query_device(nq, qt.data_ptr(), qx.data_ptr(), qy.data_ptr(), qz.data_ptr(), out.data_ptr())
+    t_q = (time.perf_counter() - c0) / steps
+    prof = engine.prof_read(reset=True)
+    engine.prof_enable(False)
+    q_ms = prof.get("k_lsquery", (float("nan"), 1))
+    q_avg = q_ms[0] / max(q_ms[1], 1) * 1e-3
+    slot_bytes = 2 * (NX * NY * NL * 3 + NX * NY) * (8 if sea.precision == engine.FP64 else 4)
+    alg = 64.0 * nq + slot_bytes
+    g.close()
+    return {"workload": f"legacy lsGrid of the N=1024 sea, {NX}x{NY} columns x {NL} levels x 2 time levels; "
+                        f"{nq} tri/bilinear queries with FP64 coordinates on the device",
+            "fill_ms": t_fill / steps * 1e3, "fill_points_per_s": 2 * NX * NY * NL * steps / t_fill,
+            "step_ms": t_step / steps * 1e3, "step_points_per_s": NX * NY * NL * steps / t_step,
+            "fill_kernels_ms": {k: v[0] / max(v[1], 1) for k, v in fill_prof.items()},
+            "query_ms": t_q * 1e3, "queries_per_s": nq / q_avg,
+            "roofline": {"bound": "hbm", "kernel": "k_lsquery", "bytes_per_launch": alg,
+                         "byte_rule": "64 B per query (t, x, y, z in, eta, u, v, w out, FP64) + one pass over the "
+                                      "two slots", "avg_launch_ms": q_avg * 1e3,
+                         "achieved": alg / q_avg / 1e9, "peak": PEAK_HBM / 1e9, "unit": "GB/s",
+                         "frac": alg / q_avg / PEAK_HBM}}
+
+
 def direct_leg(steps: int, smit: bool = False) -> dict:
     """Direct (per-point semantics) workload: N=512 long-crested sea, 256 x on
     [0, 500] m x 16 t on [0, 15] s columns, 64 z per column on [-30, eta]:
@@ -588,6 +653,7 @@ def main():
                     help="print a markdown summary of a rocprofv3 results database and exit")
     ap.add_argument("--no-surrogate", action="store_true", help="skip the learned-surrogate leg")
     ap.add_argument("--no-direct", action="store_true", help="skip the direct per-point workload leg")
+    ap.add_argument("--no-lsgrid", action="store_true", help="skip the legacy two-slot grid leg")
     ap.add_argument("--updates", type=int, default=5,
                     help="time-stepping updates of the grid to time after the fill (0: skip)")
     ap.add_argument("--batched-states", type=int, default=512,
@@ -759,6 +825,7 @@ def main():
                                "frac_unpacked": lin_flop / t_lin / PEAK_UNPACKED[a.precision]},
     }
     upd = update_leg(engine, grid, a.updates) if a.updates > 0 else None
+    lsg = None if a.no_lsgrid else lsgrid_leg(engine, sea, nq, a.steps)
 
     # accuracy gate: FP32 (or FP64) grid vs an FP64 grid on an 8 x 8 block of the same columns
     dx = (DOMAIN[1] - DOMAIN[0]) / (NX - 1)
@@ -824,6 +891,7 @@ def main():
         "direct": dirl,
         "direct_smit": smitl,
         "update": upd,
+        "legacy_lsgrid": lsg,
         "abi_batched": abil,
     }
     print(json.dumps(line), flush=True)
