@@ -328,8 +328,12 @@ __device__ __forceinline__ void qf_tile_il(const QfJob& J, int ptile, int form) 
 }
 
 // Single-job launch: block -> (point tile, form) through the swizzle.
+// One product per term stages 64 KiB of LDS per block: with the accumulators
+// in VGPRs (no AGPRs) two blocks share a CU, one computing while the other
+// waits at its barrier.  Three products stage 128 KiB: one block per CU.
 template <int NLV>
-__global__ __launch_bounds__(256) void k_qf_forms_il(QfJob J, int npt) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NLV == 1 ? 2 : 1, 2)))
+void k_qf_forms_il(QfJob J, int npt) {
     const int id = blockIdx.x;
     const int per = kQfGF * kQfGP;
     const int grp = id / per, in = id - grp * per;
@@ -354,7 +358,8 @@ __global__ __launch_bounds__(256) void k_qf_forms_il(QfJob J, int npt) {
 // tiles b, b + gridDim.x, ... of the ordered tile list; every block exits
 // after its last tile.
 template <int NLV>
-__global__ __launch_bounds__(256) void k_qf_forms_persist(const QfJob* __restrict__ jobs,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NLV == 1 ? 2 : 1, 2)))
+void k_qf_forms_persist(const QfJob* __restrict__ jobs,
                                                           const QfTile* __restrict__ tiles, int ntiles) {
     for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
         const QfTile tl = tiles[t];
