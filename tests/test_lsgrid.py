@@ -119,6 +119,24 @@ def test_lsgrid_query_device_equals_host(eng):
     g.close()
 
 
+@pytest.mark.parametrize("prec_name", ["fp64", "fp32"])
+def test_lsgrid_cell_order_equals_caller_order(eng, prec_name):
+    """A batch of 200,000 queries (evaluated in cell order) equals the same
+    queries in batches of 50,000 (caller order) bit for bit."""
+    c = dict(LSGRID_CASES["A"])
+    c.update(nx=41, ny=23, nl=9, ignore=(1e12, -1e12, 1e12, -1e12), ignore_at_init=0)
+    prec = eng.FP64 if prec_name == "fp64" else eng.FP32
+    sea, g = _make(eng, c, prec)
+    rng = np.random.default_rng(11)
+    n = 200_000
+    q = np.stack([rng.uniform(-0.1, 0.6, n), rng.uniform(-5, 125, n), rng.uniform(-25, 25, n),
+                  rng.uniform(-31, 3, n)])
+    big = g.query(*q)
+    parts = np.concatenate([g.query(*q[:, k:k + 50_000]) for k in range(0, n, 50_000)])
+    assert np.array_equal(big, parts)
+    g.close()
+
+
 def _vtu_lines(text):
     return text.decode().splitlines() if isinstance(text, bytes) else text.splitlines()
 
