@@ -523,7 +523,7 @@ private:
     int fill_both(bool use_ignore);
     int copy_slot(int src, int dst, bool active_only);
     DevBuf qin_, qout_;
-    DevBuf okey_, orec_, otmp_;              // cell-ordering buffers of large query batches
+    DevBuf okey_, oidx_, otmp_;              // cell-ordering buffers of large query batches
 };
 
 // Refill many s-grids (sea states) with their second-order forms evaluated

@@ -25,11 +25,6 @@ namespace cfdwm {
 
 namespace {
 
-// A query of a cell-ordered batch: its coordinates and its index in the caller's arrays.
-struct LsRec {
-    double t, x, y, z;
-    uint64_t p;
-};
 
 struct LsQueryJob {
     int64_t n = 0;
@@ -40,8 +35,9 @@ struct LsQueryJob {
     const void *eta = nullptr, *u = nullptr, *v = nullptr, *w = nullptr;
     const double *t = nullptr, *x = nullptr, *y = nullptr, *z = nullptr;
     double* out = nullptr;                     // [n][4]
-    // when set: thread p evaluates srec[p] (cell order) and writes out[srec[p].p]
-    const LsRec* srec = nullptr;
+    // when set: thread q evaluates the query sidx[q] (cell order): its inputs
+    // are gathered, its outputs written at its own index
+    const uint32_t* sidx = nullptr;
 };
 
 // std::min / std::max / clip (lsgrid.cpp:187-190) with the reference's
@@ -70,15 +66,8 @@ __global__ void __launch_bounds__(256) k_lsquery(LsQueryJob J) {
 #pragma clang fp contract(off)
     const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (q >= J.n) return;
-    int64_t p = q;
-    double tpt, xpt, ypt, zpt;
-    if (J.srec) {
-        const LsRec r = J.srec[q];
-        tpt = r.t; xpt = r.x; ypt = r.y; zpt = r.z;
-        p = (int64_t)r.p;
-    } else {
-        tpt = J.t[q]; xpt = J.x[q]; ypt = J.y[q]; zpt = J.z[q];
-    }
+    const int64_t p = J.sidx ? (int64_t)J.sidx[q] : q;
+    const double tpt = J.t[p], xpt = J.x[p], ypt = J.y[p], zpt = J.z[p];
     const int nx = J.nx, ny = J.ny, nl = J.nl;
     int i, j;
     const double xd = fmodf_split(smax(0., smin((xpt - J.x0) / J.dx, nx - 1.)), nx - 1, &i);
@@ -130,21 +119,17 @@ __global__ void __launch_bounds__(256) k_lsquery(LsQueryJob J) {
 }
 
 // key = column i ny + j of the query's cell (the query kernel's clamps; the
-// order only affects locality, not results) and the query's record
+// order only affects locality, not results) and the query's index
 __global__ void __launch_bounds__(256) k_ls_keys(int64_t n, int nx, int ny, double x0, double y0, double dx,
-                                                 double dy, const double* __restrict__ t, const double* __restrict__ x,
-                                                 const double* __restrict__ y, const double* __restrict__ z,
-                                                 uint32_t* __restrict__ key, LsRec* __restrict__ rec) {
+                                                 double dy, const double* __restrict__ x, const double* __restrict__ y,
+                                                 uint32_t* __restrict__ key, uint32_t* __restrict__ idx) {
     const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= n) return;
-    LsRec r;
-    r.t = t[p]; r.x = x[p]; r.y = y[p]; r.z = z[p];
-    r.p = (uint64_t)p;
     int i, j;
-    fmodf_split(smax(0., smin((r.x - x0) / dx, nx - 1.)), nx - 1, &i);
-    fmodf_split(smax(0., smin((r.y - y0) / dy, ny - 1.)), ny - 1, &j);
+    fmodf_split(smax(0., smin((x[p] - x0) / dx, nx - 1.)), nx - 1, &i);
+    fmodf_split(smax(0., smin((y[p] - y0) / dy, ny - 1.)), ny - 1, &j);
     key[p] = (uint32_t)((int64_t)i * ny + j);
-    rec[p] = r;
+    idx[p] = (uint32_t)p;
 }
 
 // slot dst := slot src for ETA and UX/UY/UZ of the listed columns (all when cols is null)
@@ -178,14 +163,14 @@ unsigned blocks_for(int64_t n) { return (unsigned)((n + 255) / 256); }
 // batches of this many queries or more run in cell order
 constexpr int64_t kLsOrderMin = 65536;
 
-// keys and records of the batch J, then their radix sort by cell into keys + n, recs + n
-hipError_t launch_ls_order(const LsQueryJob& J, uint32_t* keys, LsRec* recs, void* tmp, size_t tmp_bytes, int bits,
+// keys and indices of the batch J, then their radix sort by cell into keys + n, idx + n
+hipError_t launch_ls_order(const LsQueryJob& J, uint32_t* keys, uint32_t* idx, void* tmp, size_t tmp_bytes, int bits,
                            hipStream_t st) {
     const int64_t n = J.n;
-    hipLaunchKernelGGL(k_ls_keys, dim3(blocks_for(n)), dim3(256), 0, st, n, J.nx, J.ny, J.x0, J.y0, J.dx, J.dy, J.t,
-                       J.x, J.y, J.z, keys, recs);
+    hipLaunchKernelGGL(k_ls_keys, dim3(blocks_for(n)), dim3(256), 0, st, n, J.nx, J.ny, J.x0, J.y0, J.dx, J.dy, J.x,
+                       J.y, keys, idx);
     if (hipError_t e = hipGetLastError()) return e;
-    return hipcub::DeviceRadixSort::SortPairs(tmp, tmp_bytes, keys, keys + n, recs, recs + n, (int)n, 0, bits, st);
+    return hipcub::DeviceRadixSort::SortPairs(tmp, tmp_bytes, keys, keys + n, idx, idx + n, (int)n, 0, bits, st);
 }
 
 template <typename R>
@@ -309,19 +294,20 @@ int LsGrid::query(int64_t n, const double* t, const double* x, const double* y, 
     }
     if (n >= kLsOrderMin && n <= INT32_MAX) {
         // cell order: the lanes of a wavefront share a few cells, so their
-        // grid loads coalesce; the outputs go to each query's own index
+        // grid loads coalesce; inputs are gathered and outputs written at
+        // each query's own index
         int bits = 1;
         while (((int64_t)1 << bits) < (int64_t)g->g.nx * g->g.ny) ++bits;
         size_t tmp = 0;
         CFDWM_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, tmp, (const uint32_t*)nullptr, (uint32_t*)nullptr,
-                                                     (const LsRec*)nullptr, (LsRec*)nullptr, (int)n, 0, bits));
-        if (okey_.alloc((size_t)n * 2 * sizeof(uint32_t)) || orec_.alloc((size_t)n * 2 * sizeof(LsRec)) ||
+                                                     (const uint32_t*)nullptr, (uint32_t*)nullptr, (int)n, 0, bits));
+        if (okey_.alloc((size_t)n * 2 * sizeof(uint32_t)) || oidx_.alloc((size_t)n * 2 * sizeof(uint32_t)) ||
             otmp_.alloc(std::max<size_t>(tmp, 16)))
             return -3;
         uint32_t* keys = (uint32_t*)okey_.get();
-        LsRec* recs = (LsRec*)orec_.get();
-        PROF("k_lsquery_sort", launch_ls_order(J, keys, recs, otmp_.get(), tmp, bits, st));
-        J.srec = recs + n;
+        uint32_t* idx = (uint32_t*)oidx_.get();
+        PROF("k_lsquery_sort", launch_ls_order(J, keys, idx, otmp_.get(), tmp, bits, st));
+        J.sidx = idx + n;
     }
     if (g->precision() == kFP32)
         PROF("k_lsquery", launch_lsquery<float>(J, st));
