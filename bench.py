@@ -71,6 +71,20 @@ def _grid_fill_seconds(ncols: int):
         return "port", tot
 
 
+def _lsgrid_fill_seconds(ncols: int):
+    """One timed legacy-grid fill (initialize_surface_elevation +
+    initialize_kinematics, two time levels) of an ncols x ncols subset of the
+    N=1024 grid by the compiled reference's OpenMP code (src/lsgrid.cpp)."""
+    from cfdwavemaker_amd.seastate import config_sea
+    from oracle import RefHarness
+    s = config_sea(3).as_dict()
+    dx = (DOMAIN[1] - DOMAIN[0]) / (NX - 1)
+    dom = (0.0, dx * (ncols - 1), 0.0, dx * (ncols - 1))
+    R = RefHarness()
+    R.setup(s)
+    return "reference", R.lsgrid_init(dom, ncols, ncols, NL, T0, DT, TAN_A, TAN_B, DEPTH)
+
+
 def _direct_seconds(ncolumns: int):
     """One timed direct-mode evaluation by the compiled reference (Irregular
     members in an OpenMP parallel loop over points): eta at ncolumns (x, t)
@@ -99,7 +113,8 @@ def _direct_seconds(ncolumns: int):
 def cpu_worker(spec: str, out_path: str) -> None:
     """Timing protocol of one CPU baseline: one warm-up run, then five runs
     at each of two sample sizes; per size the median.  spec: "grid:a,b"
-    (ncols x ncols s-grid subsets) or "direct:a,b" (columns of 64 points)."""
+    (ncols x ncols s-grid subsets), "lsgrid:a,b" (the same for the legacy
+    two-slot grid) or "direct:a,b" (columns of 64 points)."""
     mode, sizes = spec.split(":")
     sizes = [int(v) for v in sizes.split(",")]
     res = {"mode": mode, "runs": {}}
@@ -108,6 +123,11 @@ def cpu_worker(spec: str, out_path: str) -> None:
         for n in sizes:
             secs = [_grid_fill_seconds(n)[1] for _ in range(5)]
             res["runs"][str(n)] = {"points": 4 * n * n * NL, "seconds": secs}
+    elif mode == "lsgrid":
+        kind, _ = _lsgrid_fill_seconds(sizes[0])                # warm-up
+        for n in sizes:
+            secs = [_lsgrid_fill_seconds(n)[1] for _ in range(5)]
+            res["runs"][str(n)] = {"points": 2 * n * n * NL, "seconds": secs}
     else:
         kind, _, _ = _direct_seconds(sizes[0])                 # warm-up
         for n in sizes:
@@ -857,6 +877,14 @@ def main():
                                f"time levels of the same N=1024 sea and spacing (value: the larger)")
         except Exception as e:  # reported, not fatal
             cpu = {"value": None, "error": str(e)[-300:]}
+        if lsg is not None:
+            try:
+                lsg["cpu_baseline"] = cpu_baseline("lsgrid:4,8", threads,
+                                                   "legacy-grid fills of 4^2 and 8^2 columns x 16 levels x 2 time "
+                                                   "levels of the same N=1024 sea and spacing (value: the larger)")
+                lsg["fill_over_cpu"] = lsg["fill_points_per_s"] / lsg["cpu_baseline"]["value"]
+            except Exception as e:  # reported, not fatal
+                lsg["cpu_baseline"] = {"value": None, "error": str(e)[-300:]}
         if dirl is not None:
             try:
                 dirl["cpu_baseline"] = cpu_baseline("direct:4,8", threads,
