@@ -12,7 +12,7 @@ call writes up to four doubles into ``out`` (n x 4).  Op codes:
   9 mean_length(opt)            10 water_depth()
 
 The same driver runs the compiled reference (tests/golden/make_golden.py) and
-this project's libcfdwm (tests/test_abi_golden.py), each in its own process
+this project's libcfdwm (tests/test_abi.py), each in its own process
 because both keep one process-global wave field.
 """
 import ctypes as C

@@ -81,6 +81,40 @@ def test_reference_programs_through_abi(fixture):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("fixture", ["abi_direct.npz", "abi_grid.npz"])
+def test_reference_style_wrapper(fixture):
+    """The same call programs through cfdwavemaker_amd.cfdwavemaker.CFDwavemaker,
+    the class and method names of the reference's examples/python/cfdwavemaker.py:
+    its answers equal the compiled reference's (1e-9 scale-normalised), and its
+    batched eval_points equals its per-point calls (rounding level: wave type 1
+    batches sum in another order)."""
+    g = golden(fixture)
+    with tempfile.TemporaryDirectory() as wd:
+        with open(os.path.join(wd, "waveinput.dat"), "w") as f:
+            f.write(str(g["waveinput"]))
+        pp, op_ = os.path.join(wd, "prog.npz"), os.path.join(wd, "out.npz")
+        np.savez(pp, op=g["op"], args=g["args"])
+        r = subprocess.run([sys.executable, os.path.join(ROOT, "tests", "wrapper_prog.py"), wd, pp, op_],
+                           capture_output=True, text=True, timeout=600)
+        assert r.returncode == 0, r.stderr[-2000:]
+        o = dict(np.load(op_))
+    assert int(o["rc"]) == 0 and int(o["initialized"]) == 1
+    out, ref, ops = o["out"], g["out"], g["op"]
+    for op in np.unique(ops):
+        sel = ops == op
+        for c in range(4):
+            err = scaled_err(out[sel, c], ref[sel, c])
+            assert err <= 1e-9, (fixture, int(op), c, err)
+    pts, batch = o["pts"], o["batch"]
+    col = {0: 0, 1: 1, 2: 2, 3: 3}
+    for k, q in enumerate(pts):
+        opq = int(ops[q])
+        got = batch[k] if opq == 5 else batch[k, col[opq]:col[opq] + 1]
+        want = out[q] if opq == 5 else out[q, :1]
+        assert np.allclose(got, want, rtol=1e-12, atol=1e-12 * max(1.0, np.abs(ref).max())), (q, got, want)
+
+
+@pytest.mark.gpu
 def test_missing_waveinput_returns_minus_one():
     code = ("import ctypes,os,sys; os.chdir(sys.argv[2]); L=ctypes.CDLL(sys.argv[1]); "
             "sys.exit(0 if L.wave_Initialize()==-1 and L.CFDwavemaker_is_initialized()==0 else 3)")
