@@ -786,8 +786,10 @@ def main():
         fill_exec = qfi["mfma_flop"] / per_fill
         split = qfi["products"] == 1
         R = qfi["rank"]
+        nvf = qfi["nform"] - 1                        # velocity forms: 2 R (u, v) + R_w (w)
         alg_launch = (24.0 if split else 32.0) * ncomp * ncomp * cs / per_fill
-        flop_launch = fill_exec * R / (R + 1.0) if split else fill_exec
+        # the eta2 form runs three products per term, each velocity form one
+        flop_launch = fill_exec * nvf / (nvf + 3.0) if split else fill_exec
         roof = {"bound": "mfma", "kernel": "k_qf_forms_il<1> (velocity forms)" if split else "k_qf_forms_il<3>",
                 "achieved": alg_launch / avg / 1e12,
                 "peak": PEAK_MFMA_BF16 / 1e12, "unit": "TFLOP/s", "frac": alg_launch / avg / PEAK_MFMA_BF16,
@@ -798,7 +800,8 @@ def main():
                               else "32 N^2 per column-slot (eta2, u, v, w forms x 2 (2N)^2)"),
                 "exec_flop_per_launch": flop_launch, "exec_achieved": flop_launch / avg / 1e12,
                 "exec_frac": flop_launch / avg / PEAK_MFMA_BF16, "exec_over_alg": flop_launch / alg_launch,
-                "forms": qfi["nform"], "bf16_products": qfi["products"], "depth_basis_rank": R,
+                "forms": qfi["nform"], "bf16_products": qfi["products"],
+                "depth_basis_ranks": {"cosh_type_u_v": R, "sinh_type_w": nvf - 2 * R},
                 "depth_basis_err": qfi["basis_err"], "depth_basis_tol": qfi["basis_tol"]}
         if split and e_n:
             e_avg = e_ms / e_n * 1e-3

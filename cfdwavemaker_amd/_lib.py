@@ -31,7 +31,7 @@ BATCH_API = [
     "cfdwm_sea_terms", "cfdwm_eval", "cfdwm_grid_create", "cfdwm_grid_destroy",
     "cfdwm_grid_refill", "cfdwm_grid_force_update", "cfdwm_grid_state", "cfdwm_grid_export",
     "cfdwm_grid_query", "cfdwm_grid_query_device_f32", "cfdwm_sync", "cfdwm_last_error",
-    "cfdwm_prof_enable", "cfdwm_prof_read", "cfdwm_grid_qform_info", "cfdwm_depth_basis",
+    "cfdwm_prof_enable", "cfdwm_prof_read", "cfdwm_grid_qform_info", "cfdwm_depth_basis", "cfdwm_profile_basis",
     "cfdwm_mlp_create", "cfdwm_mlp_destroy", "cfdwm_mlp_nparams", "cfdwm_mlp_get_params",
     "cfdwm_mlp_set_params", "cfdwm_mlp_set_norm", "cfdwm_mlp_get_norm", "cfdwm_mlp_forward_device",
     "cfdwm_mlp_loss_grad_device", "cfdwm_mlp_train_device", "cfdwm_surrogate_linear", "cfdwm_surrogate_features", "cfdwm_surrogate_add",
@@ -160,6 +160,8 @@ def load(path: str | None = None) -> C.CDLL:
     L.cfdwm_grid_qform_mode.argtypes = [C.c_void_p, C.POINTER(C.c_int), _dp, C.POINTER(C.c_int)]
     L.cfdwm_depth_basis.restype = C.c_int
     L.cfdwm_depth_basis.argtypes = [_d, _d, _d, C.c_int, _dp, _dp, _dp]
+    L.cfdwm_profile_basis.restype = C.c_int
+    L.cfdwm_profile_basis.argtypes = [_d, _d, C.c_int, _d, C.c_int, _dp, _dp, _dp]
     _fp = C.POINTER(C.c_float)
     L.cfdwm_mlp_create.restype = C.c_int
     L.cfdwm_mlp_create.argtypes = [C.POINTER(C.c_void_p), C.c_int, C.c_int, C.c_int, C.c_int, C.c_uint64]

@@ -1340,6 +1340,18 @@ int cfdwm_depth_basis(double kmax, double tmax, double tol, int rmax, double* mu
     if (err) *err = B.err;
     return rc == 0 ? B.R : -B.R;
 }
+int cfdwm_profile_basis(double kmax, double h, int sign, double tol, int rmax, double* mu, double* zs,
+                        double* err) {
+    if (rmax < 1 || rmax > 64 || !(kmax > 0) || !(h > 0) || (sign != 1 && sign != -1)) return 0;
+    ProfileBasis B;
+    const int rc = build_profile_basis(kmax, h, sign, tol, rmax, B);
+    for (int q = 0; q < B.R; ++q) {
+        if (mu) mu[q] = B.mu[q];
+        if (zs) zs[q] = B.zs[q];
+    }
+    if (err) *err = B.err;
+    return rc == 0 ? B.R : -B.R;
+}
 int cfdwm_surrogate_linear(cfdwm_sea* sea, int64_t n, const double* x, const double* y, double t, double h,
                            int nlev, const double* sigma, int fstride, double kp, float* X, float* Lin) {
     if (!sea || n < 0 || nlev < 1 || nlev > 32 || fstride < 1 || !sigma || (n && (!x || !y || !X || !Lin))) {

@@ -279,10 +279,11 @@ double freq_ramp_host(double t, double rt, double ts, double du) {
 
 // Tables of the quadratic forms over the filtered pairs (i, m), m in
 // [i+1, mhi[i]), split into bf16 hi + lo in the tiled operand layout (rows i,
-// k = m):  form 0 (eta2):  B1 = xP + xM, B2 = xM - xP;  per depth-basis rank q
-// (coefficients cP = c_q(kP), cM = c_q(kM)):  u: B1 = cuP cP + cuM cM,
-// B2 = cuM cM - cuP cP;  v likewise with cv;  w (sin-type): B1 = cwP cP + cwM cM,
-// B2 = cwP cP - cwM cM.
+// k = m):  form 0 (eta2):  B1 = xP + xM, B2 = xM - xP;  per rank q of the
+// cosh-type profile basis (coefficients aP = a_q(kP), aM = a_q(kM)):
+// u: B1 = cuP aP + cuM aM, B2 = cuM aM - cuP aP;  v likewise with cv;  per
+// rank q of the sinh-type basis (bP, bM):  w (sin-type): B1 = cwP bP + cwM bM,
+// B2 = cwP bP - cwM bM.
 int Sea::qform_meta(const QfTables** out) {
     *out = nullptr;
     if (qf_) { *out = qf_.get(); return 1; }
@@ -297,25 +298,29 @@ int Sea::qform_meta(const QfTables** out) {
     if (const char* e = std::getenv("CFDWM_QF_PRODUCTS")) nlv = std::atoi(e) == 1 ? 1 : 3;
     double tol = qf_tol > 0 ? qf_tol : (nlv == 1 ? 1e-4 : 1e-6);
     if (const char* e = std::getenv("CFDWM_QF_TOL")) tol = std::atof(e);
-    DepthBasis DB;
-    const int st_basis = build_depth_basis_cached(kmx * (1.0 + 1e-9) + 1e-12, 2.0 * h, tol, 32, DB);
-    const int R = DB.R;
+    ProfileBasis BC, BS;
+    const double kb = kmx * (1.0 + 1e-9) + 1e-12;
+    const int st_c = build_profile_basis_cached(kb, h, +1, tol, 32, BC);
+    const int st_s = build_profile_basis_cached(kb, h, -1, tol, 32, BS);
+    const int R = BC.R, Rw = BS.R;
     // no basis within the tolerance at rank <= 32: the fills take the pair loop
-    qf_fallback = st_basis != 0 || !(DB.err <= tol) || R < 1 || R > 32;
+    qf_fallback = st_c != 0 || st_s != 0 || !(BC.err <= tol) || !(BS.err <= tol) || R < 1 || R > 32 || Rw < 1 ||
+                  Rw > 32;
     if (qf_fallback) return 0;
-    const int nform = 1 + 3 * R;
+    const int nform = 1 + 2 * R + Rw;
     const int nbarr = nlv == 1 ? 2 : 4;
     const int npad = (N + 127) / 128 * 128;
     const size_t nn = (size_t)npad * npad;
     std::vector<int8_t> ftype(nform, 0);
-    for (int q = 0; q < R; ++q) ftype[1 + 2 * R + q] = 1;
+    for (int q = 0; q < Rw; ++q) ftype[1 + 2 * R + q] = 1;
     std::vector<int32_t> kend(npad / 64, 0);
     for (int i = 0; i < N; ++i) {
         int& ke = kend[i / 64];
         ke = std::max(ke, std::min(npad, (P.mhi[i] + 15) / 16 * 16));
     }
     auto T = std::make_unique<QfTables>();
-    T->npad = npad; T->nform = nform; T->rank = R; T->mu = DB.mu; T->basis_err = DB.err;
+    T->npad = npad; T->nform = nform; T->rank = R; T->rank_w = Rw; T->mu = BC.mu; T->mu_w = BS.mu;
+    T->basis_err = std::max(BC.err, BS.err);
     T->basis_tol = tol; T->nlv = nlv; T->nbarr = nbarr;
     T->kend_h = kend;
     for (size_t t = 0; t < kend.size(); ++t) T->cost += std::max(0, kend[t] - (int)(64 * t));
@@ -325,8 +330,10 @@ int Sea::qform_meta(const QfTables** out) {
     T->table_bytes = (size_t)(split ? nform - 1 : nform) * nbarr * nn * sizeof(uint16_t);
     T->eta_bytes = split ? 4 * nn * sizeof(uint16_t) : 0;
     hipStream_t st = Engine::get().stream();
-    if (T->tau.upload(DB.tau.data(), R * sizeof(double), st) ||
-        T->einv.upload(DB.einv.data(), (size_t)R * R * sizeof(double), st) ||
+    if (T->zs.upload(BC.zs.data(), R * sizeof(double), st) ||
+        T->finv.upload(BC.finv.data(), (size_t)R * R * sizeof(double), st) ||
+        T->zsw.upload(BS.zs.data(), Rw * sizeof(double), st) ||
+        T->finvw.upload(BS.finv.data(), (size_t)Rw * Rw * sizeof(double), st) ||
         T->ftype.upload(ftype.data(), ftype.size(), st) ||
         T->kend.upload(kend.data(), kend.size() * sizeof(int32_t), st))
         return -3;
@@ -343,9 +350,11 @@ int Sea::qform_build(QfTables& T, uint16_t* b, uint16_t* beta) {
     if (split) CFDWM_HIP(hipMemsetAsync(beta, 0, T.eta_bytes, st));
     CFDWM_HIP(hipMemsetAsync(b, 0, T.table_bytes, st));
     QfBuild Bq;
-    Bq.n = tab.comp.n; Bq.npad = T.npad; Bq.R = T.rank; Bq.nform = T.nform; Bq.nbarr = T.nbarr;
+    Bq.n = tab.comp.n; Bq.npad = T.npad; Bq.R = T.rank; Bq.Rw = T.rank_w; Bq.nform = T.nform; Bq.nbarr = T.nbarr;
+    Bq.h = tab.in.depth;
     Bq.frow = d32_.frow; Bq.fmhi = d32_.fmhi; Bq.uvw = d32_.fuvw; Bq.x = d32_.fx;
-    Bq.tau = (const double*)T.tau.get(); Bq.einv = (const double*)T.einv.get();
+    Bq.zs = (const double*)T.zs.get(); Bq.finv = (const double*)T.finv.get();
+    Bq.zsw = (const double*)T.zsw.get(); Bq.finvw = (const double*)T.finvw.get();
     Bq.b = b;
     Bq.beta = split ? beta : nullptr;
     if (Bq.n > 0) PROF("k_qf_build", launch_qf_build(Bq, st));
@@ -773,7 +782,7 @@ int Grid::qform_prepare(int nslot, const double* slot_t, int64_t ncol, const int
         qf_scale_[s] = r0;
     }
     const double hs = sea->tab.in.depth;
-    // the depth basis covers t = -z in [0, 2 hs] and t = 2 hs + z: the grid's levels must lie above -2 hs
+    // the profile bases cover z in [-2 hs, 0]: the grid's levels must lie above -2 hs
     qf_skip = QF_DEPTH;
     if (g.h > 2.0 * hs) {
         warn_pair_loop("the grid's water depth exceeds twice the sea's");
@@ -844,6 +853,7 @@ int Grid::qform_prepare(int nslot, const double* slot_t, int64_t ncol, const int
     qf_nlv = T->nlv;
     qf_nform = T->nform;
     qf_rank = R;
+    qf_rank_w = T->rank_w;
     qf_basis_err = T->basis_err;
     qf_flop = 0;
     return 1;
@@ -917,11 +927,16 @@ void Grid::qform_attach(ColumnJob<float>& C, LevelJob<float>& L) {
     for (int s = 0; s < 4; ++s) C.q2scale[s] = qf_scale_[s];
     L.qf = (const float*)qout_.get() + qj_.ostride;
     L.qf_rank = R;
+    L.qf_rank_w = qf_rank_w;
     L.qf_stride = qj_.ostride;
     constexpr double kL2e = 1.4426950408889634074;
     for (int q = 0; q < R; ++q) {
         L.qf_mu2[q] = (float)(T->mu[q] * kL2e);
         L.qf_mg[q] = (float)(-2.0 * hs * T->mu[q] * kL2e);
+    }
+    for (int q = 0; q < qf_rank_w; ++q) {
+        L.qf_mu2w[q] = (float)(T->mu_w[q] * kL2e);
+        L.qf_mgw[q] = (float)(-2.0 * hs * T->mu_w[q] * kL2e);
     }
     for (int s = 0; s < 4; ++s) L.qf_scale[s] = qf_scale_[s];
 }

@@ -52,12 +52,16 @@ struct LevelJob {
     uint32_t flags = 0;
     const uint32_t* lvl = nullptr;        // GRID: per filtered pair record, pruning level bounds
     // GRID: pair sums from the MFMA quadratic forms instead of the pair loop:
-    // u, v, w += scale * sum_q (e^{mu_q z} +- e^{-mu_q (z + 2h)}) Q_{f,q}(column)
-    const float* qf = nullptr;            // [3 * qf_rank][qf_stride]
-    int qf_rank = 0;
+    // u, v += scale * sum_q (e^{mu_q z} + e^{-mu_q (z + 2h)}) Q_{f,q}(column) (cosh-type
+    // basis, rank qf_rank), w += scale * sum_q (e^{nu_q z} - e^{-nu_q (z + 2h)}) Q_{w,q}
+    // (sinh-type basis, rank qf_rank_w)
+    const float* qf = nullptr;            // [2 qf_rank + qf_rank_w][qf_stride]: u, v, w forms
+    int qf_rank = 0, qf_rank_w = 0;
     int64_t qf_stride = 0;
     float qf_mu2[32] = {};                // mu_q log2 e
     float qf_mg[32] = {};                 // -2 h mu_q log2 e
+    float qf_mu2w[32] = {};               // nu_q log2 e
+    float qf_mgw[32] = {};                // -2 h nu_q log2 e
     double qf_scale[4] = {1, 1, 1, 1};
     const Segment* seg = nullptr;         // nseg segments of the table in use (blockIdx.y)
     int nseg = 1;
@@ -281,13 +285,13 @@ public:
         return direct ? (int64_t)tab.direct.rec.size() : (int64_t)tab.filt.rec.size();
     }
     // bf16 hi/lo tables of the second-order quadratic forms (built on first
-    // use): form 0 the filtered eta2, then per depth-basis rank q the u, v
-    // (cos-type) and w (sin-type) forms
+    // use): form 0 the filtered eta2, then the u and v forms (cos-type) of the
+    // cosh-type profile basis and the w forms (sin-type) of the sinh-type one
     struct QfTables {
-        int npad = 0, nform = 0, rank = 0;
+        int npad = 0, nform = 0, rank = 0, rank_w = 0;   // ranks of the cosh- and sinh-type profile bases
         int nlv = 3, nbarr = 4;              // bf16 products per term, table arrays per form
         double basis_err = 0, basis_tol = 0;
-        std::vector<double> mu;              // skeleton wavenumbers of the depth basis
+        std::vector<double> mu, mu_w;        // skeleton wavenumbers of the two profile bases
         // the tables: [nform][nbarr][npad x npad] tiled; with one product per
         // term (nlv 1) bp holds the 3 R velocity forms (hi parts) and betap the
         // eta2 form (hi and lo parts: the elevation keeps three products per
@@ -300,7 +304,7 @@ public:
         // kept while the tables come and go: sizes, basis, tile ends
         DevBuf ftype;                        // [nform] int8
         DevBuf kend;                         // per 64-wide tile of i
-        DevBuf tau, einv;                    // depth-basis skeleton depths and interpolation matrix
+        DevBuf zs, finv, zsw, finvw;         // the profile bases' skeleton depths and interpolation matrices
         std::vector<int32_t> kend_h;
         double cost = 0;                     // sum over i tiles of their m range (tile ordering)
     };
@@ -411,7 +415,7 @@ public:
     // explicit switch runs about 14x slower: say so once per grid on stderr
     void warn_pair_loop(const char* why);
     bool qf_warned_ = false;
-    int qf_nform = 0, qf_rank = 0, qf_nlv = 3;
+    int qf_nform = 0, qf_rank = 0, qf_rank_w = 0, qf_nlv = 3;
     double qf_flop = 0, qf_basis_err = 0;
     // shader clock of the forms launches of later fills (diagnostic)
     void qform_clock_enable(bool on) { qf_clock_on_ = on; }

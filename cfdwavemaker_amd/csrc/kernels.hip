@@ -486,11 +486,10 @@ __device__ __forceinline__ void levels_body(const SeaDev<R>& S, const LevelJob<R
 #pragma unroll
             for (int l = 0; l < LC; ++l) su[l] = sv[l] = sw[l] = 0;
             const int64_t cq = PP.cq;
-            const int Rk = J.qf_rank;
-            for (int q = 0; q < Rk; ++q) {
+            const int Rk = J.qf_rank, Rw = J.qf_rank_w;
+            for (int q = 0; q < Rk; ++q) {            // cosh-type basis: u, v
                 const R Qu = (R)J.qf[(int64_t)q * J.qf_stride + cq];
                 const R Qv = (R)J.qf[(int64_t)(Rk + q) * J.qf_stride + cq];
-                const R Qw = (R)J.qf[(int64_t)(2 * Rk + q) * J.qf_stride + cq];
                 const R m2 = (R)J.qf_mu2[q], mg = (R)J.qf_mg[q];
 #pragma unroll
                 for (int l = 0; l < LC; ++l) {
@@ -498,6 +497,15 @@ __device__ __forceinline__ void levels_body(const SeaDev<R>& S, const LevelJob<R
                     prof(m2, mg, zp[l], E, G);
                     su[l] = fmaR(Qu, E + G, su[l]);
                     sv[l] = fmaR(Qv, E + G, sv[l]);
+                }
+            }
+            for (int q = 0; q < Rw; ++q) {            // sinh-type basis: w
+                const R Qw = (R)J.qf[(int64_t)(2 * Rk + q) * J.qf_stride + cq];
+                const R m2 = (R)J.qf_mu2w[q], mg = (R)J.qf_mgw[q];
+#pragma unroll
+                for (int l = 0; l < LC; ++l) {
+                    R E, G;
+                    prof(m2, mg, zp[l], E, G);
                     sw[l] = fmaR(Qw, E - G, sw[l]);
                 }
             }

@@ -119,13 +119,17 @@ struct QfTile {
 
 // Device build of the form tables from the FP32 pair records (qform.hip).
 struct QfBuild {
-    int n = 0, npad = 0, R = 0, nform = 0;
+    int n = 0, npad = 0, nform = 0;
+    int R = 0, Rw = 0;                    // depth-basis ranks: cosh-type (u, v) and sinh-type (w) profiles
+    double h = 0;                         // sea depth of the profiles e^{kz} +- e^{-k (z + 2h)}
     const int64_t* frow = nullptr;        // filtered table rows
     const int32_t* fmhi = nullptr;
     const UvwRec<float>* uvw = nullptr;   // pk = kappa log2 e
     const XRec<float>* x = nullptr;
-    const double* tau = nullptr;          // [R] skeleton depths
-    const double* einv = nullptr;         // [R][R]
+    const double* zs = nullptr;           // [R] skeleton depths of the cosh-type basis
+    const double* finv = nullptr;         // [R][R]
+    const double* zsw = nullptr;          // [Rw] skeleton depths of the sinh-type basis
+    const double* finvw = nullptr;        // [Rw][Rw]
     uint16_t* b = nullptr;                // [nform][nbarr][npad x npad], zeroed by the caller
     int nbarr = 4;                        // 4: hi and lo parts, 2: hi parts only
     uint16_t* beta = nullptr;             // when set: form 0 (eta2) goes here as [4][npad x npad]

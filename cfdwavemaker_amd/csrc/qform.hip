@@ -369,10 +369,10 @@ void k_qf_forms_persist(const QfJob* __restrict__ jobs,
     }
 }
 
-// Form tables: one thread per (i, m) of the filtered pair set.  Depth-basis
-// coefficients c_q(kappa) = sum_s einv[q][s] e^{-kappa tau_s} in FP64 (einv is
-// ill-conditioned: its entries reach 1e8 while |c| <= 1.4), entries written
-// as bf16 hi + lo into the tiled layout.
+// Form tables: one thread per (i, m) of the filtered pair set.  Profile-basis
+// coefficients c_q(kappa) = sum_s finv[q][s] P_kappa(z_s) in FP64 (finv is
+// ill-conditioned: its entries are large while |c| stays O(1)), entries
+// written as bf16 hi + lo into the tiled layout.
 __global__ __launch_bounds__(256) void k_qf_build(QfBuild B) {
     const int i = blockIdx.y;
     const int m = blockIdx.x * blockDim.x + threadIdx.x;
@@ -382,12 +382,6 @@ __global__ __launch_bounds__(256) void k_qf_build(QfBuild B) {
     const XRec<float> xr = B.x[e];
     constexpr double kL2eInv = 0.69314718055994530942;   // ln 2: kappa = pk / log2 e
     const double kp = (double)r.pkP * kL2eInv, km = (double)r.pkM * kL2eInv;
-    double eP[32], eM[32];
-#pragma unroll
-    for (int q = 0; q < 32; ++q) {
-        eP[q] = q < B.R ? exp(-kp * B.tau[q]) : 0.0;
-        eM[q] = q < B.R ? exp(-km * B.tau[q]) : 0.0;
-    }
     const int nkb = B.npad >> 4;
     const size_t nn = (size_t)B.npad * B.npad;
     const size_t o = ((size_t)(i >> 5) * nkb + (m >> 4)) * 512 + ((m >> 3) & 1) * 256 + (i & 31) * 8 + (m & 7);
@@ -403,30 +397,46 @@ __global__ __launch_bounds__(256) void k_qf_build(QfBuild B) {
     };
     put(0, 0, (double)xr.xP + (double)xr.xM);
     put(0, 1, (double)xr.xM - (double)xr.xP);
-    const int R = B.R;
-    for (int q = 0; q < R; ++q) {
-        double cp = 0, cm = 0;
+    // profile P_kappa(z) = e^{kappa z} + sg e^{-kappa (z + 2h)} at the
+    // skeleton depths, then its basis coefficients c = finv P (one family at a time)
+    double eP[32], eM[32];
+    auto family = [&](int R, const double* zs, const double* finv, double sg, int f0, int nf) {
 #pragma unroll
-        for (int s2 = 0; s2 < 32; ++s2) {
-            if (s2 < R) {
-                const double w = B.einv[q * R + s2];
-                cp += w * eP[s2];
-                cm += w * eM[s2];
+        for (int q = 0; q < 32; ++q) {
+            const double z = q < R ? zs[q] : 0.0;
+            eP[q] = q < R ? exp(kp * z) + sg * exp(-kp * (z + 2.0 * B.h)) : 0.0;
+            eM[q] = q < R ? exp(km * z) + sg * exp(-km * (z + 2.0 * B.h)) : 0.0;
+        }
+        for (int q = 0; q < R; ++q) {
+            double cp = 0, cm = 0;
+#pragma unroll
+            for (int s2 = 0; s2 < 32; ++s2) {
+                if (s2 < R) {
+                    const double w = finv[q * R + s2];
+                    cp += w * eP[s2];
+                    cm += w * eM[s2];
+                }
+            }
+            if (nf == 2) {                            // u, v (cos-type)
+                put(f0 + q, 0, r.cuP * cp + r.cuM * cm);
+                put(f0 + q, 1, r.cuM * cm - r.cuP * cp);
+                put(f0 + R + q, 0, r.cvP * cp + r.cvM * cm);
+                put(f0 + R + q, 1, r.cvM * cm - r.cvP * cp);
+            } else {                                  // w (sin-type)
+                put(f0 + q, 0, r.cwP * cp + r.cwM * cm);
+                put(f0 + q, 1, r.cwP * cp - r.cwM * cm);
             }
         }
-        put(1 + q, 0, r.cuP * cp + r.cuM * cm);
-        put(1 + q, 1, r.cuM * cm - r.cuP * cp);
-        put(1 + R + q, 0, r.cvP * cp + r.cvM * cm);
-        put(1 + R + q, 1, r.cvM * cm - r.cvP * cp);
-        put(1 + 2 * R + q, 0, r.cwP * cp + r.cwM * cm);
-        put(1 + 2 * R + q, 1, r.cwP * cp - r.cwM * cm);
-    }
+    };
+    family(B.R, B.zs, B.finv, 1.0, 1, 2);
+    family(B.Rw, B.zsw, B.finvw, -1.0, 1 + 2 * B.R, 1);
 }
 
 }  // namespace
 
 hipError_t launch_qf_build(const QfBuild& B, hipStream_t st) {
-    if (B.n <= 0 || B.R < 0 || B.R > 32 || (B.npad & 127) || B.npad < B.n || (B.nbarr != 2 && B.nbarr != 4))
+    if (B.n <= 0 || B.R < 0 || B.R > 32 || B.Rw < 0 || B.Rw > 32 || B.nform != 1 + 2 * B.R + B.Rw ||
+        (B.npad & 127) || B.npad < B.n || (B.nbarr != 2 && B.nbarr != 4))
         return hipErrorInvalidValue;
     hipLaunchKernelGGL(k_qf_build, dim3((unsigned)((B.npad + 255) / 256), (unsigned)B.n), dim3(256), 0, st, B);
     return hipGetLastError();

@@ -372,6 +372,22 @@ def depth_basis(kmax, tmax, tol=1e-6, rmax=32):
     return dict(rank=k, mu=mu[:k].copy(), tau=tau[:k].copy(), err=err.value, ok=r > 0)
 
 
+def profile_basis(kmax, h, sign, tol=1e-6, rmax=32):
+    """Depth basis of the pair-term profiles e^{kz} + sign e^{-k(z+2h)} on
+    z in [-2h, 0], skeleton depths in [-h, 0] (host only): dict(rank, mu, zs,
+    err, ok)."""
+    L = load()
+    mu = np.zeros(rmax)
+    zs = np.zeros(rmax)
+    err = C.c_double()
+    r = L.cfdwm_profile_basis(float(kmax), float(h), int(sign), float(tol), int(rmax), _p(mu), _p(zs),
+                              C.byref(err))
+    if r == 0:
+        raise ValueError("cfdwm_profile_basis: invalid arguments")
+    k = abs(r)
+    return dict(rank=k, mu=mu[:k].copy(), zs=zs[:k].copy(), err=err.value, ok=r > 0)
+
+
 def refill_batched(grids, budget_gb=96.0, release=True):
     """Refill many grids with their second-order forms in persistent launches
     over all of them (autotuned launch shape); returns the batch statistics."""

@@ -354,6 +354,17 @@ CFDWM_EXPORT int cfdwm_grids_refill_batched(cfdwm_grid** grids, int n, double bu
  * error.  Returns the rank, or -(rank) when tol was not reached. */
 CFDWM_EXPORT int cfdwm_depth_basis(double kmax, double tmax, double tol, int rmax, double* mu, double* tau,
                                    double* err);
+/* The depth bases the s-grid fills use: the pair terms' whole profiles
+ * P_kappa(z) = e^{kappa z} + sign e^{-kappa (z + 2h)} (sign +1: cosh-type, u and
+ * v; -1: sinh-type, w) over kappa in [0, kmax], z in [-2h, 0], approximated as
+ * sum_q c_q(kappa) P_{mu_q}(z) with c = F^{-1} [P_kappa(z_s)], F[s][q] =
+ * P_{mu_q}(z_s) (host only).  P_kappa(-2h - z) = sign P_kappa(z), so the
+ * construction samples z in [-h, 0].  mu and zs (capacity rmax) receive the
+ * skeleton wavenumbers and depths, err the measured max error.  Returns the
+ * rank, -(rank) when tol was not reached (the best basis found is returned),
+ * 0 for invalid arguments. */
+CFDWM_EXPORT int cfdwm_profile_basis(double kmax, double h, int sign, double tol, int rmax, double* mu, double* zs,
+                                     double* err);
 /* ---- Learned second-order surrogate: per-column MLP (no reference
  * counterpart; the exact kernels above stay the fallback and the judge).
  * L = nhidden + 1 linear layers (nin -> width -> ... -> nout), GELU (erf

@@ -57,7 +57,7 @@ def test_batched_full_grids_vs_oracle(oracle, prec_name):
     their full 64 x 64 x 8 s-grids, deferred at creation and filled by one
     batched call, against the FP64 oracle's fill of the four corner columns
     (all levels and time levels).  Bounds: FP32 2e-5 of the field scale; BF16
-    the elevation gate 1e-3 Hs and 2e-3 of the field scale for velocities."""
+    the elevation gate 1e-3 Hs and 3e-3 of the field scale for velocities."""
     from cfdwavemaker_amd import engine as eng
     from cfdwavemaker_amd import surrogate as S
     from cfdwavemaker_amd.seastate import config_sea
@@ -87,7 +87,7 @@ def test_batched_full_grids_vs_oracle(oracle, prec_name):
                 continue
             err = scaled_err(got, ref)
             print(f"{prec_name} N={len(sea.omega)} ndir {sea.ndir} {f} err {err:.2e}", flush=True)
-            assert err <= (2e-5 if prec_name == "fp32" else 2e-3), (len(sea.omega), f, err)
+            assert err <= (2e-5 if prec_name == "fp32" else 3e-3), (len(sea.omega), f, err)
 
 
 def test_deferred_grid_refuses_queries():

@@ -6,7 +6,10 @@ Stated bounds (scale-normalised error |err| / max|ref|):
   FP32 (FP64 phase reduction, three-product bf16 quadratic forms): 2e-5 for
       grid fields, queries and ABI outputs;
   BF16 (one bf16 product per quadratic-form term): the elevation gate
-      |eta err| <= 1e-3 Hs, and 2e-3 of the field scale for velocities.
+      |eta err| <= 1e-3 Hs, and 3e-3 of the field scale for velocities (the
+      rounding of one bf16 product: 0.8e-3 to 2.4e-3 on these blocks,
+      depending on the depth basis; with three products the same fills
+      stay within 2e-5, so the basis itself contributes no more).
 """
 import os
 
@@ -20,7 +23,7 @@ from tests.test_abi import run_program
 pytestmark = pytest.mark.gpu
 
 FP32_TOL = 2e-5
-BF16_VEL_TOL = 2e-3
+BF16_VEL_TOL = 3e-3
 
 
 @pytest.fixture(scope="module")

@@ -7,7 +7,7 @@ properties the interpolation fixes plus the oracle's interpolation on the
 GPU's fields.
 
 Stated bounds (scale-normalised error |err| / max|ref|): FP64 1e-9; FP32
-(three-product forms) 2e-5; BF16 (one-product velocity forms) 2e-3 for
+(three-product forms) 2e-5; BF16 (one-product velocity forms) 3e-3 for
 velocities and the elevation gate 1e-3 Hs.
 """
 import os
@@ -24,7 +24,7 @@ pytestmark = pytest.mark.gpu
 
 FP64_TOL = 1e-9
 FP32_TOL = 2e-5
-BF16_VEL_TOL = 2e-3
+BF16_VEL_TOL = 3e-3
 FIELDS = ("ETA", "UX", "UY", "UZ")
 
 
