@@ -844,10 +844,14 @@ int Grid::qform_prepare(int nslot, const double* slot_t, int64_t ncol, const int
         V.nform = T->nform - 1;
         V.out = Q.out + ppad_all;
     }
-    // executed MFMA flops per padded point and form: per 64-wide i tile,
-    // 64 x (kend - i0) products, two operand pairs (c and s), 3 or 1 bf16 products
+    // executed MFMA flops per padded point and form: a 128-wide i tile runs
+    // both 64-row halves from its first row to the larger of their two kend,
+    // 128 x (kend - i0) products, two operand pairs (c and s), 3 or 1 bf16 products
     double tri = 0;
-    for (size_t t = 0; t < T->kend_h.size(); ++t) tri += 64.0 * std::max(0, T->kend_h[t] - (int)(64 * t)) * 2 * 2;
+    for (size_t t = 0; t + 1 < T->kend_h.size(); t += 2) {
+        const int ke = std::max(T->kend_h[t], T->kend_h[t + 1]);
+        tri += 128.0 * std::max(0, ke - (int)(64 * t)) * 2 * 2;
+    }
     qf_fpp_ = 3 * tri;
     qf_fpp1_ = tri;
     qf_nlv = T->nlv;
