@@ -296,7 +296,10 @@ int Sea::qform_meta(const QfTables** out) {
     for (const PairRec64& r : P.rec) kmx = std::max(kmx, std::max(std::fabs(r.kP), std::fabs(r.kM)));
     int nlv = qf_products == 1 ? 1 : 3;
     if (const char* e = std::getenv("CFDWM_QF_PRODUCTS")) nlv = std::atoi(e) == 1 ? 1 : 3;
-    double tol = qf_tol > 0 ? qf_tol : (nlv == 1 ? 1e-4 : 1e-6);
+    // The bases' error reaches the fields at about a fifth of the tolerance
+    // (relative to the field scale): 1e-6 keeps the FP32 mode's ~1e-6; at one
+    // bf16 product per term the rounding gives ~1e-3 and 1e-3 adds ~2e-4 to it
+    double tol = qf_tol > 0 ? qf_tol : (nlv == 1 ? 1e-3 : 1e-6);
     if (const char* e = std::getenv("CFDWM_QF_TOL")) tol = std::atof(e);
     ProfileBasis BC, BS;
     const double kb = kmx * (1.0 + 1e-9) + 1e-12;

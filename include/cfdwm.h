@@ -146,8 +146,8 @@ typedef struct {
     int bw_auto;      /* Irregular::bandwidth_estimator */
     int precision;    /* 0 FP64, 1 FP32 (FP64 phase reduction), 2 FP32 whose MFMA
                          velocity forms take one bf16 product per term instead
-                         of three on a coarser depth basis (tolerance 1e-4;
-                         velocity errors ~3e-4 to 4e-3 of the field scale at
+                         of three on coarser depth bases (tolerance 1e-3;
+                         velocity errors ~3e-4 to 3e-3 of the field scale at
                          N=1024); the elevation form keeps three products
                          (error ~1e-6 Hs) */
     int with_direct;  /* also build the unfiltered table of u2/v2/w2/dp2 */
