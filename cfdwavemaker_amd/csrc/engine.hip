@@ -158,7 +158,7 @@ int Sea::upload_t(bool with_direct) {
     const double h = tab.in.depth;
     std::vector<PhaseRec> ph(N);
     std::vector<CompRec<R>> cr(N);
-    std::vector<uint8_t> act(N);
+    std::vector<uint8_t> act((N + 3) / 4 * 4, 0);     // whole 4-byte words (read as such)
     for (int i = 0; i < N; ++i) {
         ph[i] = PhaseRec{c.k[i], c.cth[i], c.sth[i], c.omega[i], c.ph0[i], c.rt[i], c.ts[i], c.du[i]};
         cr[i] = comp_rec<R>(c, i, h);

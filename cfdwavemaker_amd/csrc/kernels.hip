@@ -27,6 +27,30 @@ using dm::phase_of;
 
 // ---------------------------------------------------------------- helpers
 
+// Component records are read-only for a launch's whole duration: loads
+// through the constant address space let the compiler use scalar loads for
+// the wave-uniform component index (vector loads of one address otherwise).
+template <typename T>
+__device__ __forceinline__ T ld_const(const T* p, int64_t i) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    return ((const __attribute__((address_space(4))) T*)p)[i];
+#else
+    return p[i];                              // host pass: never executed
+#endif
+}
+// the same for the FP32 kernels; the FP64 ones keep vector loads (their
+// records would take more scalar registers than they have left)
+template <typename R, typename T>
+__device__ __forceinline__ T ld_rec(const T* p, int64_t i) {
+    if constexpr (sizeof(R) == 4) return ld_const(p, i);
+    else return p[i];
+}
+// rowActive[i] from its 4-byte word (the table is padded to whole words;
+// scalar loads take no single bytes)
+__device__ __forceinline__ bool row_active(const uint8_t* a, int i) {
+    return ((ld_const((const uint32_t*)a, i >> 2) >> (8 * (i & 3))) & 0xffu) != 0;
+}
+
 // depth profile pieces: E = e^{kz}, G = e^{-k(z+2h)}; cosh ratio = C(E+G), sinh ratio = C(E-G)
 __device__ __forceinline__ void prof(double pk, double pg, double z, double& E, double& G) {
     E = exp(pk * z);
@@ -107,15 +131,23 @@ __global__ __launch_bounds__(256) void k_columns(SeaDev<R> S, ColumnJob<R> J) {
     const double dxp = P.x - S.x0, dyp = P.y - S.y0, t = P.t;
     double v[6] = {0, 0, 0, 0, 0, 0};     // e1 e2 gx gy gz gp: FP64 totals, rows in R
     const int ia = min(sg.rlo, sg.own_lo), ib = max(sg.rhi, sg.own_hi);
+    // with the eta2 form (J.q2) every component's frequency ramp takes one
+    // value at the column's time (checked on the host): one evaluation
+    // serves all components, bit for bit
+    R ri_q = 0;
+    if (sizeof(R) == 4 && J.q2) {
+        const PhaseRec p0 = ld_rec<R>(S.ph, 0);
+        ri_q = (R)freq_ramp(t, p0.rt, p0.ts, p0.du);
+    }
     for (int i = ia; i < ib; ++i) {
         const bool own = i >= sg.own_lo && i < sg.own_hi;
         const bool prs = J.want_eta2 && J.pairs && i >= sg.rlo && i < sg.rhi;
         if (!own && !prs) continue;
-        const PhaseRec pi = S.ph[i];
+        const PhaseRec pi = ld_rec<R>(S.ph, i);
         const CompRec<R>& ci_ = S.cr[i];
         R ci, si;
         cs_of(phase_of(pi, dxp, dyp, t), ci, si);
-        const R ri = (R)freq_ramp(t, pi.rt, pi.ts, pi.du);
+        const R ri = sizeof(R) == 4 && J.q2 ? ri_q : (R)freq_ramp(t, pi.rt, pi.ts, pi.du);
         const R cr = ci * ri, sr = si * ri;
         if (own) {
             v[0] += (double)(ci_.A * cr);
@@ -312,16 +344,24 @@ __device__ __forceinline__ void levels_body(const SeaDev<R>& S, const LevelJob<R
 #pragma unroll
     for (int l = 0; l < LC; ++l) ur[l] = vr[l] = wr[l] = 0;
     int nfold = 0;
+    // form-assisted fills run only when every component's frequency ramp
+    // takes one value at each slot time (checked on the host), so one
+    // evaluation per thread serves all components, bit for bit
+    R ri_f8 = 0;
+    if constexpr (F8) {
+        const PhaseRec p0 = ld_rec<R>(S.ph, 0);
+        ri_f8 = (R)freq_ramp(t, p0.rt, p0.ts, p0.du);
+    }
     for (int i = ia; i < ib; ++i) {
         const bool own = i >= sg.own_lo && i < sg.own_hi;
-        const bool sec = secf && (direct || S.rowActive[i]);
+        const bool sec = secf && (direct || (sizeof(R) == 4 ? row_active(S.rowActive, i) : S.rowActive[i] != 0));
         const bool prs = sec && !(GRID && J.qf) && i >= sg.rlo && i < sg.rhi;
         if (!own && !prs) continue;
-        const PhaseRec pi = S.ph[i];
-        const CompRec<R> c = S.cr[i];
+        const PhaseRec pi = ld_rec<R>(S.ph, i);
+        const CompRec<R> c = ld_rec<R>(S.cr, i);
         R ci, si;
         cs_of(phase_of(pi, dxp, dyp, t), ci, si);
-        const R ri = (R)freq_ramp(t, pi.rt, pi.ts, pi.du);
+        const R ri = F8 ? ri_f8 : (R)freq_ramp(t, pi.rt, pi.ts, pi.du);
         const R cr = ci * ri, sr = si * ri;
         R pr = 0;
         if constexpr (fold_n == 1) {
