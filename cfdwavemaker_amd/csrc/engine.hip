@@ -81,16 +81,20 @@ hipEvent_t Engine::take_event() {
     if (hipEventCreate(&e) != hipSuccess) return nullptr;
     return e;
 }
-void Engine::prof_begin() {
+hipStream_t Engine::side_stream() {
+    if (!side_ && hipStreamCreateWithFlags(&side_, hipStreamNonBlocking) != hipSuccess) side_ = nullptr;
+    return side_;
+}
+void Engine::prof_begin(hipStream_t st) {
     if (!prof_) return;
     cur_ = take_event();
-    if (cur_) (void)hipEventRecord(cur_, stream_);
+    if (cur_) (void)hipEventRecord(cur_, st ? st : stream_);
 }
-void Engine::prof_end(const char* name) {
+void Engine::prof_end(const char* name, hipStream_t st) {
     if (!prof_ || !cur_) return;
     hipEvent_t e = take_event();
     if (!e) return;
-    (void)hipEventRecord(e, stream_);
+    (void)hipEventRecord(e, st ? st : stream_);
     pending_.push_back({name, {cur_, e}});
     cur_ = nullptr;
 }
@@ -618,7 +622,18 @@ int Sea::eval_points(int64_t n, const double* x, const double* y, const double* 
 
 // ------------------------------------------------------------------ Grid
 Grid::Grid(Sea* s, const GridDesc& desc) : sea(s), d(desc), prec(s->prec) {}
-Grid::~Grid() = default;
+Grid::~Grid() {
+    qform_join();
+    if (qf_fork_) (void)hipEventDestroy(qf_fork_);
+    if (qf_join_) (void)hipEventDestroy(qf_join_);
+}
+
+// the main stream waits for the velocity forms on the side stream
+void Grid::qform_join() {
+    if (!qf_join_pending_) return;
+    (void)hipStreamWaitEvent(Engine::get().stream(), qf_join_, 0);
+    qf_join_pending_ = false;
+}
 
 int64_t Grid::field_count(int which) const {
     const int64_t c = (int64_t)g.nx * g.ny, v = c * g.nl;
@@ -684,6 +699,7 @@ int Grid::init(bool fill) {
     }
     if (const char* e = std::getenv("CFDWM_PRUNE")) prune_ = std::atoi(e) != 0;
     if (const char* e = std::getenv("CFDWM_QFORM")) qform_ = std::atoi(e) != 0;
+    if (const char* e = std::getenv("CFDWM_QF_OVERLAP")) qf_overlap_ = std::atoi(e) != 0;
     return fill ? refill() : 0;
 }
 
@@ -714,6 +730,11 @@ int Grid::fill_slots(int nslot, const double* slot_t, const int* slot_dst, bool 
     C.nseg = pick_nseg(C.ncol * nslot, nrec);
     LevelJob<R> L;
     bool qf_used = false;
+    // every return below joins the side stream's velocity forms first
+    struct Join {
+        Grid* g;
+        ~Join() { g->qform_join(); }
+    } join{this};
     if constexpr (sizeof(R) == 4) {
         const int rc = qform_fill(nslot, slot_t, C.ncol, C.cols, C, L);
         if (rc < 0) return rc;
@@ -735,6 +756,7 @@ int Grid::fill_slots(int nslot, const double* slot_t, const int* slot_dst, bool 
     if (C.nseg > 1) C.partial = (double*)partial_.get();
     // initialize_kinematics: ETA = eta1 + eta2 + swl (lsgrid_spline.cpp:700)
     PROF("k_columns_grid", launch_columns<R>(S, C, true, st));
+    qform_join();
     L.nslot = nslot; L.ncol = C.ncol; L.ncell = ncell; L.cols = C.cols; L.g = g;
     for (int s = 0; s < nslot; ++s) { L.slot_t[s] = slot_t[s]; L.slot_dst[s] = slot_dst[s]; }
     L.eta_grid = (const R*)f_[0].get();
@@ -892,10 +914,26 @@ int Grid::qform_run() {
         Q.ppad = (Q.P + 127) / 128 * 128;
         PROF("k_qf_zgen", launch_qf_zgen(Q, st));
         if (qf_split_) {
-            // eta2 at three products, then the velocity forms at one
+            // eta2 at three products, then the velocity forms at one.  With
+            // one chunk the velocity forms go to the side stream (they need
+            // only the Z operands), so that the column pass, which needs the
+            // eta2 form, runs beside them; the level pass joins them
+            // (qform_join), and so does anything later on the main stream.
             V.p0 = Q.p0; V.P = Q.P; V.ppad = Q.ppad;
             PROF("k_qf_forms_eta", launch_qf_forms(Q, st));
-            PROF("k_qf_forms", launch_qf_forms(V, st));
+            hipStream_t s2 = qf_overlap_ && qf_P_ <= qf_chunk_ ? Engine::get().side_stream() : nullptr;
+            if (s2 && !qf_fork_ && (hipEventCreateWithFlags(&qf_fork_, hipEventDisableTiming) != hipSuccess ||
+                                    hipEventCreateWithFlags(&qf_join_, hipEventDisableTiming) != hipSuccess))
+                s2 = nullptr;
+            if (s2 && qf_fork_ && qf_join_) {
+                CFDWM_HIP(hipEventRecord(qf_fork_, st));
+                CFDWM_HIP(hipStreamWaitEvent(s2, qf_fork_, 0));
+                PROF_ON(s2, "k_qf_forms", launch_qf_forms(V, s2));
+                CFDWM_HIP(hipEventRecord(qf_join_, s2));
+                qf_join_pending_ = true;
+            } else {
+                PROF("k_qf_forms", launch_qf_forms(V, st));
+            }
             qf_flop += (qf_fpp_ * Q.nform + qf_fpp1_ * V.nform) * (double)Q.ppad;
         } else {
             PROF("k_qf_forms", launch_qf_forms(Q, st));

@@ -233,16 +233,19 @@ public:
     static Engine& get();
     int init();                              // device + stream, idempotent
     hipStream_t stream() const { return stream_; }
+    // a second stream (default priority) for work that overlaps the main
+    // stream's, forked and joined with events; created on first use
+    hipStream_t side_stream();
     // timing: record an event pair around a launch when enabled
     void prof_enable(bool on) { prof_ = on; }
     bool prof_on() const { return prof_; }
-    void prof_begin();
-    void prof_end(const char* name);
+    void prof_begin(hipStream_t st = nullptr);
+    void prof_end(const char* name, hipStream_t st = nullptr);
     int prof_flush();                        // resolves pending events
     std::map<std::string, KernelTimes> prof_read(bool reset);
 private:
     Engine() = default;
-    hipStream_t stream_ = nullptr;
+    hipStream_t stream_ = nullptr, side_ = nullptr;
     bool ready_ = false, prof_ = false;
     std::vector<std::pair<std::string, std::pair<hipEvent_t, hipEvent_t>>> pending_;
     hipEvent_t cur_ = nullptr;
@@ -257,6 +260,14 @@ private:
         Engine::get().prof_begin();         \
         hipError_t pe_ = (call);            \
         Engine::get().prof_end(name);       \
+        CFDWM_HIP(pe_);                     \
+    } while (0)
+// the same for a launch on another stream
+#define PROF_ON(st, name, call)             \
+    do {                                    \
+        Engine::get().prof_begin(st);       \
+        hipError_t pe_ = (call);            \
+        Engine::get().prof_end(name, st);   \
         CFDWM_HIP(pe_);                     \
     } while (0)
 
@@ -437,6 +448,7 @@ private:
     DevBuf lvl_, eb_, partial_;
     bool prune_ = true;
     bool qform_ = true;                       // FP32: eta2 pair sum as an MFMA quadratic form
+    bool qf_overlap_ = true;                  // velocity forms beside the column pass (CFDWM_QF_OVERLAP=0: off)
     DevBuf zbuf_;                             // its Z operands (one chunk of points)
     uint16_t* zext_ = nullptr;                // or caller memory (a batched refill's arena)
     size_t zext_bytes_ = 0;
@@ -456,6 +468,11 @@ private:
     DevBuf qf_clk_;                           // 2 counters of the clock measurement
     double qf_fpp_ = 0, qf_scale_[4] = {1, 1, 1, 1};
     bool batch_pending_ = false;
+    // the velocity forms of a one-chunk split fill run on the side stream,
+    // beside the column pass; the level pass joins them
+    hipEvent_t qf_fork_ = nullptr, qf_join_ = nullptr;
+    bool qf_join_pending_ = false;
+    void qform_join();
     int qform_prepare(int nslot, const double* slot_t, int64_t ncol, const int32_t* cols);
     int qform_run();
     void qform_attach(ColumnJob<float>& C, LevelJob<float>& L);
