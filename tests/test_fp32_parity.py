@@ -158,3 +158,24 @@ def test_bf16_mode_reports_one_product(eng):
                                                                 0.7, 1.5, 100.0)
         info = g.qform_info()
         assert info is not None and info["products"] == products, info
+
+
+def test_bf16_side_stream_fill_equals_serial_fill(eng, monkeypatch):
+    """The velocity forms of a one-chunk bf16 fill run on a second stream
+    beside the column pass and the level pass joins them: fills and updates
+    equal those of the serial order (CFDWM_QF_OVERLAP=0) bit for bit."""
+    from cfdwavemaker_amd.seastate import config_sea
+    s = config_sea(3).as_dict()
+    dx = 1000.0 / 255
+    dom = (50 * dx, 65 * dx, 20 * dx, 35 * dx)
+    out = {}
+    for ov in ("1", "0"):
+        monkeypatch.setenv("CFDWM_QF_OVERLAP", ov)
+        g = eng.Sea(s, precision=eng.BF16, with_direct=False).grid(dom, 16, 16, 16, 0.0, 0.5, 0.7, 1.5, 100.0)
+        assert g.qform_info() is not None
+        res = [g.export(f) for f in ("ETA", "UX", "UY", "UZ", "Udt")]
+        g.force_update(0.55)
+        res += [g.export(f) for f in ("ETA", "UX", "UY", "UZ", "Udt")]
+        out[ov] = res
+    for a, b in zip(out["1"], out["0"]):
+        assert np.array_equal(a, b)
