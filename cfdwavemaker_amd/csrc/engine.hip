@@ -138,6 +138,7 @@ CompRec<R> comp_rec(const CompTable& c, int i, double h) {
     r.pg = (R)(-2.0 * h * c.k[i] * s);
     r.p2k = (R)(2.0 * c.k[i] * s);
     r.p2g = (R)(-4.0 * h * c.k[i] * s);
+    r.eg2 = (R)(2.0 * std::exp(-2.0 * h * c.k[i]));
     return r;
 }
 

@@ -403,14 +403,19 @@ __device__ __forceinline__ void levels_body(const SeaDev<R>& S, const LevelJob<R
                 const R du = direct ? c.ddu : c.dfu, dv = direct ? c.ddv : c.dfv, dw = direct ? c.ddw : c.dfw;
 #pragma unroll
                 for (int l = 0; l < LC; ++l) {
-                    R E, G;
+                    R ch, sh;
                     if constexpr (SQ) {
-                        E = Ek[l] * Ek[l];
-                        G = Gk[l] * Gk[l];
+                        // E^2 + G^2 = (E + G)^2 - 2 E G with E G = e^{-2kh} per
+                        // component, E^2 - G^2 = (E + G)(E - G)
+                        const R c1 = Ek[l] + Gk[l], s1 = Ek[l] - Gk[l];
+                        ch = fmaR(c1, c1, -c.eg2);
+                        sh = c1 * s1;
                     } else {
+                        R E, G;
                         prof(c.p2k, c.p2g, zp[l], E, G);
+                        ch = E + G;
+                        sh = E - G;
                     }
-                    const R ch = E + G, sh = E - G;
                     if constexpr (SQ) {
                         ur[l] += du * c2 * ch;
                         vr[l] += dv * c2 * ch;

@@ -26,7 +26,8 @@ struct CompRec {
     R ddu, ddv, ddw, ddp;      // unfiltered (u2..dp2) diagonal
     R Xx, Xy, Zk, kd1, kd2;    // dp2 kinetic terms
     R pk, pg, p2k, p2g;        // profile exponents for k and 2k
-    R pad[2];
+    R eg2;                     // 2 E G = 2 e^{-2kh}: the 2k profile's E^2 + G^2 is (E + G)^2 - eg2
+    R pad;
 };
 
 // Per-pair velocity record (sum branch P, difference branch M).
