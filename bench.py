@@ -774,9 +774,9 @@ def main():
         # form, over the 4 x 256 x 256 column-slots of a fill: u, v, w (24 N^2)
         # for the velocity launch, eta2 + u, v, w (32 N^2) for the forms step
         # (the 40 N^2 of five forms counts a pressure form the s-grid does not
-        # keep).  The kernel executes more: every form of the depth basis
-        # (3 R velocity forms), nlv bf16 products per term and the pair triangle
-        # in tiles (exec_flop_per_launch)
+        # keep).  The kernel executes more: every form of the depth bases
+        # (2 R + R_w velocity forms), nlv bf16 products per term and the pair
+        # triangle in tiles (exec_flop_per_launch)
         ncomp = len(s["omega"])
         cs = 4 * NX * NY
         f_ms, f_n = prof.get("k_qf_forms", (float("nan"), 1))

@@ -780,8 +780,8 @@ int Grid::fill_slots(int nslot, const double* slot_t, const int* slot_dst, bool 
 }
 
 // Second-order pair sums of an FP32 fill as MFMA quadratic forms: the
-// filtered eta2 for the column pass and, through the separable depth basis,
-// the u/v/w pair sums for the level pass.  qform_prepare checks
+// filtered eta2 for the column pass and, through the two profile depth
+// bases, the u/v/w pair sums for the level pass.  qform_prepare checks
 // applicability (switched on, pairs present, frequency ramps equal for all
 // components at each slot time, levels inside the basis range), builds the
 // sea's tables once and sizes the workspace; returns 1 when the forms apply,

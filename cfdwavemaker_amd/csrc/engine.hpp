@@ -304,7 +304,7 @@ public:
         double basis_err = 0, basis_tol = 0;
         std::vector<double> mu, mu_w;        // skeleton wavenumbers of the two profile bases
         // the tables: [nform][nbarr][npad x npad] tiled; with one product per
-        // term (nlv 1) bp holds the 3 R velocity forms (hi parts) and betap the
+        // term (nlv 1) bp holds the 2 R + R_w velocity forms (hi parts) and betap the
         // eta2 form (hi and lo parts: the elevation keeps three products per
         // term).  They live in b / beta, or in caller memory (a batched
         // refill's arena); null while not built.

@@ -525,7 +525,7 @@ __device__ __forceinline__ void levels_body(const SeaDev<R>& S, const LevelJob<R
     }
     if constexpr (GRID) {
         if (J.qf && blockIdx.y == 0) {
-            // pair sums from the quadratic forms of the separable depth basis
+            // pair sums from the quadratic forms of the profile depth bases
             // (once per point: segment 0)
             R su[LC], sv[LC], sw[LC];
 #pragma unroll

@@ -17,9 +17,10 @@
 // The outer sum over i is a fused epilogue (accumulator times c or s, summed
 // in the lane, two exchanges at the end); the column kernel folds the result into
 // its FP64 total.  The same kernel evaluates the z-independent forms of the
-// velocity pair sums under the separable depth basis (depth_basis.hpp): per
-// skeleton wavenumber mu_q one cos-type form for u and v and one sin-type
-// form for w, with B entries cuP c_q(kP) +- cuM c_q(kM) (and cv, cw).
+// velocity pair sums under the two profile depth bases (depth_basis.hpp): per
+// skeleton wavenumber mu_q of the cosh-type basis one cos-type form each for
+// u and v, per skeleton wavenumber of the sinh-type basis one sin-type form
+// for w, with B entries cuP c_q(kP) +- cuM c_q(kM) (and cv, cw).
 //
 // The per-row ramp r_i must be the same for every i at a given time (the
 // usual case: all ramps finished, or all components ramped alike); the host
