@@ -329,6 +329,49 @@ int Sea::qform_meta(const QfTables** out) {
     auto T = std::make_unique<QfTables>();
     T->npad = npad; T->nform = nform; T->rank = R; T->rank_w = Rw; T->mu = BC.mu; T->mu_w = BS.mu;
     T->basis_err = std::max(BC.err, BS.err);
+    // The level pass's first-order and diagonal terms through profile bases
+    // over [0, 2 kmax] (the diagonal's profile is that of 2 k_i): by default in
+    // the one-product (bf16) mode, whose velocity error (~1e-3 of the field
+    // scale, from the rounding of the pair forms) dwarfs the 1e-5 they add;
+    // CFDWM_LIN_TOL sets the tolerance (0: off, per-level profiles).
+    double lin_tol = nlv == 1 ? 1e-5 : 0.0;
+    if (const char* e = std::getenv("CFDWM_LIN_TOL")) lin_tol = std::atof(e);
+    if (lin_tol > 0) {
+        double kl = 0;
+        for (int i = 0; i < N; ++i) kl = std::max(kl, 2.0 * std::fabs(tab.comp.k[i]));
+        ProfileBasis LC, LS;
+        const double klb = kl * (1.0 + 1e-9) + 1e-12;
+        const int sc = build_profile_basis_cached(klb, h, +1, lin_tol, 16, LC);
+        const int ss = build_profile_basis_cached(klb, h, -1, lin_tol, 16, LS);
+        if (sc == 0 && ss == 0 && LC.R >= 1 && LC.R <= 16 && LS.R >= 1 && LS.R <= 16) {
+            // per component 4 x 16 floats: a(k), a(2k), b(k), b(2k), zeros past the ranks
+            const int R1 = LC.R, R2 = LS.R, ls = 4 * 16;
+            std::vector<float> tabc((size_t)N * ls, 0.0f);
+            double ca[64], cb[64];
+            for (int i = 0; i < N; ++i) {
+                const double k = tab.comp.k[i];
+                float* o = &tabc[(size_t)i * ls];
+                LC.coeffs(k, ca);
+                LC.coeffs(2.0 * k, cb);
+                for (int q = 0; q < R1; ++q) { o[q] = (float)ca[q]; o[16 + q] = (float)cb[q]; }
+                LS.coeffs(k, ca);
+                LS.coeffs(2.0 * k, cb);
+                for (int q = 0; q < R2; ++q) { o[32 + q] = (float)ca[q]; o[48 + q] = (float)cb[q]; }
+            }
+            constexpr double kL2e = 1.4426950408889634074;
+            float mut[64] = {};
+            for (int q = 0; q < R1; ++q) { mut[q] = (float)(LC.mu[q] * kL2e); mut[16 + q] = (float)(-2.0 * h * LC.mu[q] * kL2e); }
+            for (int q = 0; q < R2; ++q) { mut[32 + q] = (float)(LS.mu[q] * kL2e); mut[48 + q] = (float)(-2.0 * h * LS.mu[q] * kL2e); }
+            if (T->lin_c.upload(tabc.data(), tabc.size() * sizeof(float), Engine::get().stream()) ||
+                T->lin_exp.upload(mut, sizeof(mut), Engine::get().stream()))
+                return -3;
+            CFDWM_HIP(hipStreamSynchronize(Engine::get().stream()));   // the staging vector goes out of scope
+            T->lin_rank = R1; T->lin_rank_w = R2;
+            T->lin_mu = LC.mu; T->lin_mu_w = LS.mu;
+            T->lin_err = std::max(LC.err, LS.err);
+            T->lin_tol = lin_tol;
+        }
+    }
     T->basis_tol = tol; T->nlv = nlv; T->nbarr = nbarr;
     T->kend_h = kend;
     for (size_t t = 0; t < kend.size(); ++t) T->cost += std::max(0, kend[t] - (int)(64 * t));
@@ -748,10 +791,18 @@ int Grid::fill_slots(int nslot, const double* slot_t, const int* slot_dst, bool 
     if (qf_used) C.nseg = pick_nseg(C.ncol * nslot, 64 * ncomp);
     C.seg = qf_used ? sea->comp_segments(C.nseg) : sea->segments(false, C.nseg);
     if (!C.seg) return -3;
-    const int64_t lthr = level_threads(nslot, C.ncol, g.nl, level_chunk, true);
-    const int lseg = pick_nseg(lthr, qf_used ? 64 * ncomp : nrec);
+    // the linear-term bases (L.lin_c, set with the forms) take all 16 levels
+    // of a column in one thread and one segment: their per-component work no
+    // longer depends on the level count
+    const bool lin16 = qf_used && L.lin_c && level_chunk == 8 && g.nl <= 16;
+    const int lchunk = lin16 ? 16 : level_chunk;
+    const int64_t lthr = level_threads(nslot, C.ncol, g.nl, lchunk, true);
+    const int lseg = lin16 ? 1 : pick_nseg(lthr, qf_used ? 64 * ncomp : nrec);
+    if constexpr (sizeof(R) == 4) {
+        if (!lin16) L.lin_c = nullptr;
+    }
     {
-        const size_t need = std::max((size_t)C.nseg * nslot * C.ncol * 6, (size_t)lseg * lthr * (3 * level_chunk + 6));
+        const size_t need = std::max((size_t)C.nseg * nslot * C.ncol * 6, (size_t)lseg * lthr * (3 * lchunk + 6));
         if ((C.nseg > 1 || lseg > 1) && partial_.alloc(need * sizeof(double))) return -3;
     }
     if (C.nseg > 1) C.partial = (double*)partial_.get();
@@ -775,7 +826,7 @@ int Grid::fill_slots(int nslot, const double* slot_t, const int* slot_dst, bool 
                                         (uint32_t*)lvl_.get(), st));
         L.lvl = (const uint32_t*)lvl_.get();
     }
-    PROF("k_levels_grid", launch_levels<R>(S, L, true, level_chunk, st));
+    PROF("k_levels_grid", launch_levels<R>(S, L, true, lchunk, st));
     return 0;
 }
 
@@ -985,6 +1036,10 @@ void Grid::qform_attach(ColumnJob<float>& C, LevelJob<float>& L) {
         L.qf_mgw[q] = (float)(-2.0 * hs * T->mu_w[q] * kL2e);
     }
     for (int s = 0; s < 4; ++s) L.qf_scale[s] = qf_scale_[s];
+    L.lin_c = T->lin_rank > 0 ? (const float*)T->lin_c.get() : nullptr;
+    L.lin_mu = (const float*)T->lin_exp.get();
+    L.lin_R = T->lin_rank;
+    L.lin_Rw = T->lin_rank_w;
 }
 
 int Grid::qform_fill(int nslot, const double* slot_t, int64_t ncol, const int32_t* cols, ColumnJob<float>& C,

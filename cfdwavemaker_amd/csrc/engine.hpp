@@ -63,6 +63,14 @@ struct LevelJob {
     float qf_mu2w[32] = {};               // nu_q log2 e
     float qf_mgw[32] = {};                // -2 h nu_q log2 e
     double qf_scale[4] = {1, 1, 1, 1};
+    // GRID, form-assisted: the first-order and diagonal terms through two
+    // profile bases instead of per-level profiles (lin_c null: per level):
+    // u += sum_q (e^{m_q z} + e^{-m_q (z + 2h)}) sum_i [(lu_i c_i r) a_q(k_i)
+    // + (dfu_i cos 2theta_i r) a_q(2 k_i)], v likewise, w with the sinh-type
+    // basis, lw_i s_i r and dfw_i sin 2theta_i r
+    const float* lin_c = nullptr;         // [n][4 x 16]: a(k), a(2k), b(k), b(2k), zeros past the ranks
+    const float* lin_mu = nullptr;        // [4][16]: m_q log2 e, -2 h m_q log2 e (cosh-type), the same (sinh-type)
+    int lin_R = 0, lin_Rw = 0;
     const Segment* seg = nullptr;         // nseg segments of the table in use (blockIdx.y)
     int nseg = 1;
     double* partial = nullptr;            // nseg > 1: [seg][thread][3 LC + 6] partial sums
@@ -316,6 +324,14 @@ public:
         DevBuf ftype;                        // [nform] int8
         DevBuf kend;                         // per 64-wide tile of i
         DevBuf zs, finv, zsw, finvw;         // the profile bases' skeleton depths and interpolation matrices
+        // first-order and diagonal terms of the level pass through two more
+        // profile bases over [0, 2 kmax] (lin_rank 0: per-level profiles):
+        // per component a_q(k), a_q(2k) (cosh-type), b_q(k), b_q(2k) (sinh-type)
+        int lin_rank = 0, lin_rank_w = 0;
+        double lin_err = 0, lin_tol = 0;
+        std::vector<double> lin_mu, lin_mu_w;
+        DevBuf lin_c;                        // [n][4 x 16] float (zeros past the ranks)
+        DevBuf lin_exp;                      // [4][16] float: the bases' exponents for the level pass
         std::vector<int32_t> kend_h;
         double cost = 0;                     // sum over i tiles of their m range (tile ordering)
     };

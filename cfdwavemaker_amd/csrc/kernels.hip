@@ -286,8 +286,11 @@ __device__ __forceinline__ void lev_store(const SeaDev<R>& S, const LevelJob<R>&
 // Accumulation: each row i (pairs (i, m), m > i) is summed in the working
 // precision R and folded into FP64 totals, which keeps the FP32 error at the
 // level of the per-term rounding instead of growing over ~N^2/2 terms.
-// F8: FP32 grid fill whose pair sums come from the quadratic forms (J.qf).
-template <typename R, int LC, int MODE, bool F8 = false>
+// F8: FP32 grid fill whose pair sums come from the quadratic forms (J.qf);
+// F8 == 2: its first-order and diagonal terms through the linear-term
+// profile bases as well (J.lin_c): per component the basis coefficients are
+// accumulated (LU, LV, LW), the profiles evaluated once per level at the end.
+template <typename R, int LC, int MODE, int F8 = 0>
 __device__ __forceinline__ void levels_body(const SeaDev<R>& S, const LevelJob<R>& J) {
     constexpr bool GRID = MODE == 0;
     constexpr bool WP = MODE == 2;
@@ -339,7 +342,12 @@ __device__ __forceinline__ void levels_body(const SeaDev<R>& S, const LevelJob<R
     // once per 8 components; elsewhere the FP32 row sums of the pair terms
     // are folded per component.  The component loop is wave-uniform.
     constexpr bool SQ = sizeof(R) == 4;
-    constexpr int fold_n = (SQ && GRID && F8) ? 8 : 1;
+    constexpr int fold_n = (SQ && GRID && F8 != 0) ? 8 : 1;
+    constexpr bool LB = F8 == 2 && SQ && GRID;      // linear-term bases
+    constexpr int LRM = 16;                          // their largest rank
+    R LU[LB ? LRM : 1], LV[LB ? LRM : 1], LW[LB ? LRM : 1];
+#pragma unroll
+    for (int q = 0; q < (LB ? LRM : 1); ++q) LU[q] = LV[q] = LW[q] = 0;
     R ur[LC], vr[LC], wr[LC];
 #pragma unroll
     for (int l = 0; l < LC; ++l) ur[l] = vr[l] = wr[l] = 0;
@@ -348,27 +356,47 @@ __device__ __forceinline__ void levels_body(const SeaDev<R>& S, const LevelJob<R
     // takes one value at each slot time (checked on the host), so one
     // evaluation per thread serves all components, bit for bit
     R ri_f8 = 0;
-    if constexpr (F8) {
+    if constexpr (F8 != 0) {
         const PhaseRec p0 = ld_rec<R>(S.ph, 0);
         ri_f8 = (R)freq_ramp(t, p0.rt, p0.ts, p0.du);
     }
     for (int i = ia; i < ib; ++i) {
         const bool own = i >= sg.own_lo && i < sg.own_hi;
         const bool sec = secf && (direct || (sizeof(R) == 4 ? row_active(S.rowActive, i) : S.rowActive[i] != 0));
-        const bool prs = sec && !(GRID && J.qf) && i >= sg.rlo && i < sg.rhi;
+        // (form-assisted fills, F8, launch only with the forms: no pair rows)
+        const bool prs = F8 == 0 && sec && !(GRID && J.qf) && i >= sg.rlo && i < sg.rhi;
         if (!own && !prs) continue;
         const PhaseRec pi = ld_rec<R>(S.ph, i);
         const CompRec<R> c = ld_rec<R>(S.cr, i);
         R ci, si;
         cs_of(phase_of(pi, dxp, dyp, t), ci, si);
-        const R ri = F8 ? ri_f8 : (R)freq_ramp(t, pi.rt, pi.ts, pi.du);
+        const R ri = F8 != 0 ? ri_f8 : (R)freq_ramp(t, pi.rt, pi.ts, pi.du);
         const R cr = ci * ri, sr = si * ri;
         R pr = 0;
         if constexpr (fold_n == 1) {
 #pragma unroll
             for (int l = 0; l < LC; ++l) ur[l] = vr[l] = wr[l] = 0;
         }
-        if (own) {
+        if (LB && own) {
+            gx += (double)(c.gxz * cr);
+            gy += (double)(c.gyz * cr);
+            gz += (double)(c.gzz * sr);
+            // first-order terms on the profile of k, diagonal on that of 2k
+            const R xu = lin ? c.lu * cr : (R)0, xv = lin ? c.lv * cr : (R)0, xw = lin ? c.lw * sr : (R)0;
+            const R c2 = sec ? (ci * ci - si * si) * ri : (R)0, s2 = sec ? (R)2 * si * ci * ri : (R)0;
+            const R du = c.dfu * c2, dv = c.dfv * c2, dw = c.dfw * s2;
+            // the table holds LRM entries per family (zeros past the rank), so
+            // the loop has no branch and its scalar loads go out together
+            const float* cf = J.lin_c + (int64_t)i * (4 * LRM);
+#pragma unroll
+            for (int q = 0; q < LRM; ++q) {
+                const R a = ld_const(cf, q), a2 = ld_const(cf, LRM + q);
+                const R b = ld_const(cf, 2 * LRM + q), b2 = ld_const(cf, 3 * LRM + q);
+                LU[q] = fmaR(xu, a, fmaR(du, a2, LU[q]));
+                LV[q] = fmaR(xv, a, fmaR(dv, a2, LV[q]));
+                LW[q] = fmaR(xw, b, fmaR(dw, b2, LW[q]));
+            }
+        } else if (own) {
             gx += (double)(c.gxz * cr);
             gy += (double)(c.gyz * cr);
             gz += (double)(c.gzz * sr);
@@ -503,7 +531,7 @@ __device__ __forceinline__ void levels_body(const SeaDev<R>& S, const LevelJob<R
                 }
             }
         }
-        if (++nfold == fold_n) {
+        if (!LB && ++nfold == fold_n) {
 #pragma unroll
             for (int l = 0; l < LC; ++l) {
                 U[l] += (double)ur[l];
@@ -515,13 +543,59 @@ __device__ __forceinline__ void levels_body(const SeaDev<R>& S, const LevelJob<R
         }
         if (WP && direct) P2 += (double)pr;
     }
-    if (fold_n > 1 && nfold > 0) {
+    if (!LB && fold_n > 1 && nfold > 0) {
 #pragma unroll
         for (int l = 0; l < LC; ++l) {
             U[l] += (double)ur[l];
             V[l] += (double)vr[l];
             W[l] += (double)wr[l];
         }
+    }
+    if constexpr (LB) {
+        // all levels of the column (one thread, one segment): per level the
+        // linear-term bases, the forms and the Taylor extension, stored at once
+        if (!PP.live) return;
+        const int64_t o = ((int64_t)J.slot_dst[PP.slot] * J.ncell + PP.col) * J.g.nl + PP.l0;
+        const double sc = J.qf_scale[PP.slot];
+        const int64_t cq = PP.cq;
+        const int Rk = J.qf_rank, Rw = J.qf_rank_w;
+        const int nlev = min(LC, J.g.nl - PP.l0);
+        for (int l = 0; l < nlev; ++l) {
+            const double zz = PP.e + J.sigma[PP.l0 + l] * (J.g.h + PP.e);
+            const R z = (R)(zz < 0. ? zz : 0.);                          // min(0, z), as zp
+            R su = 0, sv = 0, sw = 0;
+#pragma unroll
+            for (int q = 0; q < LRM; ++q) {
+                if (q < J.lin_R) {
+                    R E, G;
+                    prof((R)ld_const(J.lin_mu, q), (R)ld_const(J.lin_mu, LRM + q), z, E, G);
+                    su = fmaR(LU[q], E + G, su);
+                    sv = fmaR(LV[q], E + G, sv);
+                }
+                if (q < J.lin_Rw) {
+                    R E, G;
+                    prof((R)ld_const(J.lin_mu, 2 * LRM + q), (R)ld_const(J.lin_mu, 3 * LRM + q), z, E, G);
+                    sw = fmaR(LW[q], E - G, sw);
+                }
+            }
+            R fu = 0, fv = 0, fw = 0;
+            for (int q = 0; q < Rk; ++q) {
+                R E, G;
+                prof((R)J.qf_mu2[q], (R)J.qf_mg[q], z, E, G);
+                fu = fmaR((R)J.qf[(int64_t)q * J.qf_stride + cq], E + G, fu);
+                fv = fmaR((R)J.qf[(int64_t)(Rk + q) * J.qf_stride + cq], E + G, fv);
+            }
+            for (int q = 0; q < Rw; ++q) {
+                R E, G;
+                prof((R)J.qf_mu2w[q], (R)J.qf_mgw[q], z, E, G);
+                fw = fmaR((R)J.qf[(int64_t)(2 * Rk + q) * J.qf_stride + cq], E - G, fw);
+            }
+            const double zt = zz - J.g.swl > 0. ? zz - J.g.swl : 0.;    // max(0, z - swl)
+            J.o_u[o + l] = (R)((double)su + sc * (double)fu + gx * zt);
+            J.o_v[o + l] = (R)((double)sv + sc * (double)fv + gy * zt);
+            J.o_w[o + l] = (R)((double)sw + sc * (double)fw + gz * zt);
+        }
+        return;
     }
     if constexpr (GRID) {
         if (J.qf && blockIdx.y == 0) {
@@ -588,7 +662,7 @@ __global__ __launch_bounds__(256) void k_levels(SeaDev<R> S, LevelJob<R> J) {
 // FP32 s-grid fill: the register budget is steered to 128 VGPRs so that four
 // waves share each SIMD (the kernel is issue-bound; more waves hide the
 // scalar-load and transcendental latencies).
-template <int LC, bool F8>
+template <int LC, int F8>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8)))
 void k_levels_grid32(SeaDev<float> S, LevelJob<float> J) {
     levels_body<float, LC, 0, F8>(S, J);
@@ -1053,10 +1127,12 @@ static hipError_t launch_levels_t(const SeaDev<R>& S, const LevelJob<R>& J, hipS
     const int64_t n = level_threads(J.nslot, J.ncol, J.g.nl, LC, MODE == 0);
     if (J.nslot * J.ncol <= 0) return hipSuccess;
     if constexpr (MODE == 0 && sizeof(R) == 4 && (LC == 8 || LC == 16)) {
-        if (J.qf)
-            hipLaunchKernelGGL((k_levels_grid32<LC, true>), dim3(nblk(n), J.nseg), dim3(256), 0, st, S, J);
+        if (J.qf && J.lin_c && LC == 16 && J.nseg == 1)
+            hipLaunchKernelGGL((k_levels_grid32<LC, 2>), dim3(nblk(n), J.nseg), dim3(256), 0, st, S, J);
+        else if (J.qf)
+            hipLaunchKernelGGL((k_levels_grid32<LC, 1>), dim3(nblk(n), J.nseg), dim3(256), 0, st, S, J);
         else
-            hipLaunchKernelGGL((k_levels_grid32<LC, false>), dim3(nblk(n), J.nseg), dim3(256), 0, st, S, J);
+            hipLaunchKernelGGL((k_levels_grid32<LC, 0>), dim3(nblk(n), J.nseg), dim3(256), 0, st, S, J);
     } else {
         hipLaunchKernelGGL((k_levels<R, LC, MODE>), dim3(nblk(n), J.nseg), dim3(256), 0, st, S, J);
     }
