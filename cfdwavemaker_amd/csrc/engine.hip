@@ -792,14 +792,12 @@ int Grid::fill_slots(int nslot, const double* slot_t, const int* slot_dst, bool 
     C.seg = qf_used ? sea->comp_segments(C.nseg) : sea->segments(false, C.nseg);
     if (!C.seg) return -3;
     // the linear-term bases (L.lin_c, set with the forms) take all 16 levels
-    // of a column in one thread and one segment: their per-component work no
-    // longer depends on the level count.  Fills of fewer column-slots than
-    // about four waves per SIMD need the component segments of the per-level
-    // kernel to fill the GPU and keep that one.
-    const bool lin16 = qf_used && L.lin_c && level_chunk == 8 && g.nl <= 16 && (int64_t)nslot * C.ncol >= 262144;
+    // of a column in one thread: their per-component work no longer depends
+    // on the level count
+    const bool lin16 = qf_used && L.lin_c && level_chunk == 8 && g.nl <= 16;
     const int lchunk = lin16 ? 16 : level_chunk;
     const int64_t lthr = level_threads(nslot, C.ncol, g.nl, lchunk, true);
-    const int lseg = lin16 ? 1 : pick_nseg(lthr, qf_used ? 64 * ncomp : nrec);
+    const int lseg = pick_nseg(lthr, qf_used ? 64 * ncomp : nrec);
     if constexpr (sizeof(R) == 4) {
         if (!lin16) L.lin_c = nullptr;
     }

@@ -552,15 +552,27 @@ __device__ __forceinline__ void levels_body(const SeaDev<R>& S, const LevelJob<R
         }
     }
     if constexpr (LB) {
-        // all levels of the column (one thread, one segment): per level the
-        // linear-term bases, the forms and the Taylor extension, stored at once
+        // all levels of the column in one thread: per level the linear-term
+        // bases, the forms (segment 0) and the Taylor extension, stored at
+        // once, or as the segment's partial record (U, V, W per level, gx, gy,
+        // gz) for k_levels_reduce
         if (!PP.live) return;
+        const bool seg0 = blockIdx.y == 0;
+        double* po = nullptr;
+        if (J.partial) {
+            const int64_t nthr = ((int64_t)J.nslot * J.ncol + 63) / 64 * 64 * ((J.g.nl + LC - 1) / LC);
+            po = J.partial + ((int64_t)blockIdx.y * nthr + tid) * RS;
+        }
         const int64_t o = ((int64_t)J.slot_dst[PP.slot] * J.ncell + PP.col) * J.g.nl + PP.l0;
         const double sc = J.qf_scale[PP.slot];
         const int64_t cq = PP.cq;
-        const int Rk = J.qf_rank, Rw = J.qf_rank_w;
+        const int Rk = seg0 ? J.qf_rank : 0, Rw = seg0 ? J.qf_rank_w : 0;
         const int nlev = min(LC, J.g.nl - PP.l0);
-        for (int l = 0; l < nlev; ++l) {
+        for (int l = 0; l < LC; ++l) {
+            if (l >= nlev) {
+                if (po) po[l] = po[LC + l] = po[2 * LC + l] = 0.;
+                continue;
+            }
             const double zz = PP.e + J.sigma[PP.l0 + l] * (J.g.h + PP.e);
             const R z = (R)(zz < 0. ? zz : 0.);                          // min(0, z), as zp
             R su = 0, sv = 0, sw = 0;
@@ -588,12 +600,24 @@ __device__ __forceinline__ void levels_body(const SeaDev<R>& S, const LevelJob<R
             for (int q = 0; q < Rw; ++q) {
                 R E, G;
                 prof((R)J.qf_mu2w[q], (R)J.qf_mgw[q], z, E, G);
-                fw = fmaR((R)J.qf[(int64_t)(2 * Rk + q) * J.qf_stride + cq], E - G, fw);
+                fw = fmaR((R)J.qf[(int64_t)(2 * J.qf_rank + q) * J.qf_stride + cq], E - G, fw);
             }
-            const double zt = zz - J.g.swl > 0. ? zz - J.g.swl : 0.;    // max(0, z - swl)
-            J.o_u[o + l] = (R)((double)su + sc * (double)fu + gx * zt);
-            J.o_v[o + l] = (R)((double)sv + sc * (double)fv + gy * zt);
-            J.o_w[o + l] = (R)((double)sw + sc * (double)fw + gz * zt);
+            const double u = (double)su + sc * (double)fu, v = (double)sv + sc * (double)fv,
+                         w = (double)sw + sc * (double)fw;
+            if (po) {
+                po[l] = u;
+                po[LC + l] = v;
+                po[2 * LC + l] = w;
+            } else {
+                const double zt = zz - J.g.swl > 0. ? zz - J.g.swl : 0.;    // max(0, z - swl)
+                J.o_u[o + l] = (R)(u + gx * zt);
+                J.o_v[o + l] = (R)(v + gy * zt);
+                J.o_w[o + l] = (R)(w + gz * zt);
+            }
+        }
+        if (po) {
+            po[3 * LC] = gx; po[3 * LC + 1] = gy; po[3 * LC + 2] = gz;
+            po[3 * LC + 3] = 0.; po[3 * LC + 4] = 0.; po[3 * LC + 5] = 0.;
         }
         return;
     }
@@ -1127,7 +1151,7 @@ static hipError_t launch_levels_t(const SeaDev<R>& S, const LevelJob<R>& J, hipS
     const int64_t n = level_threads(J.nslot, J.ncol, J.g.nl, LC, MODE == 0);
     if (J.nslot * J.ncol <= 0) return hipSuccess;
     if constexpr (MODE == 0 && sizeof(R) == 4 && (LC == 8 || LC == 16)) {
-        if (J.qf && J.lin_c && LC == 16 && J.nseg == 1)
+        if (J.qf && J.lin_c && LC == 16)
             hipLaunchKernelGGL((k_levels_grid32<LC, 2>), dim3(nblk(n), J.nseg), dim3(256), 0, st, S, J);
         else if (J.qf)
             hipLaunchKernelGGL((k_levels_grid32<LC, 1>), dim3(nblk(n), J.nseg), dim3(256), 0, st, S, J);

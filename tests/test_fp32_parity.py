@@ -181,25 +181,26 @@ def test_bf16_side_stream_fill_equals_serial_fill(eng, monkeypatch):
         assert np.array_equal(a, b)
 
 
-@pytest.mark.parametrize("prec_name,lin_tol,bound", [("bf16", "1e-5", 1e-4), ("fp32", "1e-6", 2e-5)])
-def test_linear_term_bases_vs_per_level_profiles(eng, oracle, monkeypatch, prec_name, lin_tol, bound):
+@pytest.mark.parametrize("prec_name,lin_tol,bound,n", [("bf16", "1e-5", 1e-4, 256), ("fp32", "1e-6", 2e-5, 256),
+                                                       ("bf16", "1e-5", 1e-4, 12)])
+def test_linear_term_bases_vs_per_level_profiles(eng, oracle, monkeypatch, prec_name, lin_tol, bound, n):
     """The level pass's first-order and diagonal terms through the
-    linear-term profile bases (default in the bf16 mode for fills of 262,144
-    column-slots or more, CFDWM_LIN_TOL) against the per-level profiles
-    (CFDWM_LIN_TOL=0) on the same 256 x 256 x 16 x 4 fill: the difference stays
-    within a small multiple of the bases' tolerance of the field scale, and
-    the FP32 fill with them stays within the FP32 bound of the oracle on a
-    corner block of columns."""
+    linear-term profile bases (default in the bf16 mode, CFDWM_LIN_TOL) against
+    the per-level profiles (CFDWM_LIN_TOL=0) on the same n x n x 16 x 4 fill
+    (n = 12: the level pass splits the components over segments and sums
+    their partial records): the difference stays within a small multiple of
+    the bases' tolerance of the field scale, and the FP32 fill with them stays
+    within the FP32 bound of the oracle on a corner block of columns."""
     from cfdwavemaker_amd.seastate import config_sea
     s = config_sea(3).as_dict()
     dx = 1000.0 / 255
-    n, nl = 256, 16
+    nl = 16
     prec = eng.FP32 if prec_name == "fp32" else eng.BF16
     out = {}
     for tol in (lin_tol, "0"):
         monkeypatch.setenv("CFDWM_LIN_TOL", tol)
-        g = eng.Sea(s, precision=prec, with_direct=False).grid((0.0, 255 * dx, 0.0, 255 * dx), n, n, nl, 0.0, 0.5,
-                                                                0.7, 1.5, 100.0)
+        g = eng.Sea(s, precision=prec, with_direct=False).grid((0.0, (n - 1) * dx, 0.0, (n - 1) * dx), n, n, nl, 0.0,
+                                                                0.5, 0.7, 1.5, 100.0)
         assert g.qform_info() is not None
         out[tol] = {f: g.export(f) for f in ("ETA", "UX", "UY", "UZ")}
         del g
