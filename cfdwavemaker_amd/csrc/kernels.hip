@@ -345,9 +345,12 @@ __device__ __forceinline__ void levels_body(const SeaDev<R>& S, const LevelJob<R
     constexpr int fold_n = (SQ && GRID && F8 != 0) ? 8 : 1;
     constexpr bool LB = F8 == 2 && SQ && GRID;      // linear-term bases
     constexpr int LRM = 16;                          // their largest rank
-    R LU[LB ? LRM : 1], LV[LB ? LRM : 1], LW[LB ? LRM : 1];
+    constexpr int LQ = LB ? LRM : 1;
+    R LU[LQ], LV[LQ], LW[LQ];
 #pragma unroll
-    for (int q = 0; q < (LB ? LRM : 1); ++q) LU[q] = LV[q] = LW[q] = 0;
+    for (int q = 0; q < LQ; ++q) LU[q] = LV[q] = LW[q] = 0;
+    // LB: the Taylor gradients' component sums in the working precision
+    R gxr = 0, gyr = 0, gzr = 0;
     R ur[LC], vr[LC], wr[LC];
 #pragma unroll
     for (int l = 0; l < LC; ++l) ur[l] = vr[l] = wr[l] = 0;
@@ -378,9 +381,9 @@ __device__ __forceinline__ void levels_body(const SeaDev<R>& S, const LevelJob<R
             for (int l = 0; l < LC; ++l) ur[l] = vr[l] = wr[l] = 0;
         }
         if (LB && own) {
-            gx += (double)(c.gxz * cr);
-            gy += (double)(c.gyz * cr);
-            gz += (double)(c.gzz * sr);
+            gxr = fmaR(c.gxz, cr, gxr);
+            gyr = fmaR(c.gyz, cr, gyr);
+            gzr = fmaR(c.gzz, sr, gzr);
             // first-order terms on the profile of k, diagonal on that of 2k
             const R xu = lin ? c.lu * cr : (R)0, xv = lin ? c.lv * cr : (R)0, xw = lin ? c.lw * sr : (R)0;
             const R c2 = sec ? (ci * ci - si * si) * ri : (R)0, s2 = sec ? (R)2 * si * ci * ri : (R)0;
@@ -389,7 +392,7 @@ __device__ __forceinline__ void levels_body(const SeaDev<R>& S, const LevelJob<R
             // the loop has no branch and its scalar loads go out together
             const float* cf = J.lin_c + (int64_t)i * (4 * LRM);
 #pragma unroll
-            for (int q = 0; q < LRM; ++q) {
+            for (int q = 0; q < LQ; ++q) {
                 const R a = ld_const(cf, q), a2 = ld_const(cf, LRM + q);
                 const R b = ld_const(cf, 2 * LRM + q), b2 = ld_const(cf, 3 * LRM + q);
                 LU[q] = fmaR(xu, a, fmaR(du, a2, LU[q]));
@@ -557,6 +560,7 @@ __device__ __forceinline__ void levels_body(const SeaDev<R>& S, const LevelJob<R
         // once, or as the segment's partial record (U, V, W per level, gx, gy,
         // gz) for k_levels_reduce
         if (!PP.live) return;
+        gx = (double)gxr; gy = (double)gyr; gz = (double)gzr;
         const bool seg0 = blockIdx.y == 0;
         double* po = nullptr;
         if (J.partial) {
@@ -577,7 +581,7 @@ __device__ __forceinline__ void levels_body(const SeaDev<R>& S, const LevelJob<R
             const R z = (R)(zz < 0. ? zz : 0.);                          // min(0, z), as zp
             R su = 0, sv = 0, sw = 0;
 #pragma unroll
-            for (int q = 0; q < LRM; ++q) {
+            for (int q = 0; q < LQ; ++q) {
                 if (q < J.lin_R) {
                     R E, G;
                     prof((R)ld_const(J.lin_mu, q), (R)ld_const(J.lin_mu, LRM + q), z, E, G);
@@ -835,6 +839,23 @@ __device__ __forceinline__ double cart2sigmaS(double zpt, double e, double h, do
     return 1 - atan(pow(-sigma * tanab, 1. / tb)) / ta;
 }
 
+// The same map in FP32 for FP32 grids (k_query<float, .>): divisions by the
+// grid constants become products with their reciprocals, pow and atan the
+// FP32 library functions.  The sigma coordinate then carries a few 1e-7 of
+// absolute error, far below the FP32 grid values' own.
+struct QueryConst {
+    double inv_dx, inv_dy, inv_dt;     // 1 / dx, 1 / dy, 1 / dt
+    float h, swl, tanab;               // the FP64 map's depth, swl and tanab
+    float ita, itb, inv_ds;            // 1 / tan_a, 1 / tan_b, 1 / ds
+};
+__device__ __forceinline__ float cart2sigmaS_f(float zpt, float e, const QueryConst& K) {
+    const float z = zpt < e ? zpt : e;
+    float sigma = (z - (e - K.swl)) / (K.h + (e - K.swl));
+    sigma = sigma < -1.f ? -1.f : sigma;
+    sigma = sigma > 0.f ? 0.f : sigma;
+    return 1.f - atanf(powf(-sigma * K.tanab, K.itb)) * K.ita;
+}
+
 // A packed record of N values (N a multiple of 4) at a 16-byte aligned
 // address, loaded as 16-byte vectors.  Node records hold 16 values (3 fields x
 // {value, d/dt, d/dx, d/dy, d/ds} + pad), column surface records 8.
@@ -919,7 +940,7 @@ __device__ __forceinline__ void slot_velo(R (&v)[3], R (&vt)[3], const R* __rest
 // J.srec thread k evaluates the k-th query in cell order; p is always the
 // query's index in the caller's arrays.
 template <typename R, typename TQ>
-__global__ __launch_bounds__(256) void k_query(QueryJob<R, TQ> J) {
+__global__ __launch_bounds__(256) void k_query(QueryJob<R, TQ> J, QueryConst K) {
     const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= J.n) return;
     int64_t p = k;
@@ -933,16 +954,17 @@ __global__ __launch_bounds__(256) void k_query(QueryJob<R, TQ> J) {
     }
     const GridGeom& g = J.g;
     const double tpt = J.t ? (double)J.t[p] : J.tconst;
-    double cx = (xpt - g.x0) / g.dx;
+    constexpr bool F = sizeof(R) == 4;
+    double cx = F ? (xpt - g.x0) * K.inv_dx : (xpt - g.x0) / g.dx;
     cx = cx < (g.nx - 1.) ? cx : (g.nx - 1.);
     cx = cx > 0. ? cx : 0.;
-    double cy = (ypt - g.y0) / g.dy;
+    double cy = F ? (ypt - g.y0) * K.inv_dy : (ypt - g.y0) / g.dy;
     cy = cy < (g.ny - 1.) ? cy : (g.ny - 1.);
     cy = cy > 0. ? cy : 0.;
     const float fx = (float)cx, fy = (float)cy;
     const float ix = truncf(fx), iy = truncf(fy);
     const R xd = (R)(fx - ix), yd = (R)(fy - iy);
-    double td = (tpt - J.t0) / J.dt;
+    double td = F ? (tpt - J.t0) * K.inv_dt : (tpt - J.t0) / J.dt;
     td = td > 0. ? td : 0.;
     td = td < 1. ? td : 1.;
     const int i = (int)ix, j = (int)iy;
@@ -975,9 +997,16 @@ __global__ __launch_bounds__(256) void k_query(QueryJob<R, TQ> J) {
     const R ta = dt * CCt - (e1 - e0), tb = -dt * DDt + (e1 - e0);
     const R tdR = (R)td;
     const R eta = herm(e0, e1, ta, tb, tdR);
-    const double sp0 = cart2sigmaS(zpt, (double)e0, g.h, g.swl, J.tan_a, J.tan_b, J.tanab);
-    const double sp1 = cart2sigmaS(zpt, (double)e1, g.h, g.swl, J.tan_a, J.tan_b, J.tanab);
-    const float f0 = (float)(sp0 / g.ds), f1 = (float)(sp1 / g.ds);
+    float f0, f1;
+    if constexpr (F) {
+        f0 = cart2sigmaS_f((float)zpt, (float)e0, K) * K.inv_ds;
+        f1 = cart2sigmaS_f((float)zpt, (float)e1, K) * K.inv_ds;
+    } else {
+        const double sp0 = cart2sigmaS(zpt, (double)e0, g.h, g.swl, J.tan_a, J.tan_b, J.tanab);
+        const double sp1 = cart2sigmaS(zpt, (double)e1, g.h, g.swl, J.tan_a, J.tan_b, J.tanab);
+        f0 = (float)(sp0 / g.ds);
+        f1 = (float)(sp1 / g.ds);
+    }
     const float k0 = truncf(f0), k1 = truncf(f1);
     const R sd0 = (R)(f0 - k0), sd1 = (R)(f1 - k1);
     int l0 = (int)k0, l1 = (int)k1;
@@ -1218,7 +1247,11 @@ hipError_t launch_clamp_below(int64_t n, const double* z, double lo, double* out
 template <typename R, typename TQ>
 hipError_t launch_query(const QueryJob<R, TQ>& J, hipStream_t st) {
     if (J.n <= 0) return hipSuccess;
-    hipLaunchKernelGGL((k_query<R, TQ>), dim3(nblk(J.n)), dim3(256), 0, st, J);
+    QueryConst K;
+    K.inv_dx = 1.0 / J.g.dx; K.inv_dy = 1.0 / J.g.dy; K.inv_dt = 1.0 / J.dt;
+    K.h = (float)J.g.h; K.swl = (float)J.g.swl; K.tanab = (float)J.tanab;
+    K.ita = (float)(1.0 / J.tan_a); K.itb = (float)(1.0 / J.tan_b); K.inv_ds = (float)(1.0 / J.g.ds);
+    hipLaunchKernelGGL((k_query<R, TQ>), dim3(nblk(J.n)), dim3(256), 0, st, J, K);
     return hipGetLastError();
 }
 
