@@ -277,6 +277,113 @@ __device__ __forceinline__ void lev_store(const SeaDev<R>& S, const LevelJob<R>&
     }
 }
 
+// Component loop of the linear-term-basis mode (levels_body, F8 == 2): per
+// component one phase, cos/sin, and the 16 x 6 products of the basis
+// coefficients.  The component records (phase constants, coefficients and the
+// 64-entry basis row, 368 bytes) are staged block-wide in LDS, 32 components
+// at a time with the next chunk's global loads in flight meanwhile, and read
+// back as wave-uniform 16-byte LDS loads; each component's phase and cos/sin
+// are computed one component ahead.  Scalar loads would leave every wave
+// waiting for its records once per component.  Same operations and order as
+// the per-component form, so the sums are the same bit for bit.
+constexpr int kLbChunk = 32, kLbPieces = 23;   // components per chunk, 16-byte pieces per component
+template <int LRM>
+__device__ __forceinline__ void lb_component_sums(const SeaDev<float>& S, const LevelJob<float>& J, int lo, int hi,
+                                                  double dxp, double dyp, double t, float ri, bool lin, bool secf,
+                                                  float (&LU)[LRM], float (&LV)[LRM], float (&LW)[LRM],
+                                                  float& gxr, float& gyr, float& gzr) {
+    // piece k of chunk slot j: 0-2 PhaseRec (k, cth, sth, omega, ph0, rt),
+    // 3-6 CompRec floats 0-15, 7-22 the basis row (4 x 16 floats)
+    __shared__ uint4 lb[2][kLbChunk * kLbPieces];
+    __shared__ int lsec[2][kLbChunk];
+    const int tb = threadIdx.x;
+    auto fetch = [&](int c0, uint4 (&r)[3], int& sf) {
+#pragma unroll
+        for (int u = 0; u < 3; ++u) {
+            const int k = tb + u * 256;
+            r[u] = make_uint4(0u, 0u, 0u, 0u);
+            const int cc = k / kLbPieces, pc = k - cc * kLbPieces;
+            const int i = c0 + cc;
+            if (k < kLbChunk * kLbPieces && i < hi) {
+                const uint4* src = pc < 3 ? reinterpret_cast<const uint4*>(S.ph + i) + pc
+                                 : pc < 7 ? reinterpret_cast<const uint4*>(S.cr + i) + (pc - 3)
+                                          : reinterpret_cast<const uint4*>(J.lin_c + (int64_t)i * (4 * LRM)) + (pc - 7);
+                r[u] = *src;
+            }
+        }
+        sf = 0;
+        if (tb < kLbChunk && c0 + tb < hi) {
+            const int i = c0 + tb;
+            sf = secf && ((reinterpret_cast<const uint32_t*>(S.rowActive)[i >> 2] >> (8 * (i & 3))) & 0xffu) != 0;
+        }
+    };
+    auto stash = [&](int b, const uint4 (&r)[3], int sf) {
+#pragma unroll
+        for (int u = 0; u < 3; ++u) {
+            const int k = tb + u * 256;
+            if (k < kLbChunk * kLbPieces) lb[b][k] = r[u];
+        }
+        if (tb < kLbChunk) lsec[b][tb] = sf;
+    };
+    uint4 rg[3];
+    int sfr = 0;
+    if (lo < hi) {
+        fetch(lo, rg, sfr);
+        stash(0, rg, sfr);
+    }
+    __syncthreads();
+    int b = 0;
+    for (int c0 = lo; c0 < hi; c0 += kLbChunk) {          // block-uniform
+        const int c1 = c0 + kLbChunk;
+        if (c1 < hi) fetch(c1, rg, sfr);                   // in flight during this chunk
+        const int nc = min(kLbChunk, hi - c0);
+        const uint4* L = lb[b];
+        auto phase_cs = [&](int j, float& cj, float& sj) {
+            const uint4 a0 = L[j * kLbPieces], a1 = L[j * kLbPieces + 1], a2 = L[j * kLbPieces + 2];
+            PhaseRec p;
+            p.k = __hiloint2double((int)a0.y, (int)a0.x);
+            p.cth = __hiloint2double((int)a0.w, (int)a0.z);
+            p.sth = __hiloint2double((int)a1.y, (int)a1.x);
+            p.omega = __hiloint2double((int)a1.w, (int)a1.z);
+            p.ph0 = __hiloint2double((int)a2.y, (int)a2.x);
+            cs_of(phase_of(p, dxp, dyp, t), cj, sj);
+        };
+        float ci, si;
+        phase_cs(0, ci, si);
+        for (int j = 0; j < nc; ++j) {
+            float cn = 0.f, sn = 0.f;
+            if (j + 1 < nc) phase_cs(j + 1, cn, sn);       // next component, independent of this one's sums
+            const uint4* P = L + j * kLbPieces;
+            const uint4 r0 = P[3], r1 = P[4], r2 = P[5], r3 = P[6];
+            const float lu = __uint_as_float(r0.x), lv = __uint_as_float(r0.y), lw = __uint_as_float(r0.z);
+            const float gxz = __uint_as_float(r1.x), gyz = __uint_as_float(r1.y), gzz = __uint_as_float(r1.z);
+            const float dfu = __uint_as_float(r2.z), dfv = __uint_as_float(r2.w), dfw = __uint_as_float(r3.x);
+            const bool sec = lsec[b][j] != 0;
+            const float cr = ci * ri, sr = si * ri;
+            gxr = fmaR(gxz, cr, gxr);
+            gyr = fmaR(gyz, cr, gyr);
+            gzr = fmaR(gzz, sr, gzr);
+            // first-order terms on the profile of k, diagonal on that of 2k
+            const float xu = lin ? lu * cr : 0.f, xv = lin ? lv * cr : 0.f, xw = lin ? lw * sr : 0.f;
+            const float c2 = sec ? (ci * ci - si * si) * ri : 0.f, s2 = sec ? 2.f * si * ci * ri : 0.f;
+            const float du = dfu * c2, dv = dfv * c2, dw = dfw * s2;
+            const float* cf = reinterpret_cast<const float*>(P + 7);
+#pragma unroll
+            for (int q = 0; q < LRM; ++q) {
+                const float a = cf[q], a2 = cf[LRM + q], bq = cf[2 * LRM + q], b2 = cf[3 * LRM + q];
+                LU[q] = fmaR(xu, a, fmaR(du, a2, LU[q]));
+                LV[q] = fmaR(xv, a, fmaR(dv, a2, LV[q]));
+                LW[q] = fmaR(xw, bq, fmaR(dw, b2, LW[q]));
+            }
+            ci = cn;
+            si = sn;
+        }
+        if (c1 < hi) stash(b ^ 1, rg, sfr);                // the other buffer's readers passed the last barrier
+        __syncthreads();
+        b ^= 1;
+    }
+}
+
 // Per (column, chunk of LC levels): first order + second order + Taylor over
 // the pair records of segment blockIdx.y.
 // MODE 0 (GRID, lsgrid_spline.cpp:708-721): z from the sigma map of the
@@ -363,7 +470,10 @@ __device__ __forceinline__ void levels_body(const SeaDev<R>& S, const LevelJob<R
         const PhaseRec p0 = ld_rec<R>(S.ph, 0);
         ri_f8 = (R)freq_ramp(t, p0.rt, p0.ts, p0.du);
     }
-    for (int i = ia; i < ib; ++i) {
+    if constexpr (LB) {
+        lb_component_sums<LRM>(S, J, sg.own_lo, sg.own_hi, dxp, dyp, t, ri_f8, lin, secf, LU, LV, LW, gxr, gyr, gzr);
+    }
+    for (int i = ia; !LB && i < ib; ++i) {
         const bool own = i >= sg.own_lo && i < sg.own_hi;
         const bool sec = secf && (direct || (sizeof(R) == 4 ? row_active(S.rowActive, i) : S.rowActive[i] != 0));
         // (form-assisted fills, F8, launch only with the forms: no pair rows)
@@ -380,26 +490,7 @@ __device__ __forceinline__ void levels_body(const SeaDev<R>& S, const LevelJob<R
 #pragma unroll
             for (int l = 0; l < LC; ++l) ur[l] = vr[l] = wr[l] = 0;
         }
-        if (LB && own) {
-            gxr = fmaR(c.gxz, cr, gxr);
-            gyr = fmaR(c.gyz, cr, gyr);
-            gzr = fmaR(c.gzz, sr, gzr);
-            // first-order terms on the profile of k, diagonal on that of 2k
-            const R xu = lin ? c.lu * cr : (R)0, xv = lin ? c.lv * cr : (R)0, xw = lin ? c.lw * sr : (R)0;
-            const R c2 = sec ? (ci * ci - si * si) * ri : (R)0, s2 = sec ? (R)2 * si * ci * ri : (R)0;
-            const R du = c.dfu * c2, dv = c.dfv * c2, dw = c.dfw * s2;
-            // the table holds LRM entries per family (zeros past the rank), so
-            // the loop has no branch and its scalar loads go out together
-            const float* cf = J.lin_c + (int64_t)i * (4 * LRM);
-#pragma unroll
-            for (int q = 0; q < LQ; ++q) {
-                const R a = ld_const(cf, q), a2 = ld_const(cf, LRM + q);
-                const R b = ld_const(cf, 2 * LRM + q), b2 = ld_const(cf, 3 * LRM + q);
-                LU[q] = fmaR(xu, a, fmaR(du, a2, LU[q]));
-                LV[q] = fmaR(xv, a, fmaR(dv, a2, LV[q]));
-                LW[q] = fmaR(xw, b, fmaR(dw, b2, LW[q]));
-            }
-        } else if (own) {
+        if (own) {
             gx += (double)(c.gxz * cr);
             gy += (double)(c.gyz * cr);
             gz += (double)(c.gzz * sr);
