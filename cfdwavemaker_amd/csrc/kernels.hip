@@ -866,6 +866,21 @@ __global__ __launch_bounds__(256) void k_grad_dt(const R* __restrict__ D, R* __r
     }
 }
 
+// Store N values (N a multiple of 4) at a 16-byte aligned address as 16-byte
+// vectors.
+template <int N, typename R>
+__device__ __forceinline__ void store_rec(R* __restrict__ p, const R (&r)[N]) {
+    if constexpr (sizeof(R) == 4) {
+        float4* q = reinterpret_cast<float4*>(p);
+#pragma unroll
+        for (int k = 0; k < N / 4; ++k) q[k] = make_float4(r[4 * k], r[4 * k + 1], r[4 * k + 2], r[4 * k + 3]);
+    } else {
+        double2* q = reinterpret_cast<double2*>(p);
+#pragma unroll
+        for (int k = 0; k < N / 2; ++k) q[k] = make_double2(r[2 * k], r[2 * k + 1]);
+    }
+}
+
 // ================================================================ spline query
 // The two time slots a query reads, (tstep+1)%4 and (tstep+2)%4, are packed
 // after every fill/update by k_pack:
@@ -879,8 +894,9 @@ __global__ __launch_bounds__(256) void k_pack(PackJob<R> J) {
     const int64_t per = (int64_t)J.nx * J.ny * J.nl;
     const int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (n >= per) return;
-    R* c = J.pk_C + 16 * n;
-    R* d = J.pk_D + 16 * n;
+    // records are assembled in registers and stored as 16-byte vectors (the
+    // packed arrays are 256-byte aligned allocations, records 16 R long)
+    R c[16], d[16];
 #pragma unroll
     for (int f = 0; f < 3; ++f) {
         c[5 * f + 0] = J.F[f][J.tid1 * per + n];
@@ -896,9 +912,11 @@ __global__ __launch_bounds__(256) void k_pack(PackJob<R> J) {
     }
     c[15] = 0;
     d[15] = 0;
+    store_rec<16>(J.pk_C + 16 * n, c);
+    store_rec<16>(J.pk_D + 16 * n, d);
     if (n % J.nl == 0) {
         const int64_t pc = (int64_t)J.nx * J.ny, col = n / J.nl;
-        R* e = J.pk_eta + 8 * col;
+        R e[8];
         e[0] = J.E[J.tid1 * pc + col];
         e[1] = J.E[J.tid2 * pc + col];
         e[2] = J.Et[J.tid1 * pc + col];
@@ -907,6 +925,7 @@ __global__ __launch_bounds__(256) void k_pack(PackJob<R> J) {
         e[5] = J.Ex[pc + col];
         e[6] = J.Ey[col];
         e[7] = J.Ey[pc + col];
+        store_rec<8>(J.pk_eta + 8 * col, e);
     }
 }
 
