@@ -68,6 +68,21 @@ __device__ __forceinline__ R fmaR(R a, R b, R c) { return a * b + c; }
 __device__ __forceinline__ double pexp(double a) { return exp(a); }
 __device__ __forceinline__ float pexp(float a) { return __builtin_amdgcn_exp2f(a); }
 
+// Store N values (N a multiple of 4) at a 16-byte aligned address as 16-byte
+// vectors.
+template <int N, typename R>
+__device__ __forceinline__ void store_rec(R* __restrict__ p, const R (&r)[N]) {
+    if constexpr (sizeof(R) == 4) {
+        float4* q = reinterpret_cast<float4*>(p);
+#pragma unroll
+        for (int k = 0; k < N / 4; ++k) q[k] = make_float4(r[4 * k], r[4 * k + 1], r[4 * k + 2], r[4 * k + 3]);
+    } else {
+        double2* q = reinterpret_cast<double2*>(p);
+#pragma unroll
+        for (int k = 0; k < N / 2; ++k) q[k] = make_double2(r[2 * k], r[2 * k + 1]);
+    }
+}
+
 }  // namespace
 
 // =================================================== thread -> column mapping
@@ -663,51 +678,103 @@ __device__ __forceinline__ void levels_body(const SeaDev<R>& S, const LevelJob<R
         const int64_t cq = PP.cq;
         const int Rk = seg0 ? J.qf_rank : 0, Rw = seg0 ? J.qf_rank_w : 0;
         const int nlev = min(LC, J.g.nl - PP.l0);
-        for (int l = 0; l < LC; ++l) {
-            if (l >= nlev) {
-                if (po) po[l] = po[LC + l] = po[2 * LC + l] = 0.;
-                continue;
+        // four levels at a time: each form value and basis exponent is loaded
+        // once for the four, and the outputs leave as 16-byte vectors when the
+        // group is whole and aligned (level counts that are multiples of 4);
+        // per level the operations and their order are those of one level alone
+        static_assert(LC % 4 == 0, "level groups of four");
+        const bool vec = (o & 3) == 0;
+        for (int lg = 0; lg < LC; lg += 4) {
+            double zz[4];
+            R z[4];
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                const int lv = min(PP.l0 + lg + k, J.g.nl - 1);
+                zz[k] = PP.e + J.sigma[lv] * (J.g.h + PP.e);
+                z[k] = (R)(zz[k] < 0. ? zz[k] : 0.);                      // min(0, z), as zp
             }
-            const double zz = PP.e + J.sigma[PP.l0 + l] * (J.g.h + PP.e);
-            const R z = (R)(zz < 0. ? zz : 0.);                          // min(0, z), as zp
-            R su = 0, sv = 0, sw = 0;
+            R su[4] = {0, 0, 0, 0}, sv[4] = {0, 0, 0, 0}, sw[4] = {0, 0, 0, 0};
 #pragma unroll
             for (int q = 0; q < LQ; ++q) {
                 if (q < J.lin_R) {
-                    R E, G;
-                    prof((R)ld_const(J.lin_mu, q), (R)ld_const(J.lin_mu, LRM + q), z, E, G);
-                    su = fmaR(LU[q], E + G, su);
-                    sv = fmaR(LV[q], E + G, sv);
+                    const R m = (R)ld_const(J.lin_mu, q), g = (R)ld_const(J.lin_mu, LRM + q);
+#pragma unroll
+                    for (int k = 0; k < 4; ++k) {
+                        R E, G;
+                        prof(m, g, z[k], E, G);
+                        su[k] = fmaR(LU[q], E + G, su[k]);
+                        sv[k] = fmaR(LV[q], E + G, sv[k]);
+                    }
                 }
                 if (q < J.lin_Rw) {
-                    R E, G;
-                    prof((R)ld_const(J.lin_mu, 2 * LRM + q), (R)ld_const(J.lin_mu, 3 * LRM + q), z, E, G);
-                    sw = fmaR(LW[q], E - G, sw);
+                    const R m = (R)ld_const(J.lin_mu, 2 * LRM + q), g = (R)ld_const(J.lin_mu, 3 * LRM + q);
+#pragma unroll
+                    for (int k = 0; k < 4; ++k) {
+                        R E, G;
+                        prof(m, g, z[k], E, G);
+                        sw[k] = fmaR(LW[q], E - G, sw[k]);
+                    }
                 }
             }
-            R fu = 0, fv = 0, fw = 0;
+            R fu[4] = {0, 0, 0, 0}, fv[4] = {0, 0, 0, 0}, fw[4] = {0, 0, 0, 0};
             for (int q = 0; q < Rk; ++q) {
-                R E, G;
-                prof((R)J.qf_mu2[q], (R)J.qf_mg[q], z, E, G);
-                fu = fmaR((R)J.qf[(int64_t)q * J.qf_stride + cq], E + G, fu);
-                fv = fmaR((R)J.qf[(int64_t)(Rk + q) * J.qf_stride + cq], E + G, fv);
+                const R m2 = (R)J.qf_mu2[q], mg = (R)J.qf_mg[q];
+                const R qu = (R)J.qf[(int64_t)q * J.qf_stride + cq], qv = (R)J.qf[(int64_t)(Rk + q) * J.qf_stride + cq];
+#pragma unroll
+                for (int k = 0; k < 4; ++k) {
+                    R E, G;
+                    prof(m2, mg, z[k], E, G);
+                    fu[k] = fmaR(qu, E + G, fu[k]);
+                    fv[k] = fmaR(qv, E + G, fv[k]);
+                }
             }
             for (int q = 0; q < Rw; ++q) {
-                R E, G;
-                prof((R)J.qf_mu2w[q], (R)J.qf_mgw[q], z, E, G);
-                fw = fmaR((R)J.qf[(int64_t)(2 * J.qf_rank + q) * J.qf_stride + cq], E - G, fw);
+                const R m2 = (R)J.qf_mu2w[q], mg = (R)J.qf_mgw[q];
+                const R qw = (R)J.qf[(int64_t)(2 * J.qf_rank + q) * J.qf_stride + cq];
+#pragma unroll
+                for (int k = 0; k < 4; ++k) {
+                    R E, G;
+                    prof(m2, mg, z[k], E, G);
+                    fw[k] = fmaR(qw, E - G, fw[k]);
+                }
             }
-            const double u = (double)su + sc * (double)fu, v = (double)sv + sc * (double)fv,
-                         w = (double)sw + sc * (double)fw;
+            double u[4], v[4], w[4];
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                const bool in = lg + k < nlev;
+                u[k] = in ? (double)su[k] + sc * (double)fu[k] : 0.;
+                v[k] = in ? (double)sv[k] + sc * (double)fv[k] : 0.;
+                w[k] = in ? (double)sw[k] + sc * (double)fw[k] : 0.;
+            }
             if (po) {
-                po[l] = u;
-                po[LC + l] = v;
-                po[2 * LC + l] = w;
+                double2* pu = reinterpret_cast<double2*>(po + lg);
+                double2* pv = reinterpret_cast<double2*>(po + LC + lg);
+                double2* pw = reinterpret_cast<double2*>(po + 2 * LC + lg);
+                pu[0] = make_double2(u[0], u[1]); pu[1] = make_double2(u[2], u[3]);
+                pv[0] = make_double2(v[0], v[1]); pv[1] = make_double2(v[2], v[3]);
+                pw[0] = make_double2(w[0], w[1]); pw[1] = make_double2(w[2], w[3]);
             } else {
-                const double zt = zz - J.g.swl > 0. ? zz - J.g.swl : 0.;    // max(0, z - swl)
-                J.o_u[o + l] = (R)(u + gx * zt);
-                J.o_v[o + l] = (R)(v + gy * zt);
-                J.o_w[o + l] = (R)(w + gz * zt);
+                R ou[4], ov[4], ow[4];
+#pragma unroll
+                for (int k = 0; k < 4; ++k) {
+                    const double zt = zz[k] - J.g.swl > 0. ? zz[k] - J.g.swl : 0.;   // max(0, z - swl)
+                    ou[k] = (R)(u[k] + gx * zt);
+                    ov[k] = (R)(v[k] + gy * zt);
+                    ow[k] = (R)(w[k] + gz * zt);
+                }
+                if (vec && lg + 4 <= nlev) {
+                    store_rec<4>(J.o_u + o + lg, ou);
+                    store_rec<4>(J.o_v + o + lg, ov);
+                    store_rec<4>(J.o_w + o + lg, ow);
+                } else {
+#pragma unroll
+                    for (int k = 0; k < 4; ++k) {
+                        if (lg + k >= nlev) break;
+                        J.o_u[o + lg + k] = ou[k];
+                        J.o_v[o + lg + k] = ov[k];
+                        J.o_w[o + lg + k] = ow[k];
+                    }
+                }
             }
         }
         if (po) {
@@ -863,21 +930,6 @@ __global__ __launch_bounds__(256) void k_grad_dt(const R* __restrict__ D, R* __r
     } else {
         GD[1 * per + e] = (R)((D[2 * per + e] - D[1 * per + e]) / dt);
         GD[2 * per + e] = (R)((D[3 * per + e] - D[1 * per + e]) / (2 * dt));
-    }
-}
-
-// Store N values (N a multiple of 4) at a 16-byte aligned address as 16-byte
-// vectors.
-template <int N, typename R>
-__device__ __forceinline__ void store_rec(R* __restrict__ p, const R (&r)[N]) {
-    if constexpr (sizeof(R) == 4) {
-        float4* q = reinterpret_cast<float4*>(p);
-#pragma unroll
-        for (int k = 0; k < N / 4; ++k) q[k] = make_float4(r[4 * k], r[4 * k + 1], r[4 * k + 2], r[4 * k + 3]);
-    } else {
-        double2* q = reinterpret_cast<double2*>(p);
-#pragma unroll
-        for (int k = 0; k < N / 2; ++k) q[k] = make_double2(r[2 * k], r[2 * k + 1]);
     }
 }
 
