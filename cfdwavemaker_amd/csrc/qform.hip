@@ -54,17 +54,19 @@ __device__ __forceinline__ float bf16_half(uint2 v, int e) {
 
 // ------------------------------------------------------------------ Z operands
 // One thread per (point, 8 consecutive components): FP64 phase and range
-// reduction, hardware sin/cos, bf16 hi/lo split, 16-byte stores.
+// reduction, hardware sin/cos, bf16 hi/lo split, 16-byte stores.  A chunk
+// holds at most 2^28 (point, component) entries (its Z arrays stay under
+// 2 GiB), so thread, tile and s-grid column indices fit 32-bit arithmetic.
 __global__ __launch_bounds__(256) void k_qf_zgen(QfJob J) {
-    const int64_t tid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const int nkb = J.npad >> 4;
-    const int64_t ntile = (J.ppad >> 5) * nkb;
-    if ((tid >> 6) >= ntile) return;
-    const int64_t tile = tid >> 6;
+    const uint32_t tid = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t nkb = (uint32_t)J.npad >> 4;
+    const uint32_t ntile = (uint32_t)(J.ppad >> 5) * nkb;
+    const uint32_t tile = tid >> 6;
+    if (tile >= ntile) return;
     const int h = (int)((tid >> 5) & 1), r = (int)(tid & 31);
-    const int64_t rb = tile / nkb;
+    const uint32_t rb = tile / nkb;
     const int kb = (int)(tile - rb * nkb);
-    const int64_t p = rb * 32 + r;              // point within the chunk
+    const int64_t p = (int64_t)rb * 32 + r;     // point within the chunk
     const int m0 = kb * 16 + h * 8;
     uint32_t ch[8], cl[8], sh[8], sl[8];
     const bool live = p < J.P;
@@ -72,11 +74,14 @@ __global__ __launch_bounds__(256) void k_qf_zgen(QfJob J) {
     if (live) {
         const int64_t q = J.p0 + p;
         if (J.grid) {
-            const int slot = (int)(q / J.ncol);
-            const int64_t cq = q - (int64_t)slot * J.ncol;
-            const int64_t col = J.cols ? J.cols[cq] : cq;
-            x = J.g.x0 + J.g.dx * (double)(col / J.g.ny);
-            y = J.g.y0 + J.g.dy * (double)(col % J.g.ny);
+            // q < 4 ncol and col < nx ny: 32-bit
+            const uint32_t ncol = (uint32_t)J.ncol, ny = (uint32_t)J.g.ny;
+            const uint32_t slot = (uint32_t)q / ncol;
+            const uint32_t cq = (uint32_t)q - slot * ncol;
+            const uint32_t col = J.cols ? (uint32_t)J.cols[cq] : cq;
+            const uint32_t ci = col / ny;
+            x = J.g.x0 + J.g.dx * (double)ci;
+            y = J.g.y0 + J.g.dy * (double)(col - ci * ny);
             t = J.slot_t[slot];
         } else {
             x = J.x[q]; y = J.y[q]; t = J.t[q];
@@ -102,7 +107,7 @@ __global__ __launch_bounds__(256) void k_qf_zgen(QfJob J) {
     auto pack = [](const uint32_t* v) {
         return make_uint4(v[0] | (v[1] << 16), v[2] | (v[3] << 16), v[4] | (v[5] << 16), v[6] | (v[7] << 16));
     };
-    const int64_t o = tile * 512 + h * 256 + r * 8;
+    const int64_t o = (int64_t)tile * 512 + h * 256 + r * 8;
     *reinterpret_cast<uint4*>(J.zc_hi + o) = pack(ch);
     *reinterpret_cast<uint4*>(J.zc_lo + o) = pack(cl);
     *reinterpret_cast<uint4*>(J.zs_hi + o) = pack(sh);
@@ -452,6 +457,9 @@ hipError_t launch_qf_zgen(const QfJob& J, hipStream_t st) {
     if (J.P <= 0) return hipSuccess;
     if (!qf_shape_ok(J)) return hipErrorInvalidValue;
     const int64_t nthr = (J.ppad >> 5) * (J.npad >> 4) * 64;
+    // the kernel's 32-bit indices: at most 2^28 Z entries per chunk, and an
+    // s-grid chunk's points within 4 slots of ncol columns
+    if ((int64_t)J.ppad * J.npad > ((int64_t)1 << 28) || (J.grid && J.p0 + J.P > ((int64_t)1 << 31))) return hipErrorInvalidValue;
     hipLaunchKernelGGL(k_qf_zgen, dim3((unsigned)((nthr + 255) / 256)), dim3(256), 0, st, J);
     return hipGetLastError();
 }
