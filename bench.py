@@ -695,8 +695,8 @@ def main():
         abi_worker(a.abi_worker)
         return
     if a.gpus != 1:
-        print(f"bench.py runs on one GPU: the s-grid fill shards by columns across GPUs "
-              f"(see DESIGN.md), but only N=1 is measured here; got --gpus {a.gpus}", file=sys.stderr)
+        print(f"bench.py runs on one GPU: the s-grid fill would shard by columns across GPUs, "
+              f"but only N=1 is measured here; got --gpus {a.gpus}", file=sys.stderr)
         sys.exit(2)
 
     import torch

@@ -366,7 +366,7 @@ CFDWM_EXPORT int cfdwm_depth_basis(double kmax, double tmax, double tol, int rma
 CFDWM_EXPORT int cfdwm_profile_basis(double kmax, double h, int sign, double tol, int rmax, double* mu, double* zs,
                                      double* err);
 /* ---- Learned second-order surrogate: per-column MLP (no reference
- * counterpart; the exact kernels above stay the fallback and the judge).
+ * counterpart; the exact kernels above stay the fallback and the check).
  * L = nhidden + 1 linear layers (nin -> width -> ... -> nout), GELU (erf
  * form) after all but the last; FP32 master parameters, bf16 products with
  * FP32 accumulation on the matrix cores; inputs / outputs standardised with
