@@ -292,6 +292,112 @@ __device__ __forceinline__ void lev_store(const SeaDev<R>& S, const LevelJob<R>&
     }
 }
 
+// The per-level evaluation of the linear-term-basis mode, once per column:
+// the basis sums LU, LV, LW (all components), the form values and the Taylor
+// gradients give u, v, w at the column's levels, written to the s-grid.
+template <int LC>
+__device__ __forceinline__ void lb_levels_out(const LevelJob<float>& J, const LevPos<float, LC, true>& PP,
+                                              const float (&LU)[16], const float (&LV)[16], const float (&LW)[16],
+                                              double gx, double gy, double gz) {
+    using R = float;
+    constexpr int LRM = 16, LQ = 16;
+    const int64_t o = ((int64_t)J.slot_dst[PP.slot] * J.ncell + PP.col) * J.g.nl + PP.l0;
+    const double sc = J.qf_scale[PP.slot];
+    const int64_t cq = PP.cq;
+    const int Rk = J.qf_rank, Rw = J.qf_rank_w;
+    const int nlev = min(LC, J.g.nl - PP.l0);
+    // four levels at a time: each form value and basis exponent is loaded
+    // once for the four, and the outputs leave as 16-byte vectors when the
+    // group is whole and aligned (level counts that are multiples of 4);
+    // per level the operations and their order are those of one level alone
+    static_assert(LC % 4 == 0, "level groups of four");
+    const bool vec = (o & 3) == 0;
+    for (int lg = 0; lg < LC; lg += 4) {
+        double zz[4];
+        R z[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const int lv = min(PP.l0 + lg + k, J.g.nl - 1);
+            zz[k] = PP.e + J.sigma[lv] * (J.g.h + PP.e);
+            z[k] = (R)(zz[k] < 0. ? zz[k] : 0.);                      // min(0, z), as zp
+        }
+        R su[4] = {0, 0, 0, 0}, sv[4] = {0, 0, 0, 0}, sw[4] = {0, 0, 0, 0};
+#pragma unroll
+        for (int q = 0; q < LQ; ++q) {
+            if (q < J.lin_R) {
+                const R m = (R)ld_const(J.lin_mu, q), g = (R)ld_const(J.lin_mu, LRM + q);
+#pragma unroll
+                for (int k = 0; k < 4; ++k) {
+                    R E, G;
+                    prof(m, g, z[k], E, G);
+                    su[k] = fmaR(LU[q], E + G, su[k]);
+                    sv[k] = fmaR(LV[q], E + G, sv[k]);
+                }
+            }
+            if (q < J.lin_Rw) {
+                const R m = (R)ld_const(J.lin_mu, 2 * LRM + q), g = (R)ld_const(J.lin_mu, 3 * LRM + q);
+#pragma unroll
+                for (int k = 0; k < 4; ++k) {
+                    R E, G;
+                    prof(m, g, z[k], E, G);
+                    sw[k] = fmaR(LW[q], E - G, sw[k]);
+                }
+            }
+        }
+        R fu[4] = {0, 0, 0, 0}, fv[4] = {0, 0, 0, 0}, fw[4] = {0, 0, 0, 0};
+        for (int q = 0; q < Rk; ++q) {
+            const R m2 = (R)J.qf_mu2[q], mg = (R)J.qf_mg[q];
+            const R qu = (R)J.qf[(int64_t)q * J.qf_stride + cq], qv = (R)J.qf[(int64_t)(Rk + q) * J.qf_stride + cq];
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                R E, G;
+                prof(m2, mg, z[k], E, G);
+                fu[k] = fmaR(qu, E + G, fu[k]);
+                fv[k] = fmaR(qv, E + G, fv[k]);
+            }
+        }
+        for (int q = 0; q < Rw; ++q) {
+            const R m2 = (R)J.qf_mu2w[q], mg = (R)J.qf_mgw[q];
+            const R qw = (R)J.qf[(int64_t)(2 * J.qf_rank + q) * J.qf_stride + cq];
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                R E, G;
+                prof(m2, mg, z[k], E, G);
+                fw[k] = fmaR(qw, E - G, fw[k]);
+            }
+        }
+        double u[4], v[4], w[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const bool in = lg + k < nlev;
+            u[k] = in ? (double)su[k] + sc * (double)fu[k] : 0.;
+            v[k] = in ? (double)sv[k] + sc * (double)fv[k] : 0.;
+            w[k] = in ? (double)sw[k] + sc * (double)fw[k] : 0.;
+        }
+        R ou[4], ov[4], ow[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const double zt = zz[k] - J.g.swl > 0. ? zz[k] - J.g.swl : 0.;   // max(0, z - swl)
+            ou[k] = (R)(u[k] + gx * zt);
+            ov[k] = (R)(v[k] + gy * zt);
+            ow[k] = (R)(w[k] + gz * zt);
+        }
+        if (vec && lg + 4 <= nlev) {
+            store_rec<4>(J.o_u + o + lg, ou);
+            store_rec<4>(J.o_v + o + lg, ov);
+            store_rec<4>(J.o_w + o + lg, ow);
+        } else {
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                if (lg + k >= nlev) break;
+                J.o_u[o + lg + k] = ou[k];
+                J.o_v[o + lg + k] = ov[k];
+                J.o_w[o + lg + k] = ow[k];
+            }
+        }
+    }
+}
+
 // Component loop of the linear-term-basis mode (levels_body, F8 == 2): per
 // component one phase, cos/sin, and the 16 x 6 products of the basis
 // coefficients.  The component records (phase constants, coefficients and the
@@ -302,6 +408,7 @@ __device__ __forceinline__ void lev_store(const SeaDev<R>& S, const LevelJob<R>&
 // waiting for its records once per component.  Same operations and order as
 // the per-component form, so the sums are the same bit for bit.
 constexpr int kLbChunk = 32, kLbPieces = 23;   // components per chunk, 16-byte pieces per component
+constexpr int kLbRec = 52;                     // floats of a segment's partial record: LU, LV, LW, gx, gy, gz, pad
 template <int LRM>
 __device__ __forceinline__ void lb_component_sums(const SeaDev<float>& S, const LevelJob<float>& J, int lo, int hi,
                                                   double dxp, double dyp, double t, float ri, bool lin, bool secf,
@@ -661,126 +768,22 @@ __device__ __forceinline__ void levels_body(const SeaDev<R>& S, const LevelJob<R
         }
     }
     if constexpr (LB) {
-        // all levels of the column in one thread: per level the linear-term
-        // bases, the forms (segment 0) and the Taylor extension, stored at
-        // once, or as the segment's partial record (U, V, W per level, gx, gy,
-        // gz) for k_levels_reduce
+        // all levels of the column in one thread: the segment's basis sums
+        // (LU, LV, LW and the Taylor gradients, kLbRec floats) go to
+        // k_levels_reduce_lb when the components are split over segments;
+        // otherwise the levels are evaluated here
         if (!PP.live) return;
-        gx = (double)gxr; gy = (double)gyr; gz = (double)gzr;
-        const bool seg0 = blockIdx.y == 0;
-        double* po = nullptr;
         if (J.partial) {
             const int64_t nthr = ((int64_t)J.nslot * J.ncol + 63) / 64 * 64 * ((J.g.nl + LC - 1) / LC);
-            po = J.partial + ((int64_t)blockIdx.y * nthr + tid) * RS;
+            float* pf = reinterpret_cast<float*>(J.partial) + ((int64_t)blockIdx.y * nthr + tid) * kLbRec;
+            store_rec<16>(pf, LU);
+            store_rec<16>(pf + 16, LV);
+            store_rec<16>(pf + 32, LW);
+            const float g4[4] = {gxr, gyr, gzr, 0.f};
+            store_rec<4>(pf + 48, g4);
+            return;
         }
-        const int64_t o = ((int64_t)J.slot_dst[PP.slot] * J.ncell + PP.col) * J.g.nl + PP.l0;
-        const double sc = J.qf_scale[PP.slot];
-        const int64_t cq = PP.cq;
-        const int Rk = seg0 ? J.qf_rank : 0, Rw = seg0 ? J.qf_rank_w : 0;
-        const int nlev = min(LC, J.g.nl - PP.l0);
-        // four levels at a time: each form value and basis exponent is loaded
-        // once for the four, and the outputs leave as 16-byte vectors when the
-        // group is whole and aligned (level counts that are multiples of 4);
-        // per level the operations and their order are those of one level alone
-        static_assert(LC % 4 == 0, "level groups of four");
-        const bool vec = (o & 3) == 0;
-        for (int lg = 0; lg < LC; lg += 4) {
-            double zz[4];
-            R z[4];
-#pragma unroll
-            for (int k = 0; k < 4; ++k) {
-                const int lv = min(PP.l0 + lg + k, J.g.nl - 1);
-                zz[k] = PP.e + J.sigma[lv] * (J.g.h + PP.e);
-                z[k] = (R)(zz[k] < 0. ? zz[k] : 0.);                      // min(0, z), as zp
-            }
-            R su[4] = {0, 0, 0, 0}, sv[4] = {0, 0, 0, 0}, sw[4] = {0, 0, 0, 0};
-#pragma unroll
-            for (int q = 0; q < LQ; ++q) {
-                if (q < J.lin_R) {
-                    const R m = (R)ld_const(J.lin_mu, q), g = (R)ld_const(J.lin_mu, LRM + q);
-#pragma unroll
-                    for (int k = 0; k < 4; ++k) {
-                        R E, G;
-                        prof(m, g, z[k], E, G);
-                        su[k] = fmaR(LU[q], E + G, su[k]);
-                        sv[k] = fmaR(LV[q], E + G, sv[k]);
-                    }
-                }
-                if (q < J.lin_Rw) {
-                    const R m = (R)ld_const(J.lin_mu, 2 * LRM + q), g = (R)ld_const(J.lin_mu, 3 * LRM + q);
-#pragma unroll
-                    for (int k = 0; k < 4; ++k) {
-                        R E, G;
-                        prof(m, g, z[k], E, G);
-                        sw[k] = fmaR(LW[q], E - G, sw[k]);
-                    }
-                }
-            }
-            R fu[4] = {0, 0, 0, 0}, fv[4] = {0, 0, 0, 0}, fw[4] = {0, 0, 0, 0};
-            for (int q = 0; q < Rk; ++q) {
-                const R m2 = (R)J.qf_mu2[q], mg = (R)J.qf_mg[q];
-                const R qu = (R)J.qf[(int64_t)q * J.qf_stride + cq], qv = (R)J.qf[(int64_t)(Rk + q) * J.qf_stride + cq];
-#pragma unroll
-                for (int k = 0; k < 4; ++k) {
-                    R E, G;
-                    prof(m2, mg, z[k], E, G);
-                    fu[k] = fmaR(qu, E + G, fu[k]);
-                    fv[k] = fmaR(qv, E + G, fv[k]);
-                }
-            }
-            for (int q = 0; q < Rw; ++q) {
-                const R m2 = (R)J.qf_mu2w[q], mg = (R)J.qf_mgw[q];
-                const R qw = (R)J.qf[(int64_t)(2 * J.qf_rank + q) * J.qf_stride + cq];
-#pragma unroll
-                for (int k = 0; k < 4; ++k) {
-                    R E, G;
-                    prof(m2, mg, z[k], E, G);
-                    fw[k] = fmaR(qw, E - G, fw[k]);
-                }
-            }
-            double u[4], v[4], w[4];
-#pragma unroll
-            for (int k = 0; k < 4; ++k) {
-                const bool in = lg + k < nlev;
-                u[k] = in ? (double)su[k] + sc * (double)fu[k] : 0.;
-                v[k] = in ? (double)sv[k] + sc * (double)fv[k] : 0.;
-                w[k] = in ? (double)sw[k] + sc * (double)fw[k] : 0.;
-            }
-            if (po) {
-                double2* pu = reinterpret_cast<double2*>(po + lg);
-                double2* pv = reinterpret_cast<double2*>(po + LC + lg);
-                double2* pw = reinterpret_cast<double2*>(po + 2 * LC + lg);
-                pu[0] = make_double2(u[0], u[1]); pu[1] = make_double2(u[2], u[3]);
-                pv[0] = make_double2(v[0], v[1]); pv[1] = make_double2(v[2], v[3]);
-                pw[0] = make_double2(w[0], w[1]); pw[1] = make_double2(w[2], w[3]);
-            } else {
-                R ou[4], ov[4], ow[4];
-#pragma unroll
-                for (int k = 0; k < 4; ++k) {
-                    const double zt = zz[k] - J.g.swl > 0. ? zz[k] - J.g.swl : 0.;   // max(0, z - swl)
-                    ou[k] = (R)(u[k] + gx * zt);
-                    ov[k] = (R)(v[k] + gy * zt);
-                    ow[k] = (R)(w[k] + gz * zt);
-                }
-                if (vec && lg + 4 <= nlev) {
-                    store_rec<4>(J.o_u + o + lg, ou);
-                    store_rec<4>(J.o_v + o + lg, ov);
-                    store_rec<4>(J.o_w + o + lg, ow);
-                } else {
-#pragma unroll
-                    for (int k = 0; k < 4; ++k) {
-                        if (lg + k >= nlev) break;
-                        J.o_u[o + lg + k] = ou[k];
-                        J.o_v[o + lg + k] = ov[k];
-                        J.o_w[o + lg + k] = ow[k];
-                    }
-                }
-            }
-        }
-        if (po) {
-            po[3 * LC] = gx; po[3 * LC + 1] = gy; po[3 * LC + 2] = gz;
-            po[3 * LC + 3] = 0.; po[3 * LC + 4] = 0.; po[3 * LC + 5] = 0.;
-        }
+        lb_levels_out<LC>(J, PP, LU, LV, LW, (double)gxr, (double)gyr, (double)gzr);
         return;
     }
     if constexpr (GRID) {
@@ -852,6 +855,35 @@ template <int LC, int F8>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8)))
 void k_levels_grid32(SeaDev<float> S, LevelJob<float> J) {
     levels_body<float, LC, 0, F8>(S, J);
+}
+
+// Linear-term-basis mode split over component segments: the sum of the
+// segments' basis-sum records (segment order), then the per-level evaluation
+// once per column.
+template <int LC>
+__global__ __launch_bounds__(256) void k_levels_reduce_lb(SeaDev<float> S, LevelJob<float> J) {
+    const int64_t tid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t nthr = ((int64_t)J.nslot * J.ncol + 63) / 64 * 64 * ((J.g.nl + LC - 1) / LC);
+    if (tid >= nthr) return;
+    const LevPos<float, LC, true> PP = lev_pos<float, LC, true>(S, J, tid, (uint32_t)(EV_LIN | EV_SEC | EV_TAYLOR));
+    if (!PP.live) return;
+    float LU[16], LV[16], LW[16], g[4] = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int q = 0; q < 16; ++q) LU[q] = LV[q] = LW[q] = 0.f;
+    for (int sgi = 0; sgi < J.nseg; ++sgi) {
+        const float4* r = reinterpret_cast<const float4*>(reinterpret_cast<const float*>(J.partial) +
+                                                          ((int64_t)sgi * nthr + tid) * kLbRec);
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const float4 a = r[k], b = r[4 + k], c = r[8 + k];
+            LU[4 * k] += a.x; LU[4 * k + 1] += a.y; LU[4 * k + 2] += a.z; LU[4 * k + 3] += a.w;
+            LV[4 * k] += b.x; LV[4 * k + 1] += b.y; LV[4 * k + 2] += b.z; LV[4 * k + 3] += b.w;
+            LW[4 * k] += c.x; LW[4 * k + 1] += c.y; LW[4 * k + 2] += c.z; LW[4 * k + 3] += c.w;
+        }
+        const float4 d = r[12];
+        g[0] += d.x; g[1] += d.y; g[2] += d.z;
+    }
+    lb_levels_out<LC>(J, PP, LU, LV, LW, (double)g[0], (double)g[1], (double)g[2]);
 }
 
 // sum of the segment partials of k_levels
@@ -1342,9 +1374,13 @@ static hipError_t launch_levels_t(const SeaDev<R>& S, const LevelJob<R>& J, hipS
     const int64_t n = level_threads(J.nslot, J.ncol, J.g.nl, LC, MODE == 0);
     if (J.nslot * J.ncol <= 0) return hipSuccess;
     if constexpr (MODE == 0 && sizeof(R) == 4 && (LC == 8 || LC == 16)) {
-        if (J.qf && J.lin_c && LC == 16)
+        if (J.qf && J.lin_c && LC == 16) {
             hipLaunchKernelGGL((k_levels_grid32<LC, 2>), dim3(nblk(n), J.nseg), dim3(256), 0, st, S, J);
-        else if (J.qf)
+            const hipError_t e = hipGetLastError();
+            if (e != hipSuccess || !J.partial) return e;
+            hipLaunchKernelGGL((k_levels_reduce_lb<LC>), dim3(nblk(n)), dim3(256), 0, st, S, J);
+            return hipGetLastError();
+        } else if (J.qf)
             hipLaunchKernelGGL((k_levels_grid32<LC, 1>), dim3(nblk(n), J.nseg), dim3(256), 0, st, S, J);
         else
             hipLaunchKernelGGL((k_levels_grid32<LC, 0>), dim3(nblk(n), J.nseg), dim3(256), 0, st, S, J);
