@@ -181,6 +181,35 @@ def test_bf16_side_stream_fill_equals_serial_fill(eng, monkeypatch):
         assert np.array_equal(a, b)
 
 
+@pytest.mark.gpu
+def test_segmented_level_pass_matches_one_segment(eng, monkeypatch):
+    """A bf16 fill of 64 x 64 columns x 16 levels x 4 time levels splits its
+    column and level passes over component segments (partial records reduced
+    in segment order; the level pass's records carry the linear-term basis
+    sums, evaluated at the levels once per column by the reduction).  With
+    CFDWM_NSEG=1 the same fill runs in one segment.  The two agree to the FP32
+    rounding of the partial sums."""
+    from cfdwavemaker_amd.seastate import config_sea
+    s = config_sea(3).as_dict()
+    dx = 1000.0 / 255
+    n, nl = 64, 16
+    out = {}
+    for nseg in ("1", "default"):
+        if nseg == "1":
+            monkeypatch.setenv("CFDWM_NSEG", "1")
+        else:
+            monkeypatch.delenv("CFDWM_NSEG", raising=False)
+        g = eng.Sea(s, precision=eng.BF16, with_direct=False).grid((0.0, (n - 1) * dx, 0.0, (n - 1) * dx), n, n, nl,
+                                                                    0.0, 0.5, 0.7, 1.5, 100.0)
+        assert g.qform_info() is not None
+        out[nseg] = {f: g.export(f) for f in ("ETA", "UX", "UY", "UZ")}
+        del g
+    for f in ("ETA", "UX", "UY", "UZ"):
+        err = scaled_err(out["default"][f], out["1"][f])
+        print(f"segments vs one segment {f} {err:.2e}", flush=True)
+        assert err <= 1e-5, (f, err)
+
+
 @pytest.mark.parametrize("prec_name,lin_tol,bound,n", [("bf16", "1e-5", 1e-4, 256), ("fp32", "1e-6", 2e-5, 256),
                                                        ("bf16", "1e-5", 1e-4, 12)])
 def test_linear_term_bases_vs_per_level_profiles(eng, oracle, monkeypatch, prec_name, lin_tol, bound, n):
